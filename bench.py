@@ -1,0 +1,214 @@
+"""Benchmark of the photon gather hot path on one MI355X.
+
+Workload: the gather microbenchmark — 16M FP32 photons (70% uniform on the
+unit-cube faces, 30% in 64 Gaussian clusters on the floor), 4M queries on the
+faces, k = 64 nearest photons within r = 0.01, hash-grid cell = r. One step =
+build the photon hash grid from the photon block (count -> scan -> stable
+scatter) + the fused k-nearest gather and radiance estimate for all 4M
+queries. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the README of the harness contract in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-photon-mapping_amd"))
+
+METRIC = "kNN photon queries/s (k=64) + photons traced/s, 1 GPU; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
+    """Oracle (C++ restatement of the reference CPU path) on the host cores:
+    grid build once, then kNN + radiance on a bounded sample of the queries."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, 64))
+    t0 = time.perf_counter()
+    og = pyoracle.OracleGrid(ph, aabb, cell_size=r)
+    t_build = time.perf_counter() - t0
+    n = 20_000
+    sel = np.random.default_rng(0).choice(q.shape[1], n, replace=False)
+    t0 = time.perf_counter()
+    oi, od, oc = og.knn(q[:, sel], r, k, threads=threads)
+    pyoracle.radiance(oi, od, oc, og.photons, qn[:, sel], k)
+    dt = time.perf_counter() - t0
+    m = int(min(q.shape[1], max(n, n * (budget_s / max(dt, 1e-3)))))
+    if m > n:
+        sel = np.random.default_rng(1).choice(q.shape[1], m, replace=False)
+        t0 = time.perf_counter()
+        oi, od, oc = og.knn(q[:, sel], r, k, threads=threads)
+        pyoracle.radiance(oi, od, oc, og.photons, qn[:, sel], k)
+        dt = time.perf_counter() - t0
+        n = m
+    return {
+        "value": n / dt,
+        "unit": "queries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} of the 4M queries (seeded sample), kNN k={k} + radiance on the oracle grid; "
+                  f"oracle grid build of 16M photons took {t_build:.2f} s on 1 thread (not in value)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--photons", type=int, default=16_000_000)
+    ap.add_argument("--queries", type=int, default=4_000_000)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--radius", type=float, default=0.01)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--sort-queries", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the photon path "
+              f"runs on a single MI355X; multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import pm355
+    from pm355.workloads import gather_microbench
+
+    log("generating workload")
+    ph, q, qn, aabb = gather_microbench(seed=args.seed, n_photons=args.photons, n_queries=args.queries)
+    r, k = args.radius, args.k
+    ctx = pm355.Context(0)
+    dev = ctx.device
+    d_ph = torch.from_numpy(ph).to(dev)
+    d_q = torch.from_numpy(q).to(dev)
+    d_n = torch.from_numpy(qn).to(dev)
+    nq = q.shape[1]
+    flags = pm355.KNN_SORT_QUERIES if args.sort_queries else 0
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        g = ctx.grid_build(d_ph, aabb, cell_size=r)
+        rgb, cnt = ctx.gather_radiance(g, d_q, d_n, r, k, flags=flags, want_count=True)
+        return g, rgb, cnt
+
+    log("warmup")
+    for _ in range(args.warmup):
+        g, rgb, cnt = step()
+        g.close()
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grids = []
+    for _ in range(args.steps):
+        g, rgb, cnt = step()
+        grids.append(g)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for g in grids:
+        g.close()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # Component timing with HIP events on the stream the kernels run on.
+    g = ctx.grid_build(d_ph, aabb, cell_size=r)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    reps = max(3, args.steps)
+    ev[0].record(stream)
+    for _ in range(reps):
+        gg = ctx.grid_build(d_ph, aabb, cell_size=r)
+        gg.close()
+    ev[1].record(stream)
+    for _ in range(reps):
+        ctx.gather_radiance(g, d_q, d_n, r, k, flags=0)
+    ev[2].record(stream)
+    for _ in range(reps):
+        ctx.gather_radiance(g, d_q, d_n, r, k, flags=pm355.KNN_SORT_QUERIES)
+    ev[3].record(stream)
+    for _ in range(reps):
+        idx, dist, cnt2 = ctx.knn(g, d_q, r, k, flags=flags)
+    ev[4].record(stream)
+    torch.cuda.synchronize()
+    build_ms = ev[0].elapsed_time(ev[1]) / reps
+    gather_ms = ev[1].elapsed_time(ev[2]) / reps       # k_knn alone (no query sort)
+    gather_sorted_ms = ev[2].elapsed_time(ev[3]) / reps
+    knn_only_ms = ev[3].elapsed_time(ev[4]) / reps
+
+    # Algorithmic bytes of one fused gather launch (k_knn):
+    # per query 12 (query) + 8*C_q (start+size) + 12*P_q (candidate positions)
+    # + 24*kept (power+dir of kept photons) + 12 (normal) + 12 (RGB out).
+    rgb, cnt = ctx.gather_radiance(g, d_q, d_n, r, k, flags=0, want_count=True)
+    torch.cuda.synchronize()
+    kept = int(cnt.to(torch.int64).sum().item())
+    cells, cands = ctx.box_stats(g, d_q, r)
+    alg_bytes = nq * (12 + 12 + 12) + 8 * cells + 12 * cands + 24 * kept
+    achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
+
+    value = nq / (ms_per_step * 1e-3)
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "gather microbenchmark: 16M photons (70% cube faces, 30% 64 floor clusters sigma=0.005), "
+                        "4M face queries, k=64, r=0.01, grid cell=r; step = grid build + fused kNN gather/radiance",
+            "photons": args.photons,
+            "queries": nq,
+            "k": k,
+            "radius": r,
+            "sort_queries": bool(args.sort_queries),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_knn (fused gather + radiance, queries in input order)",
+            "kernel_ms": gather_ms,
+            "alg_bytes": alg_bytes,
+        },
+        "components_ms": {
+            "grid_build": build_ms,
+            "gather_radiance": gather_ms,
+            "gather_radiance_sorted_queries": gather_sorted_ms,
+            "knn_only_with_index_outputs": knn_only_ms,
+        },
+        "gather_only_queries_per_s": nq / (min(gather_ms, gather_sorted_ms) * 1e-3),
+        "box_stats": {"cells": cells, "candidates": cands, "kept": kept},
+    }
+    g.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

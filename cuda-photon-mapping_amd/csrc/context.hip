@@ -1,0 +1,112 @@
+// Context, error reporting, scratch arena and small utility kernels.
+#include <cstdarg>
+#include <cstring>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+pm_status fail(pm_ctx* ctx, pm_status code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+pm_status hip_fail(pm_ctx* ctx, hipError_t e, const char* what) {
+    pm_status code = (e == hipErrorOutOfMemory) ? PM_ERR_NOMEM : PM_ERR_HIP;
+    return fail(ctx, code, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+pm_status scratch_reserve(pm_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->scratch_bytes) {
+        if (ctx->scratch) {
+            PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            PM_HIP(ctx, hipFree(ctx->scratch));
+            ctx->scratch = nullptr;
+            ctx->scratch_bytes = 0;
+        }
+        size_t want = bytes + bytes / 4 + (1u << 20);
+        hipError_t e = hipMalloc(&ctx->scratch, want);
+        if (e != hipSuccess) {
+            ctx->scratch = nullptr;
+            return hip_fail(ctx, e, "hipMalloc(scratch)");
+        }
+        ctx->scratch_bytes = want;
+    }
+    *out = ctx->scratch;
+    return PM_OK;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = v;
+}
+
+pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count) {
+    if (count == 0) return PM_OK;
+    uint32_t blocks = (uint32_t)std::min<size_t>((count + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, ctx->stream, p, value, count);
+    PM_CHECK_LAUNCH(ctx, "k_fill_u32");
+    return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" {
+
+const char* pm_version(void) { return "pm355 0.1 (gfx950)"; }
+
+pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
+    if (!out) return PM_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return PM_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PM_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PM_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PM_ERR_NO_DEVICE;
+    pm_ctx* c = new pm_ctx();
+    c->device = device;
+    c->stream = (hipStream_t)hip_stream;
+    c->cu_count = prop.multiProcessorCount;
+    if (hipMalloc(&c->scan_counter, 256) != hipSuccess || hipMemset(c->scan_counter, 0, 256) != hipSuccess) {
+        delete c;
+        return PM_ERR_NOMEM;
+    }
+    *out = c;
+    return PM_OK;
+}
+
+pm_status pm_ctx_destroy(pm_ctx* ctx) {
+    if (!ctx) return PM_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->scan_state) (void)hipFree(ctx->scan_state);
+    if (ctx->scan_counter) (void)hipFree(ctx->scan_counter);
+    delete ctx;
+    return PM_OK;
+}
+
+pm_status pm_ctx_set_stream(pm_ctx* ctx, void* hip_stream) {
+    if (!ctx) return PM_ERR_INVALID;
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream = (hipStream_t)hip_stream;
+    return PM_OK;
+}
+
+pm_status pm_ctx_sync(pm_ctx* ctx) {
+    if (!ctx) return PM_ERR_INVALID;
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PM_OK;
+}
+
+const char* pm_last_error(const pm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+}  // extern "C"

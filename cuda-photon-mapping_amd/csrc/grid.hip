@@ -1,0 +1,247 @@
+// Photon hash grid build: count -> wave64 scan -> stable scatter.
+// Replaces PhotonGrid::build (PhotonGrid.cuh:51-66; PhotonGrid.cu:41-218).
+#include <cmath>
+
+#include "pm_internal.hpp"
+
+namespace {
+
+using pmd::GridGeom;
+
+// Per-photon cell id (the sort key) and the per-cell histogram
+// (count_photons_per_cell, PhotonGrid.cu:58-68).
+__global__ __launch_bounds__(256) void k_grid_hash(const float* __restrict__ px, const float* __restrict__ py,
+                                                  const float* __restrict__ pz, uint32_t n, GridGeom G,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = pmd::cell_coord(px[i], G.mn[0], G.cell, G.g[0]);
+    int y = pmd::cell_coord(py[i], G.mn[1], G.cell, G.g[1]);
+    int z = pmd::cell_coord(pz[i], G.mn[2], G.cell, G.g[2]);
+    uint32_t c = pmd::cell_index(G, x, y, z);
+    keys[i] = c;
+    // Photons arrive in emission order, so a wave often holds runs of the
+    // same cell: one atomic per wave when all its lanes share a cell.
+    uint64_t same = __ballot(__shfl(c, 0, 64) == c);
+    if (same == __ballot(true)) {
+        if (__lane_id() == (uint32_t)__builtin_ctzll(same)) atomicAdd(&counts[c], (uint32_t)__popcll(same));
+    } else {
+        atomicAdd(&counts[c], 1u);
+    }
+}
+
+// analyze_density_kernel's max (PhotonGrid.cu:70-90), as a reduction.
+__global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, a[i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// assign_photons_to_cells (PhotonGrid.cu:152-174) with the slot taken from
+// the stable sort instead of an atomic: in-cell order = input order.
+__global__ __launch_bounds__(256) void k_grid_scatter(const float* __restrict__ in9, uint32_t n,
+                                                     const uint32_t* __restrict__ order, float* __restrict__ out9,
+                                                     uint32_t* __restrict__ perm) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t src = order[j];
+    perm[j] = src;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) out9[(size_t)p * n + j] = in9[(size_t)p * n + src];
+}
+
+struct HostDims {
+    float cell;
+    int g[3];
+    uint32_t n_cells;
+};
+
+// compute_grid_and_cell_size (PhotonGrid.cu:41-56), host FP32.
+float reference_cell(uint32_t n, const float aabb[6], int desired) {
+    float sx = std::fabs(aabb[3] - aabb[0]);
+    float sy = std::fabs(aabb[4] - aabb[1]);
+    float sz = std::fabs(aabb[5] - aabb[2]);
+    int N = ((int)n + desired - 1) / desired;
+    float volume = sx * sy * sz;
+    return std::cbrt(volume / (float)N);
+}
+
+bool make_dims(const float aabb[6], float cell, int layout, HostDims& d) {
+    d.cell = cell;
+    uint64_t total = 1;
+    int mx = 1;
+    for (int a = 0; a < 3; ++a) {
+        float size = std::fabs(aabb[a + 3] - aabb[a]);
+        float gf = std::ceil(size / cell);
+        if (!(gf < 1048576.f)) return false;
+        int g = (int)gf;
+        d.g[a] = g < 1 ? 1 : g;
+        total *= (uint64_t)d.g[a];
+        mx = std::max(mx, d.g[a]);
+    }
+    if (layout == PM_LAYOUT_MORTON) {
+        if (mx > 1024) return false;
+        uint64_t side = 1;
+        while ((int)side < mx) side <<= 1;
+        total = side * side * side;
+    }
+    if (total > (1ull << 30)) return false;
+    d.n_cells = (uint32_t)total;
+    return true;
+}
+
+int bits_for(uint32_t n_cells) {
+    int b = 1;
+    while (b < 32 && (1ull << b) < (uint64_t)n_cells) ++b;
+    return b;
+}
+
+GridGeom geom_of(const float aabb[6], const HostDims& d, int layout) {
+    GridGeom G;
+    for (int a = 0; a < 3; ++a) {
+        G.mn[a] = aabb[a];
+        G.g[a] = d.g[a];
+    }
+    G.cell = d.cell;
+    G.layout = layout;
+    return G;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
+                                   float cell_size, int32_t desired, int32_t layout, pm_grid** out) {
+    if (!ctx || !out) return PM_ERR_INVALID;
+    *out = nullptr;
+    if (!aabb || (n > 0 && !d_photons9)) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: null argument");
+    if (layout != PM_LAYOUT_LINEAR && layout != PM_LAYOUT_MORTON)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: bad layout %d", layout);
+    if (n == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: no photons to build grid");
+    if (n > 0x7FFFFFFFu) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: n too large");
+    if (!(cell_size > 0.f) && desired <= 0)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: need cell_size > 0 or desired_per_cell > 0");
+
+    const float* px = d_photons9;
+    const float* py = d_photons9 + n;
+    const float* pz = d_photons9 + 2 * (size_t)n;
+    const uint32_t blocks = pm::div_up(n, 256);
+
+    HostDims d;
+    float cell = cell_size > 0.f ? cell_size : reference_cell(n, aabb, desired);
+    if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
+
+    // scratch: keys, sorted keys, sorted order, radix work, max counter
+    size_t rs = pm::radix_sort_scratch_bytes(n);
+    void* work = nullptr;
+    pm_status st;
+    size_t need_counts = (size_t)d.n_cells * 4;
+    if (!(cell_size > 0.f)) {
+        // analyze_density (PhotonGrid.cu:92-141): count once at the reference
+        // size and shrink the cells when the fullest one exceeds 5 * desired.
+        HostDims d2;
+        float cell2 = reference_cell(n, aabb, std::max(1, desired / 5));
+        if (make_dims(aabb, cell2, layout, d2)) need_counts = std::max(need_counts, (size_t)d2.n_cells * 4);
+    }
+    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, need_counts});
+    if ((st = pm::scratch_reserve(ctx, total, &work)) != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    uint32_t* keys = cv.take<uint32_t>(n);
+    uint32_t* skeys = cv.take<uint32_t>(n);
+    uint32_t* order = cv.take<uint32_t>(n);
+    void* rwork = cv.take<char>(rs);
+    uint32_t* dmax = cv.take<uint32_t>(64);
+    uint32_t* tmp_counts = cv.take<uint32_t>(need_counts / 4);
+
+    if (!(cell_size > 0.f)) {
+        GridGeom G = geom_of(aabb, d, layout);
+        if ((st = pm::fill_u32(ctx, tmp_counts, 0u, d.n_cells)) != PM_OK) return st;
+        if ((st = pm::fill_u32(ctx, dmax, 0u, 1)) != PM_OK) return st;
+        hipLaunchKernelGGL(k_grid_hash, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys, tmp_counts);
+        PM_CHECK_LAUNCH(ctx, "k_grid_hash");
+        hipLaunchKernelGGL(k_max_u32, dim3(std::min<uint32_t>(pm::div_up(d.n_cells, 256), 4096)), dim3(256), 0,
+                           ctx->stream, tmp_counts, d.n_cells, dmax);
+        PM_CHECK_LAUNCH(ctx, "k_max_u32");
+        uint32_t mx = 0;
+        PM_HIP(ctx, hipMemcpyAsync(&mx, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (mx > 5u * (uint32_t)desired) {
+            cell = reference_cell(n, aabb, desired / 5);
+            if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
+                return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
+        }
+    }
+
+    pm_grid* g = new pm_grid();
+    size_t bytes = pm::carve_size({(size_t)d.n_cells * 4, (size_t)d.n_cells * 4, (size_t)n * 36, (size_t)n * 4});
+    hipError_t e = hipMalloc(&g->mem, bytes);
+    if (e != hipSuccess) {
+        delete g;
+        return pm::hip_fail(ctx, e, "hipMalloc(grid)");
+    }
+    pm::Carve gc{static_cast<char*>(g->mem)};
+    uint32_t* starts = gc.take<uint32_t>(d.n_cells);
+    uint32_t* sizes = gc.take<uint32_t>(d.n_cells);
+    float* photons = gc.take<float>((size_t)n * 9);
+    uint32_t* perm = gc.take<uint32_t>(n);
+
+    GridGeom G = geom_of(aabb, d, layout);
+    auto bail = [&](pm_status s) {
+        (void)hipFree(g->mem);
+        delete g;
+        return s;
+    };
+    if ((st = pm::fill_u32(ctx, sizes, 0u, d.n_cells)) != PM_OK) return bail(st);
+    hipLaunchKernelGGL(k_grid_hash, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys, sizes);
+    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_hash launch"));
+    if ((st = pm::scan_u32(ctx, sizes, starts, d.n_cells, false, nullptr)) != PM_OK) return bail(st);
+    if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(d.n_cells), rwork)) != PM_OK)
+        return bail(st);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, order, photons, perm);
+    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
+
+    pm_grid_desc& D = g->desc;
+    for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
+    D.cell_size = d.cell;
+    for (int a = 0; a < 3; ++a) D.dims[a] = d.g[a];
+    D.layout = layout;
+    D.n_cells = d.n_cells;
+    D.n_photons = n;
+    D.d_cell_starts = starts;
+    D.d_cell_sizes = sizes;
+    D.d_photons = photons;
+    D.d_perm = perm;
+    *out = g;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out) {
+    if (!grid || !out) return PM_ERR_INVALID;
+    *out = grid->desc;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint32_t* h_starts, uint32_t* h_sizes,
+                                          float* h_photons9, uint32_t* h_perm) {
+    if (!ctx || !grid) return PM_ERR_INVALID;
+    const pm_grid_desc& D = grid->desc;
+    if (h_starts)
+        PM_HIP(ctx, hipMemcpyAsync(h_starts, D.d_cell_starts, (size_t)D.n_cells * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_sizes)
+        PM_HIP(ctx, hipMemcpyAsync(h_sizes, D.d_cell_sizes, (size_t)D.n_cells * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_photons9)
+        PM_HIP(ctx, hipMemcpyAsync(h_photons9, D.d_photons, (size_t)D.n_photons * 36, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_perm)
+        PM_HIP(ctx, hipMemcpyAsync(h_perm, D.d_perm, (size_t)D.n_photons * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PM_OK;
+}
+
+extern "C" pm_status pm_grid_destroy(pm_grid* grid) {
+    if (!grid) return PM_OK;
+    if (grid->mem) (void)hipFree(grid->mem);
+    delete grid;
+    return PM_OK;
+}

@@ -1,0 +1,623 @@
+// k-nearest-photon gather and radiance estimate on the hash grid.
+//
+// Replaces PhotonGrid::find_nearests + PhotonMaxHeap (PhotonGrid.cu:222-284,
+// PhotonMaxHeap.cuh:90-107) and Raytracer::get_color_from_photon_grid
+// (Raytracer.cuh:110-138). One wave64 serves one query:
+//   1. the candidate box (query cell +- ceil(r/cell), clamped) is split into
+//      contiguous photon runs (one per (y,z) row of cells in the linear
+//      layout, one per cell in the Morton layout); lane s loads run s and a
+//      wave scan turns the run lengths into a flat candidate index space;
+//   2. pass 1 streams the candidates 64 at a time (coalesced SoA loads),
+//      computes d2 = (dx*dx + dy*dy) + dz*dz exactly as the reference host
+//      does, and histograms the in-radius d2 into 64 bins given by the top
+//      float bits (two bins per octave below the radius threshold) in LDS;
+//   3. a wave scan of the histogram gives the smallest bin edge that keeps
+//      at least k photons (rarely, a refinement pass narrows a crowded bin);
+//   4. pass 2 compacts the keys (d2 bits << 32 | grid slot) below that edge
+//      into an LDS buffer with ballot + mbcnt;
+//   5. a register bitonic sort across the wave orders them; the first k are
+//      the answer — the exact k smallest by (d2, slot);
+//   6. optionally the radiance estimate runs on those k in the same wave.
+#include <cmath>
+#include <cstring>
+
+#include "pm_internal.hpp"
+
+namespace {
+
+using pmd::GridGeom;
+
+constexpr int kWaves = 4;    // queries per 256-thread workgroup
+constexpr int kCap = 512;    // compaction buffer (keys) per wave
+constexpr int kMaxSeg = 64;  // photon runs handled per batch
+
+struct KnnParams {
+    GridGeom G;
+    const uint32_t* starts;
+    const uint32_t* sizes;
+    const float* px;
+    const float* py;
+    const float* pz;
+    const float* power;  // 3 planes of n
+    const float* dir;    // 3 planes of n
+    uint32_t n;
+    const float* qx;
+    const float* qy;
+    const float* qz;
+    const float* nrm;  // 3 planes of nq (radiance only)
+    uint32_t nq;
+    const uint32_t* order;  // optional processing order
+    float r;
+    float T;      // largest d2 with sqrtf(d2) <= r
+    uint32_t bT;  // bits(T) >> 22
+    int rc;
+    int k;
+    uint32_t* out_idx;
+    float* out_dist;
+    uint32_t* out_count;
+    float* rgb;  // 3 planes of nq (radiance only)
+    int remap;
+};
+
+struct WaveLds {
+    uint32_t seg_begin[kMaxSeg];
+    uint32_t seg_off[kMaxSeg + 1];
+    uint32_t hist[64];
+    uint64_t buf[kCap];
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Squared distance in the reference's host order, never contracted to FMA.
+__device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
+    float dx = px - qx, dy = py - qy, dz = pz - qz;
+    float xx = __fmul_rn(dx, dx), yy = __fmul_rn(dy, dy), zz = __fmul_rn(dz, dz);
+    return __fadd_rn(__fadd_rn(xx, yy), zz);
+}
+
+struct Box {
+    int lo[3], hi[3];
+};
+
+// Visit every candidate of the box, 64 per step, calling
+// f(valid, slot, d2) on all lanes (valid = lane holds a candidate).
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const KnnParams& P, WaveLds& L, uint32_t lane, const Box& B,
+                                                   float qx, float qy, float qz, F&& f) {
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
+    const int nseg = linear ? ny * nz : nx * ny * nz;
+    for (int sbase = 0; sbase < nseg; sbase += kMaxSeg) {
+        int s = sbase + (int)lane;
+        uint32_t b = 0, len = 0;
+        if (s < nseg) {
+            if (linear) {
+                int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
+                uint32_t c0 = pmd::cell_index(P.G, B.lo[0], y, z);
+                uint32_t c1 = c0 + (uint32_t)(nx - 1);
+                b = P.starts[c0];
+                len = P.starts[c1] + P.sizes[c1] - b;
+            } else {
+                int x = B.lo[0] + s % nx, t = s / nx;
+                int y = B.lo[1] + t % ny, z = B.lo[2] + t / ny;
+                uint32_t c = pmd::cell_index(P.G, x, y, z);
+                b = P.starts[c];
+                len = P.sizes[c];
+            }
+        }
+        uint32_t incl = wave_incl_scan(len, lane);
+        uint32_t M = __shfl(incl, 63, 64);
+        L.seg_begin[lane] = b;
+        L.seg_off[lane] = incl - len;
+        if (lane == 63) L.seg_off[kMaxSeg] = incl;
+        wave_sync();
+        int seg = 0;
+        for (uint32_t base = 0; base < M; base += 64) {
+            uint32_t fi = base + lane;
+            bool valid = fi < M;
+            uint32_t slot = 0;
+            float d2 = 0.f;
+            if (valid) {
+                while (L.seg_off[seg + 1] <= fi) ++seg;
+                slot = L.seg_begin[seg] + (fi - L.seg_off[seg]);
+                d2 = dist2(P.px[slot], P.py[slot], P.pz[slot], qx, qy, qz);
+            }
+            f(valid, slot, d2);
+        }
+        wave_sync();
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&v)[R], uint32_t lane) {
+    constexpr int N = 64 * R;
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= 64) {
+                const int rs = stride / 64;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    if ((j & rs) == 0) {
+                        const int jp = j | rs;
+                        const bool asc = (((j * 64) & size) == 0);
+                        uint64_t a = v[j], b = v[jp];
+                        bool sw = asc ? (a > b) : (a < b);
+                        v[j] = sw ? b : a;
+                        v[jp] = sw ? a : b;
+                    }
+                }
+            } else {
+                const bool lower = (lane & (uint32_t)stride) == 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const uint32_t e = (uint32_t)j * 64 + lane;
+                    const bool asc = (e & (uint32_t)size) == 0;
+                    uint64_t p = shfl_xor_u64(v[j], stride);
+                    bool take_min = (asc == lower);
+                    uint64_t mn = v[j] < p ? v[j] : p;
+                    uint64_t mx = v[j] < p ? p : v[j];
+                    v[j] = take_min ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
+// Radiance estimate of Raytracer.cuh:110-138 over the m kept photons held in
+// v (sorted); lanes own elements j*64 + lane.
+template <int R>
+__device__ __forceinline__ void radiance_epilogue(const KnnParams& P, const uint64_t (&v)[R], uint32_t m,
+                                                  uint32_t lane, uint32_t q) {
+    const float alpha = 1.818f, beta = 1.953f;
+    const float pi_f = 3.141592654f;
+    float res0 = 0.f, res1 = 0.f, res2 = 0.f;
+    float rmax = 0.f;
+    if (m > 0) {
+        const int jm = (int)((m - 1) / 64);
+        float dm = 0.f;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (j == jm) dm = __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32)));
+        rmax = __shfl(dm, (int)((m - 1) & 63), 64);
+        const float filter_r = __fmul_rn(rmax, rmax);
+        const float denom = 1.f - expf(-beta);
+        const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            uint32_t e = (uint32_t)j * 64 + lane;
+            if (e < m) {
+                uint32_t idx = (uint32_t)v[j];
+                float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+                float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
+                if (cosNL > 0.f) {
+                    float d = __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32)));
+                    float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                    float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                    res0 += P.power[idx] * w;
+                    res1 += P.power[P.n + idx] * w;
+                    res2 += P.power[2 * (size_t)P.n + idx] * w;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        res0 += __shfl_xor(res0, d, 64);
+        res1 += __shfl_xor(res1, d, 64);
+        res2 += __shfl_xor(res2, d, 64);
+    }
+    if (lane == 0) {
+        float coef = 1.f;
+        if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
+        P.rgb[q] = m > 0 ? res0 * coef : 0.f;
+        P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
+        P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds& L, uint32_t lane, uint32_t q,
+                                              uint32_t cnt, uint32_t m) {
+    uint64_t v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        uint32_t e = (uint32_t)j * 64 + lane;
+        v[j] = e < cnt ? L.buf[e] : ~0ull;
+    }
+    bitonic_sort<R>(v, lane);
+    if (P.out_idx) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            uint32_t e = (uint32_t)j * 64 + lane;
+            if (e < (uint32_t)P.k) {
+                bool keep = e < m;
+                P.out_idx[(size_t)q * P.k + e] = keep ? (uint32_t)v[j] : 0xFFFFFFFFu;
+                P.out_dist[(size_t)q * P.k + e] =
+                    keep ? __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32))) : __builtin_inff();
+            }
+        }
+    }
+    if (P.out_count && lane == 0) P.out_count[q] = m;
+    if (P.rgb) radiance_epilogue<R>(P, v, m, lane, q);
+}
+
+__global__ __launch_bounds__(256) void k_knn(KnnParams P) {
+    __shared__ WaveLds lds[kWaves];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t ql = blk * kWaves + wave;
+    if (ql >= P.nq) return;  // wave-uniform; no workgroup barriers below
+    const uint32_t q = P.order ? P.order[ql] : ql;
+    WaveLds& L = lds[wave];
+
+    const float qx = P.qx[q], qy = P.qy[q], qz = P.qz[q];
+    const float qv[3] = {qx, qy, qz};
+    Box B;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int c = pmd::cell_coord(qv[a], P.G.mn[a], P.G.cell, P.G.g[a]);
+        B.lo[a] = max(0, c - P.rc);
+        B.hi[a] = min(P.G.g[a] - 1, c + P.rc);
+    }
+
+    // Pass 1: histogram of in-radius d2 over 64 float-bit bins.
+    L.hist[lane] = 0;
+    wave_sync();
+    const float T = P.T;
+    const uint32_t bT = P.bT;
+    for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t, float d2) {
+        if (valid && d2 <= T) {
+            uint32_t h = bT - (__float_as_uint(d2) >> 22);
+            atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
+        }
+    });
+    wave_sync();
+    uint32_t c = L.hist[63 - lane];  // lane 0 = nearest bin
+    uint32_t cum = wave_incl_scan(c, lane);
+    const uint32_t found = __shfl(cum, 63, 64);
+    const uint32_t k = (uint32_t)P.k;
+    const uint32_t m = found < k ? found : k;
+
+    uint64_t khi = ~0ull, klo = 0;
+    uint32_t below = 0, A = found;
+    if (found > k) {
+        uint64_t ge = __ballot(cum >= k);
+        int ls = __builtin_ctzll(ge);
+        uint32_t hstar = 63u - (uint32_t)ls;
+        A = __shfl(cum, ls, 64);
+        uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
+        khi = (uint64_t)(bT - hstar + 1u) << 54;
+        if (hstar < 63u) {
+            klo = (uint64_t)(bT - hstar) << 54;
+            below = ls > 0 ? prev : 0u;
+        }
+        // Rare: a crowded bin. Split [klo, khi) into 64 sub-ranges until the
+        // kept set fits the buffer (terminates: keys are unique).
+        while (A > (uint32_t)kCap) {
+            uint64_t width = khi - klo;
+            int s = width <= 64 ? 0 : 64 - __builtin_clzll(width - 1) - 6;
+            L.hist[lane] = 0;
+            wave_sync();
+            const uint64_t lo_k = klo, hi_k = khi;
+            for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+                if (valid && d2 <= T) {
+                    uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | slot;
+                    if (key >= lo_k && key < hi_k) atomicAdd(&L.hist[(uint32_t)((key - lo_k) >> s)], 1u);
+                }
+            });
+            wave_sync();
+            uint32_t cc = L.hist[lane];
+            uint32_t cu = wave_incl_scan(cc, lane);
+            uint32_t need = k - below;
+            uint64_t g2 = __ballot(cu >= need);
+            int js = __builtin_ctzll(g2);
+            uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
+            uint32_t at = __shfl(cu, js, 64);
+            uint64_t nlo = klo + ((uint64_t)js << s);
+            uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
+            if (nhi > khi) nhi = khi;
+            A = below + at;
+            below = below + (js > 0 ? pv : 0u);
+            klo = nlo;
+            khi = nhi;
+        }
+    }
+
+    // Pass 2: compact keys below khi into the LDS buffer.
+    uint32_t cnt = 0;
+    const uint64_t kh = khi;
+    for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+        uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | slot;
+        bool acc = valid && d2 <= T && key < kh;
+        uint64_t bal = __ballot(acc);
+        if (acc) {
+            uint32_t pos = cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            L.buf[pos] = key;
+        }
+        cnt += (uint32_t)__popcll(bal);
+    });
+    wave_sync();
+
+    if (cnt <= 64)
+        sort_and_emit<1>(P, L, lane, q, cnt, m);
+    else if (cnt <= 128)
+        sort_and_emit<2>(P, L, lane, q, cnt, m);
+    else if (cnt <= 256)
+        sort_and_emit<4>(P, L, lane, q, cnt, m);
+    else
+        sort_and_emit<8>(P, L, lane, q, cnt, m);
+}
+
+// Radiance from a stored kNN result (pm_radiance): one wave per query.
+__global__ __launch_bounds__(256) void k_radiance(KnnParams P) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t q = blockIdx.x * kWaves + wave;
+    if (q >= P.nq) return;
+    const uint32_t m = min(P.out_count[q], (uint32_t)P.k);
+    const float alpha = 1.818f, beta = 1.953f;
+    const float pi_f = 3.141592654f;
+    float res0 = 0.f, res1 = 0.f, res2 = 0.f;
+    float rmax = m > 0 ? P.out_dist[(size_t)q * P.k + (m - 1)] : 0.f;
+    if (m > 0) {
+        const float filter_r = __fmul_rn(rmax, rmax);
+        const float denom = 1.f - expf(-beta);
+        const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
+        for (uint32_t e = lane; e < m; e += 64) {
+            uint32_t idx = P.out_idx[(size_t)q * P.k + e];
+            float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+            float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
+            if (cosNL > 0.f) {
+                float d = P.out_dist[(size_t)q * P.k + e];
+                float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                res0 += P.power[idx] * w;
+                res1 += P.power[P.n + idx] * w;
+                res2 += P.power[2 * (size_t)P.n + idx] * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        res0 += __shfl_xor(res0, d, 64);
+        res1 += __shfl_xor(res1, d, 64);
+        res2 += __shfl_xor(res2, d, 64);
+    }
+    if (lane == 0) {
+        float coef = m > 0 ? __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f)) : 0.f;
+        P.rgb[q] = res0 * coef;
+        P.rgb[P.nq + q] = res1 * coef;
+        P.rgb[2 * (size_t)P.nq + q] = res2 * coef;
+    }
+}
+
+// Cell key of each query for the optional cell-ordered processing.
+__global__ __launch_bounds__(256) void k_query_keys(GridGeom G, const float* __restrict__ qx,
+                                                   const float* __restrict__ qy, const float* __restrict__ qz,
+                                                   uint32_t nq, uint32_t* __restrict__ keys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    int x = pmd::cell_coord(qx[i], G.mn[0], G.cell, G.g[0]);
+    int y = pmd::cell_coord(qy[i], G.mn[1], G.cell, G.g[1]);
+    int z = pmd::cell_coord(qz[i], G.mn[2], G.cell, G.g[2]);
+    keys[i] = pmd::cell_index(G, x, y, z);
+}
+
+// Sum over queries of C_q (cells in the box) and P_q (photons in them).
+__global__ __launch_bounds__(256) void k_box_stats(GridGeom G, const uint32_t* __restrict__ starts,
+                                                  const uint32_t* __restrict__ sizes, const float* __restrict__ qx,
+                                                  const float* __restrict__ qy, const float* __restrict__ qz,
+                                                  uint32_t nq, int rc, unsigned long long* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cells = 0, cands = 0;
+    if (i < nq) {
+        const float qv[3] = {qx[i], qy[i], qz[i]};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            int c = pmd::cell_coord(qv[a], G.mn[a], G.cell, G.g[a]);
+            lo[a] = max(0, c - rc);
+            hi[a] = min(G.g[a] - 1, c + rc);
+        }
+        cells = (unsigned long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y) {
+                if (G.layout == PM_LAYOUT_LINEAR) {
+                    uint32_t c0 = pmd::cell_index(G, lo[0], y, z), c1 = c0 + (uint32_t)(hi[0] - lo[0]);
+                    cands += starts[c1] + sizes[c1] - starts[c0];
+                } else {
+                    for (int x = lo[0]; x <= hi[0]; ++x) cands += sizes[pmd::cell_index(G, x, y, z)];
+                }
+            }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        cells += __shfl_xor(cells, d, 64);
+        cands += __shfl_xor(cands, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], cells);
+        atomicAdd(&out[1], cands);
+    }
+}
+
+__global__ void k_zero2(unsigned long long* p) {
+    p[0] = 0;
+    p[1] = 0;
+}
+
+float radius_threshold(float r) {
+    float t = r * r;
+    while (std::sqrt(t) > r) t = std::nextafter(t, 0.f);
+    for (;;) {
+        float u = std::nextafter(t, INFINITY);
+        if (std::sqrt(u) <= r)
+            t = u;
+        else
+            break;
+    }
+    return t;
+}
+
+pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq, float radius, int32_t k,
+                uint32_t flags, KnnParams& P, const char* who) {
+    if (!grid) return pm::fail(ctx, PM_ERR_INVALID, "%s: null grid", who);
+    if (nq > 0 && !d_q) return pm::fail(ctx, PM_ERR_INVALID, "%s: null queries", who);
+    if (!(radius >= 0.f) || !std::isfinite(radius)) return pm::fail(ctx, PM_ERR_INVALID, "%s: bad radius", who);
+    if (k < 1 || k > 256) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: k=%d outside [1, 256]", who, k);
+    const pm_grid_desc& D = grid->desc;
+    P = KnnParams{};
+    for (int a = 0; a < 3; ++a) {
+        P.G.mn[a] = D.aabb[a];
+        P.G.g[a] = D.dims[a];
+    }
+    P.G.cell = D.cell_size;
+    P.G.layout = D.layout;
+    P.starts = D.d_cell_starts;
+    P.sizes = D.d_cell_sizes;
+    P.n = D.n_photons;
+    P.px = D.d_photons;
+    P.py = D.d_photons + P.n;
+    P.pz = D.d_photons + 2 * (size_t)P.n;
+    P.power = D.d_photons + 3 * (size_t)P.n;
+    P.dir = D.d_photons + 6 * (size_t)P.n;
+    P.qx = d_q;
+    P.qy = d_q + nq;
+    P.qz = d_q + 2 * (size_t)nq;
+    P.nq = nq;
+    P.r = radius;
+    P.T = radius_threshold(radius);
+    uint32_t tb;
+    std::memcpy(&tb, &P.T, 4);
+    P.bT = tb >> 22;
+    // PhotonGrid.cu:266: radius_cells = (int)ceilf(radius / cell_size)
+    float rcf = std::ceil(radius / D.cell_size);
+    if (!(rcf < 4096.f)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: radius/cell ratio too large", who);
+    P.rc = (int)rcf;
+    P.k = k;
+    P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : 1;
+    return PM_OK;
+}
+
+pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags) {
+    if (P.nq == 0) return PM_OK;
+    if (flags & PM_KNN_SORT_QUERIES) {
+        size_t rs = pm::radix_sort_scratch_bytes(P.nq);
+        void* work = nullptr;
+        pm_status st = pm::scratch_reserve(ctx, pm::carve_size({(size_t)P.nq * 4, (size_t)P.nq * 4, (size_t)P.nq * 4, rs}),
+                                           &work);
+        if (st != PM_OK) return st;
+        pm::Carve cv{static_cast<char*>(work)};
+        uint32_t* keys = cv.take<uint32_t>(P.nq);
+        uint32_t* skeys = cv.take<uint32_t>(P.nq);
+        uint32_t* order = cv.take<uint32_t>(P.nq);
+        void* rw = cv.take<char>(rs);
+        hipLaunchKernelGGL(k_query_keys, dim3(pm::div_up(P.nq, 256)), dim3(256), 0, ctx->stream, P.G, P.qx, P.qy, P.qz,
+                           P.nq, keys);
+        PM_CHECK_LAUNCH(ctx, "k_query_keys");
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < (uint64_t)grid->desc.n_cells) ++bits;
+        if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, P.nq, bits, rw)) != PM_OK) return st;
+        P.order = order;
+    }
+    hipLaunchKernelGGL(k_knn, dim3(pm::div_up(P.nq, kWaves)), dim3(256), 0, ctx->stream, P);
+    PM_CHECK_LAUNCH(ctx, "k_knn");
+    return PM_OK;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_grid_knn(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq, float radius,
+                                 int32_t k, uint32_t flags, uint32_t* d_idx, float* d_dist, uint32_t* d_count) {
+    if (!ctx) return PM_ERR_INVALID;
+    KnnParams P;
+    pm_status st = setup(ctx, grid, d_queries3, nq, radius, k, flags, P, "pm_grid_knn");
+    if (st != PM_OK) return st;
+    if (nq > 0 && (!d_idx || !d_dist || !d_count)) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_knn: null output");
+    P.out_idx = d_idx;
+    P.out_dist = d_dist;
+    P.out_count = d_count;
+    return launch(ctx, grid, P, flags);
+}
+
+extern "C" pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3,
+                                             const float* d_normals3, uint32_t nq, float radius, int32_t k,
+                                             uint32_t flags, float* d_rgb3, uint32_t* d_count) {
+    if (!ctx) return PM_ERR_INVALID;
+    KnnParams P;
+    pm_status st = setup(ctx, grid, d_queries3, nq, radius, k, flags, P, "pm_grid_gather_radiance");
+    if (st != PM_OK) return st;
+    if (nq > 0 && (!d_normals3 || !d_rgb3))
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_gather_radiance: null normals/output");
+    P.nrm = d_normals3;
+    P.rgb = d_rgb3;
+    P.out_count = d_count;
+    return launch(ctx, grid, P, flags);
+}
+
+extern "C" pm_status pm_radiance(pm_ctx* ctx, const pm_grid* grid, const uint32_t* d_idx, const float* d_dist,
+                                 const uint32_t* d_count, const float* d_normals3, uint32_t nq, int32_t k,
+                                 float* d_rgb3) {
+    if (!ctx) return PM_ERR_INVALID;
+    if (!grid) return pm::fail(ctx, PM_ERR_INVALID, "pm_radiance: null grid");
+    if (k < 1) return pm::fail(ctx, PM_ERR_INVALID, "pm_radiance: k < 1");
+    if (nq == 0) return PM_OK;
+    if (!d_idx || !d_dist || !d_count || !d_normals3 || !d_rgb3)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_radiance: null buffer");
+    const pm_grid_desc& D = grid->desc;
+    KnnParams P{};
+    P.n = D.n_photons;
+    P.power = D.d_photons + 3 * (size_t)P.n;
+    P.dir = D.d_photons + 6 * (size_t)P.n;
+    P.nq = nq;
+    P.k = k;
+    P.out_idx = const_cast<uint32_t*>(d_idx);
+    P.out_dist = const_cast<float*>(d_dist);
+    P.out_count = const_cast<uint32_t*>(d_count);
+    P.nrm = d_normals3;
+    P.rgb = d_rgb3;
+    hipLaunchKernelGGL(k_radiance, dim3(pm::div_up(nq, kWaves)), dim3(256), 0, ctx->stream, P);
+    PM_CHECK_LAUNCH(ctx, "k_radiance");
+    return PM_OK;
+}
+
+extern "C" pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
+                                       float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands) {
+    if (!ctx) return PM_ERR_INVALID;
+    KnnParams P;
+    pm_status st = setup(ctx, grid, d_queries3, nq, radius, 1, 0, P, "pm_grid_box_stats");
+    if (st != PM_OK) return st;
+    void* work = nullptr;
+    if ((st = pm::scratch_reserve(ctx, 256, &work)) != PM_OK) return st;
+    unsigned long long* acc = static_cast<unsigned long long*>(work);
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, ctx->stream, acc);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_box_stats, dim3(pm::div_up(nq, 256)), dim3(256), 0, ctx->stream, P.G, P.starts, P.sizes,
+                           P.qx, P.qy, P.qz, nq, P.rc, acc);
+    PM_CHECK_LAUNCH(ctx, "k_box_stats");
+    unsigned long long h[2] = {0, 0};
+    PM_HIP(ctx, hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_sum_cells) *h_sum_cells = h[0];
+    if (h_sum_cands) *h_sum_cands = h[1];
+    return PM_OK;
+}

@@ -1,0 +1,143 @@
+// Internal declarations shared by the pm355 HIP sources (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pm355.h"
+
+#define PM_WAVE 64
+
+struct pm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // Growable device scratch arena (carved per call; never shared between
+    // two calls in flight because one context drives one stream).
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // Decoupled look-back state for pm scans: tile flags carry an epoch so
+    // no clearing pass is needed between launches.
+    uint64_t* scan_state = nullptr;
+    size_t scan_state_tiles = 0;
+    uint32_t* scan_counter = nullptr;  // ordered tile ids, reset by the last tile
+    uint32_t scan_epoch = 0;
+    int cu_count = 256;
+};
+
+struct pm_grid {
+    pm_grid_desc desc{};
+    void* mem = nullptr;  // one allocation backing starts/sizes/photons/perm
+};
+
+namespace pm {
+
+pm_status fail(pm_ctx* ctx, pm_status code, const char* fmt, ...);
+pm_status hip_fail(pm_ctx* ctx, hipError_t e, const char* what);
+
+#define PM_HIP(ctx, call)                                      \
+    do {                                                       \
+        hipError_t _e = (call);                                \
+        if (_e != hipSuccess) return pm::hip_fail(ctx, _e, #call); \
+    } while (0)
+
+#define PM_CHECK_LAUNCH(ctx, what)                               \
+    do {                                                         \
+        hipError_t _e = hipGetLastError();                       \
+        if (_e != hipSuccess) return pm::hip_fail(ctx, _e, what); \
+    } while (0)
+
+// Scratch arena: returns at least `bytes` of device memory valid until the
+// next scratch request on this context. Grows (after a stream sync) as needed.
+pm_status scratch_reserve(pm_ctx* ctx, size_t bytes, void** out);
+
+// Bump allocator over one reserved scratch block.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+static inline size_t carve_size(std::initializer_list<size_t> bytes) {
+    size_t t = 0;
+    for (size_t b : bytes) t = ((t + 255) & ~size_t(255)) + b;
+    return t + 256;
+}
+
+// Scan over u32 (exclusive or inclusive), optional u64 device total.
+pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total);
+
+// Stable LSD radix sort of (key, value) pairs over key bits [0, key_bits).
+// Sorted output ends in keys_out / vals_out. vals_in may be null (identity).
+pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                           uint32_t* vals_out, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
+                           uint32_t* hist, uint32_t* offs);
+size_t radix_sort_scratch_bytes(uint32_t n);
+pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                                uint32_t* vals_out, uint32_t n, int key_bits, void* work);
+
+pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
+
+static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace pm
+
+// ---------------------------------------------------------------------------
+// Device helpers shared by the kernels.
+// ---------------------------------------------------------------------------
+namespace pmd {
+
+struct GridGeom {
+    float mn[3];
+    float cell;
+    int g[3];
+    int layout;
+};
+
+// separated_hash_position (PhotonGrid.cu:10-23): i = (int)((p - min)/cell)
+// clamped to [0, g-1]. The float is range-checked before conversion so the
+// result is defined for every input (and equals the reference inside the
+// AABB). Division is IEEE correctly rounded (hipcc's default for FP32).
+__device__ __forceinline__ int cell_coord(float p, float mn, float cell, int g) {
+    float f = (p - mn) / cell;
+    if (!(f < (float)g)) return g - 1;
+    if (f < 1.0f) return 0;
+    int i = (int)f;
+    return i < g - 1 ? i : g - 1;
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3FF;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t cell_index(const GridGeom& G, int x, int y, int z) {
+    if (G.layout == PM_LAYOUT_MORTON) return spread3((uint32_t)x) | (spread3((uint32_t)y) << 1) | (spread3((uint32_t)z) << 2);
+    return (uint32_t)x + (uint32_t)y * (uint32_t)G.g[0] + (uint32_t)z * (uint32_t)G.g[0] * (uint32_t)G.g[1];
+}
+
+// Map hardware workgroup id to a logical tile so that each of the 8 XCDs
+// (which receive workgroups round-robin) works on one contiguous range of
+// tiles and keeps its neighbouring cells in its own L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+    const uint32_t X = 8;
+    uint32_t per = nb / X, rem = nb % X;
+    uint32_t x = bid % X, i = bid / X;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+}  // namespace pmd

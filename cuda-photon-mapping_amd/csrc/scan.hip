@@ -1,0 +1,203 @@
+// Single-pass u32 prefix scan: wave64 shuffle scan inside a 4096-element
+// tile, decoupled look-back across tiles (Merrill & Garland 2016). Replaces
+// the reference's 44-block Blelloch scan with its software grid barrier
+// (PrefixScanSum.cuh:12-279, CudaGridSynchronizer.cuh:14-27), which needs all
+// blocks co-resident; here a tile only waits on tiles that took a lower
+// ordered id, so any number of tiles may be in flight.
+#include "pm_internal.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;  // 4096
+
+constexpr uint64_t kStatusAgg = 1, kStatusIncl = 2;
+
+__device__ __forceinline__ uint64_t pack_state(uint32_t epoch, uint64_t status, uint32_t value) {
+    return ((uint64_t)epoch << 34) | (status << 32) | value;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+template <bool kInclusive>
+__global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        uint32_t n, uint64_t* __restrict__ state,
+                                                        uint32_t* __restrict__ counter, uint32_t epoch,
+                                                        uint32_t n_tiles, uint64_t* __restrict__ total) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kThreads / 64];
+    __shared__ uint32_t s_prefix;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) {
+        uint32_t id = atomicAdd(counter, 1u);
+        // Every tile has taken its id once the last id is handed out: reset
+        // the counter for the next launch on this context.
+        if (id == n_tiles - 1) atomicExch(counter, 0u);
+        s_tile = id;
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base = (size_t)tile * kTile + (size_t)t * kItems;
+
+    uint32_t v[kItems];
+    const bool full = base + kItems <= n;
+    if (full && ((reinterpret_cast<uintptr_t>(in) & 15) == 0)) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int j = 0; j < kItems / 4; ++j) {
+            uint4 q = p[j];
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) v[j] = (base + j < n) ? in[base + j] : 0u;
+    }
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) tsum += v[j];
+
+    uint32_t incl = wave_incl_scan(tsum, lane);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_excl = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        uint32_t s = s_wave[w];
+        if ((uint32_t)w < wave) wave_excl += s;
+        agg += s;
+    }
+
+    if (wave == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&state[0], pack_state(epoch, kStatusIncl, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], pack_state(epoch, kStatusAgg, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                int64_t idx = j - (int64_t)lane;
+                uint64_t w;
+                uint32_t status;
+                uint64_t incl_mask, inval_mask, upto;
+                for (;;) {
+                    if (idx >= 0) {
+                        w = __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        status = ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3) : 0u;
+                    } else {
+                        w = 0;
+                        status = (uint32_t)kStatusIncl;
+                    }
+                    incl_mask = __ballot(status == kStatusIncl);
+                    inval_mask = __ballot(status == 0);
+                    // lanes 0..first-inclusive must all be published
+                    int first = incl_mask ? __builtin_ctzll(incl_mask) : 63;
+                    upto = (first == 63) ? ~0ull : ((2ull << first) - 1);
+                    if ((inval_mask & upto) == 0) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                uint32_t contrib = ((upto >> lane) & 1ull) ? (uint32_t)w : 0u;
+                // wave reduction
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) contrib += __shfl_xor(contrib, d, 64);
+                prefix += contrib;
+                if (incl_mask & upto) break;
+                j -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], pack_state(epoch, kStatusIncl, prefix + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            s_prefix = prefix;
+            if (total && tile == n_tiles - 1) *total = (uint64_t)(uint32_t)(prefix + agg);
+        }
+    }
+    __syncthreads();
+
+    uint32_t run = s_prefix + wave_excl + (incl - tsum);
+    uint32_t o[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (kInclusive) {
+            run += v[j];
+            o[j] = run;
+        } else {
+            o[j] = run;
+            run += v[j];
+        }
+    }
+    if (full && ((reinterpret_cast<uintptr_t>(out) & 15) == 0)) {
+        uint4* p = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int j = 0; j < kItems / 4; ++j) p[j] = make_uint4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (base + j < n) out[base + j] = o[j];
+    }
+}
+
+__global__ void k_zero_u64(uint64_t* p) { *p = 0; }
+
+}  // namespace
+
+namespace pm {
+
+pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total) {
+    if (n == 0) {
+        if (total) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, total);
+            PM_CHECK_LAUNCH(ctx, "k_zero_u64");
+        }
+        return PM_OK;
+    }
+    uint32_t tiles = div_up(n, kTile);
+    if (tiles > ctx->scan_state_tiles) {
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->scan_state) PM_HIP(ctx, hipFree(ctx->scan_state));
+        ctx->scan_state = nullptr;
+        size_t want = (size_t)tiles + tiles / 2 + 64;
+        PM_HIP(ctx, hipMalloc(&ctx->scan_state, want * sizeof(uint64_t)));
+        PM_HIP(ctx, hipMemset(ctx->scan_state, 0, want * sizeof(uint64_t)));
+        ctx->scan_state_tiles = want;
+        ctx->scan_epoch = 0;
+    }
+    ctx->scan_epoch++;
+    if (ctx->scan_epoch >= (1u << 30)) {
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PM_HIP(ctx, hipMemset(ctx->scan_state, 0, ctx->scan_state_tiles * sizeof(uint64_t)));
+        ctx->scan_epoch = 1;
+    }
+    if (inclusive)
+        hipLaunchKernelGGL(k_scan_u32<true>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n,
+                           ctx->scan_state, ctx->scan_counter, ctx->scan_epoch, tiles, total);
+    else
+        hipLaunchKernelGGL(k_scan_u32<false>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n,
+                           ctx->scan_state, ctx->scan_counter, ctx->scan_epoch, tiles, total);
+    PM_CHECK_LAUNCH(ctx, "k_scan_u32");
+    return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" pm_status pm_scan_u32(pm_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t n, int32_t inclusive,
+                                 uint64_t* d_total) {
+    if (!ctx) return PM_ERR_INVALID;
+    if ((!d_in || !d_out) && n > 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_scan_u32: null buffer");
+    return pm::scan_u32(ctx, d_in, d_out, n, inclusive != 0, d_total);
+}

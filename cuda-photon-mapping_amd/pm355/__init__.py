@@ -1,0 +1,260 @@
+"""Python harness over the pm355 C ABI (include/pm355.h).
+
+This module is plumbing for tests and benchmarks: it loads the in-tree
+``libpm355.so`` with ctypes and passes torch device tensors' pointers to it.
+Every computation happens in the HIP library; there is no Python or CPU
+fallback, and loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_PKG_DIR), "libpm355.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_PKG_DIR)), "include", "pm355.h")
+
+PM_OK = 0
+PM_ERR_INVALID = -1
+PM_ERR_HIP = -2
+PM_ERR_NOMEM = -3
+PM_ERR_CAPACITY = -4
+PM_ERR_UNSUPPORTED = -5
+PM_ERR_NO_DEVICE = -6
+
+LAYOUT_LINEAR = 0
+LAYOUT_MORTON = 1
+KNN_SORT_QUERIES = 1
+KNN_NO_XCD_REMAP = 2
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I32 = C.c_int32
+_F32 = C.c_float
+
+
+class PMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pm355 error {code}: {msg}")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("aabb", C.c_float * 6),
+        ("cell_size", C.c_float),
+        ("dims", C.c_int32 * 3),
+        ("layout", C.c_int32),
+        ("n_cells", C.c_uint32),
+        ("n_photons", C.c_uint32),
+        ("d_cell_starts", C.c_void_p),
+        ("d_cell_sizes", C.c_void_p),
+        ("d_photons", C.c_void_p),
+        ("d_perm", C.c_void_p),
+    ]
+
+
+_SIGS = {
+    "pm_version": (C.c_char_p, []),
+    "pm_ctx_create": (_I32, [_I32, _P, C.POINTER(_P)]),
+    "pm_ctx_destroy": (_I32, [_P]),
+    "pm_ctx_set_stream": (_I32, [_P, _P]),
+    "pm_ctx_sync": (_I32, [_P]),
+    "pm_last_error": (C.c_char_p, [_P]),
+    "pm_scan_u32": (_I32, [_P, _P, _P, _U32, _I32, _P]),
+    "pm_grid_build": (_I32, [_P, _P, _U32, C.POINTER(_F32), _F32, _I32, _I32, C.POINTER(_P)]),
+    "pm_grid_describe": (_I32, [_P, C.POINTER(GridDesc)]),
+    "pm_grid_copy_to_host": (_I32, [_P, _P, _P, _P, _P, _P]),
+    "pm_grid_destroy": (_I32, [_P]),
+    "pm_grid_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _U32, _P, _P, _P]),
+    "pm_radiance": (_I32, [_P, _P, _P, _P, _P, _P, _U32, _I32, _P]),
+    "pm_grid_gather_radiance": (_I32, [_P, _P, _P, _P, _U32, _F32, _I32, _U32, _P, _P]),
+    "pm_grid_box_stats": (_I32, [_P, _P, _P, _U32, _F32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libpm355.so; raises if it is missing (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"libpm355.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols(header: str = HEADER_PATH) -> list[str]:
+    """Names of the functions declared in include/pm355.h."""
+    import re
+
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pm_[a-z0-9_]+)\s*\(", text)))
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def _f32_dev(a, dev):
+    import torch
+
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=torch.float32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+class Context:
+    """A pm_ctx bound to a torch device and that device's current stream."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        self.lib = load_library()
+        self.device = torch.device("cuda", device)
+        self.stream = torch.cuda.current_stream(self.device)
+        h = _P()
+        rc = self.lib.pm_ctx_create(device, _P(self.stream.cuda_stream), C.byref(h))
+        if rc != PM_OK:
+            raise PMError(rc, "pm_ctx_create failed (no gfx950 device?)")
+        self.h = h
+
+    def check(self, rc: int):
+        if rc != PM_OK:
+            raise PMError(rc, self.lib.pm_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self.lib.pm_ctx_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scan ---------------------------------------------------------------
+    def scan_u32(self, x, inclusive: bool = False, with_total: bool = False):
+        import torch
+
+        x = x.to(self.device).contiguous()
+        out = torch.empty_like(x)
+        tot = torch.zeros(1, dtype=torch.int64, device=self.device) if with_total else None
+        self.check(self.lib.pm_scan_u32(self.h, _ptr(x), _ptr(out), x.numel(), int(inclusive), _ptr(tot)))
+        return (out, tot) if with_total else out
+
+    # -- grid -----------------------------------------------------------------
+    def grid_build(self, photons9, aabb, cell_size: float = 0.0, desired: int = 1000,
+                   layout: int = LAYOUT_LINEAR) -> "Grid":
+        import torch
+
+        ph = _f32_dev(photons9, self.device)
+        assert ph.dim() == 2 and ph.shape[0] == 9, "photons9 must be [9, n]"
+        n = ph.shape[1]
+        ab = (C.c_float * 6)(*[float(v) for v in aabb])
+        g = _P()
+        self.check(self.lib.pm_grid_build(self.h, _ptr(ph), n, ab, float(cell_size), int(desired), int(layout),
+                                          C.byref(g)))
+        return Grid(self, g, keep=ph)
+
+    def knn(self, grid: "Grid", queries3, radius: float, k: int, flags: int = 0):
+        import torch
+
+        q = _f32_dev(queries3, self.device)
+        nq = q.shape[1]
+        idx = torch.empty((nq, k), dtype=torch.int32, device=self.device)
+        dist = torch.empty((nq, k), dtype=torch.float32, device=self.device)
+        cnt = torch.empty((nq,), dtype=torch.int32, device=self.device)
+        self.check(self.lib.pm_grid_knn(self.h, grid.h, _ptr(q), nq, float(radius), int(k), int(flags),
+                                        _ptr(idx), _ptr(dist), _ptr(cnt)))
+        return idx, dist, cnt
+
+    def radiance(self, grid: "Grid", idx, dist, cnt, normals3, k: int):
+        import torch
+
+        nrm = _f32_dev(normals3, self.device)
+        nq = nrm.shape[1]
+        rgb = torch.empty((3, nq), dtype=torch.float32, device=self.device)
+        self.check(self.lib.pm_radiance(self.h, grid.h, _ptr(idx), _ptr(dist), _ptr(cnt), _ptr(nrm), nq, int(k),
+                                        _ptr(rgb)))
+        return rgb
+
+    def gather_radiance(self, grid: "Grid", queries3, normals3, radius: float, k: int, flags: int = 0,
+                        want_count: bool = False):
+        import torch
+
+        q = _f32_dev(queries3, self.device)
+        nrm = _f32_dev(normals3, self.device)
+        nq = q.shape[1]
+        rgb = torch.empty((3, nq), dtype=torch.float32, device=self.device)
+        cnt = torch.empty((nq,), dtype=torch.int32, device=self.device) if want_count else None
+        self.check(self.lib.pm_grid_gather_radiance(self.h, grid.h, _ptr(q), _ptr(nrm), nq, float(radius), int(k),
+                                                    int(flags), _ptr(rgb), _ptr(cnt)))
+        return (rgb, cnt) if want_count else rgb
+
+    def box_stats(self, grid: "Grid", queries3, radius: float):
+        q = _f32_dev(queries3, self.device)
+        c = C.c_uint64()
+        p = C.c_uint64()
+        self.check(self.lib.pm_grid_box_stats(self.h, grid.h, _ptr(q), q.shape[1], float(radius), C.byref(c),
+                                              C.byref(p)))
+        return int(c.value), int(p.value)
+
+
+@dataclass
+class GridInfo:
+    aabb: tuple
+    cell_size: float
+    dims: tuple
+    layout: int
+    n_cells: int
+    n_photons: int
+
+
+class Grid:
+    def __init__(self, ctx: Context, h, keep=None):
+        self.ctx = ctx
+        self.h = h
+        self._keep = keep
+        d = GridDesc()
+        ctx.check(ctx.lib.pm_grid_describe(h, C.byref(d)))
+        self.desc = d
+        self.info = GridInfo(tuple(d.aabb), d.cell_size, tuple(d.dims), d.layout, d.n_cells, d.n_photons)
+
+    def to_host(self):
+        d = self.desc
+        starts = np.empty(d.n_cells, np.uint32)
+        sizes = np.empty(d.n_cells, np.uint32)
+        photons = np.empty((9, d.n_photons), np.float32)
+        perm = np.empty(d.n_photons, np.uint32)
+        self.ctx.check(self.ctx.lib.pm_grid_copy_to_host(
+            self.ctx.h, self.h, starts.ctypes.data, sizes.ctypes.data, photons.ctypes.data, perm.ctypes.data))
+        return starts, sizes, photons, perm
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.pm_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

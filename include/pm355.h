@@ -1,0 +1,132 @@
+/* pm355 — MI355X-native photon pipeline: C ABI.
+ *
+ * This is the drop-in boundary for the reference's GPU photon path
+ * (Alator-ux/Cuda-Photon-Mapping, directory CudaPhotonMapping/). The
+ * reference has no FFI; its entry points are C++ free functions and classes
+ * compiled into one executable. Each function below names the reference
+ * interface it replaces. All functions return a pm_status (0 = OK, negative
+ * = error class); pm_last_error() gives the message. Nothing here exits the
+ * process (the reference calls exit(99) on any CUDA error, CudaUtils.cu:6-14).
+ *
+ * Memory: pointers named d_* are device (HBM) pointers, h_* are host
+ * pointers. Inputs are borrowed and never freed by the callee (unlike
+ * PhotonGrid::build, which frees its input, PhotonGrid.cu:216). Handles
+ * (pm_grid, pm_kdtree, ...) own their device buffers and are released with
+ * their *_destroy call. All work is enqueued on the context's HIP stream;
+ * pm_ctx_sync() waits for it.
+ *
+ * Photon block layout ("photons9"): 9 planes of n FP32 values in the field
+ * order of cpm::Photon (Photon.cuh:7-10):
+ *   pos.x pos.y pos.z | power.r power.g power.b | inc_dir.x inc_dir.y inc_dir.z
+ * i.e. plane p of photon i is at photons9[p * n + i]. Query, normal and RGB
+ * blocks are 3 planes of nq values.
+ */
+#ifndef PM355_H
+#define PM355_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pm_status;
+enum {
+    PM_OK = 0,
+    PM_ERR_INVALID = -1,     /* bad argument */
+    PM_ERR_HIP = -2,         /* HIP runtime error */
+    PM_ERR_NOMEM = -3,       /* device allocation failed */
+    PM_ERR_CAPACITY = -4,    /* an output or internal capacity was exceeded */
+    PM_ERR_UNSUPPORTED = -5, /* parameter outside what this build supports */
+    PM_ERR_NO_DEVICE = -6    /* no usable gfx950 device */
+};
+
+enum { PM_LAYOUT_LINEAR = 0, PM_LAYOUT_MORTON = 1 };
+
+/* pm_grid_knn / pm_grid_gather_radiance flags */
+enum {
+    PM_KNN_SORT_QUERIES = 1u << 0, /* process queries in cell order (outputs stay in caller order) */
+    PM_KNN_NO_XCD_REMAP = 1u << 1  /* disable the XCD-aware block mapping */
+};
+
+typedef struct pm_ctx pm_ctx;
+typedef struct pm_grid pm_grid;
+typedef struct pm_kdtree pm_kdtree;
+
+/* ---- context (replaces the reference's global device state: GlobalParams
+ * __constant__ mirrors GlobalParams.cu:1-58, the heap pointers of
+ * set_photon_heap_parameters PhotonMaxHeap.cu:9-17, and checkCudaErrors
+ * Defines.cuh:36) ---------------------------------------------------------- */
+const char* pm_version(void);
+pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out);
+pm_status pm_ctx_destroy(pm_ctx* ctx);
+pm_status pm_ctx_set_stream(pm_ctx* ctx, void* hip_stream);
+pm_status pm_ctx_sync(pm_ctx* ctx);
+const char* pm_last_error(const pm_ctx* ctx);
+
+/* ---- prefix scan: prescan<uint, inclusive> (PrefixScanSum.cuh:12-148) and
+ * the dead cooperative_inclusive_prescan (CGPrefixScanSum.cu:65-101). Single
+ * pass, wave64 scan with decoupled look-back; no grid barrier. d_total
+ * (nullable, device u64) receives the sum of all inputs. */
+pm_status pm_scan_u32(pm_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t n, int32_t inclusive,
+                      uint64_t* d_total);
+
+/* ---- hash grid: PhotonGrid(int, AABB) + PhotonGrid::build
+ * (PhotonGrid.cuh:44-66, PhotonGrid.cu:41-218). cell_size > 0 is used as
+ * given; cell_size <= 0 applies the reference rule cbrt(V / ceil(n/desired))
+ * with its density re-check (PhotonGrid.cu:92-141). The build is count ->
+ * exclusive scan -> stable scatter: in-cell order is ascending input index,
+ * so the grid is deterministic (the reference's atomicAdd slot is not,
+ * PhotonGrid.cu:167). */
+typedef struct {
+    float aabb[6];       /* min xyz, max xyz */
+    float cell_size;
+    int32_t dims[3];
+    int32_t layout;      /* PM_LAYOUT_* */
+    uint32_t n_cells;    /* linear: gx*gy*gz; Morton: side^3 */
+    uint32_t n_photons;
+    const uint32_t* d_cell_starts; /* [n_cells] exclusive scan of sizes */
+    const uint32_t* d_cell_sizes;  /* [n_cells] photons per cell */
+    const float* d_photons;        /* photons9 in grid order */
+    const uint32_t* d_perm;        /* [n] grid slot -> input index */
+} pm_grid_desc;
+
+pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6], float cell_size,
+                        int32_t desired_per_cell, int32_t layout, pm_grid** out);
+pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out);
+/* PhotonGrid::copy_to_cpu (PhotonGrid.cuh:75-98). Any output may be NULL. */
+pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint32_t* h_starts, uint32_t* h_sizes,
+                               float* h_photons9, uint32_t* h_perm);
+pm_status pm_grid_destroy(pm_grid* grid);
+
+/* ---- k-nearest-photon gather: PhotonGrid::find_nearests +
+ * PhotonMaxHeap::push (PhotonGrid.cu:222-284, PhotonMaxHeap.cuh:90-107),
+ * batched over nq queries. Same candidate box (query cell +- ceil(r/cell),
+ * clamped) and radius test (sqrtf(d2) <= r) as the reference; the kept set is
+ * the k smallest by (d2, grid index) with d2 = (dx*dx + dy*dy) + dz*dz in
+ * FP32. Outputs are [nq][k] in ascending order: d_idx = grid slot (as the
+ * heap's PhotonMaxHeapItem::idx), d_dist = sqrtf(d2); slots past d_count[q]
+ * hold idx 0xFFFFFFFF and dist +inf. 1 <= k <= 256. */
+pm_status pm_grid_knn(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq, float radius,
+                      int32_t k, uint32_t flags, uint32_t* d_idx, float* d_dist, uint32_t* d_count);
+
+/* Radiance estimate: Raytracer::get_color_from_photon_grid
+ * (Raytracer.cuh:110-138) over the output of pm_grid_knn. d_rgb3 is 3 planes. */
+pm_status pm_radiance(pm_ctx* ctx, const pm_grid* grid, const uint32_t* d_idx, const float* d_dist,
+                      const uint32_t* d_count, const float* d_normals3, uint32_t nq, int32_t k, float* d_rgb3);
+
+/* Fused gather + radiance: the kNN selection of pm_grid_knn with the estimate
+ * of pm_radiance in its epilogue. d_count (nullable) receives the kept count. */
+pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3,
+                                  const float* d_normals3, uint32_t nq, float radius, int32_t k, uint32_t flags,
+                                  float* d_rgb3, uint32_t* d_count);
+
+/* Per-query candidate statistics of the gather box (cells visited C_q and
+ * photons in them P_q, summed), used for the algorithmic-byte count. */
+pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
+                            float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

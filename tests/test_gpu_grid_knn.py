@@ -1,0 +1,231 @@
+"""GPU parity: scan, grid build, kNN gather and radiance against the oracle.
+
+Bit-exact for integer/index outputs (scan, counts, starts, permutation, kNN
+index sets and distances); radiance within 1e-5 relative of the oracle's FP64
+recomputation over the same kept set.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pm355
+import pyoracle as O
+from conftest import GOLDEN
+from pm355.workloads import gather_microbench, read_photon_file
+
+pytestmark = pytest.mark.gpu
+
+RADIANCE_RTOL = 1e-5
+
+
+def _np(t, dtype=None):
+    a = t.detach().cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def test_scan_kat(ctx):
+    import torch
+
+    from test_oracle import scan_kat_input
+
+    a = scan_kat_input()
+    x = torch.from_numpy(a.view(np.int32)).cuda()
+    inc, tot = ctx.scan_u32(x, inclusive=True, with_total=True)
+    exc = ctx.scan_u32(x, inclusive=False)
+    ctx.sync()
+    ref_inc, ref_tot = O.scan_u32(a, inclusive=True)
+    ref_exc, _ = O.scan_u32(a, inclusive=False)
+    assert np.array_equal(_np(inc, np.uint32), ref_inc)
+    assert np.array_equal(_np(exc, np.uint32), ref_exc)
+    assert int(tot.item()) == ref_tot == 665_385
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 123_457, 3_000_001])
+def test_scan_sizes(ctx, n):
+    import torch
+
+    a = np.random.default_rng(n).integers(0, 50, n).astype(np.uint32)
+    x = torch.from_numpy(a.view(np.int32)).cuda()
+    out, tot = ctx.scan_u32(x, inclusive=False, with_total=True)
+    ctx.sync()
+    ref, ref_tot = O.scan_u32(a)
+    assert np.array_equal(_np(out, np.uint32), ref)
+    assert int(tot.item()) == ref_tot
+
+
+def _check_grid(ctx, ph, aabb, cell, desired=1000, layout=0):
+    g = ctx.grid_build(ph, aabb, cell_size=cell, desired=desired, layout=layout)
+    og = O.OracleGrid(ph, aabb, cell_size=cell, desired=desired, layout=layout)
+    starts, sizes, photons, perm = g.to_host()
+    assert g.info.cell_size == np.float32(og.cell)
+    assert tuple(g.info.dims) == tuple(og.dims)
+    assert g.info.n_cells == og.n_cells
+    assert np.array_equal(sizes, og.counts)
+    assert np.array_equal(starts, og.starts)
+    assert np.array_equal(perm, og.perm)
+    assert np.array_equal(photons.view(np.uint32), og.photons.view(np.uint32))
+    return g, og
+
+
+def test_grid_fixture_file(ctx):
+    ph = read_photon_file(os.path.join(GOLDEN, "global_map_test.bin"))
+    aabb = tuple(ph[0:3].min(1)) + tuple(ph[0:3].max(1))
+    for cell in (0.0, 0.25):
+        for layout in (0, 1):
+            _check_grid(ctx, ph, aabb, cell, desired=10, layout=layout)
+
+
+def test_grid_reference_sizing_and_resize(ctx):
+    rng = np.random.default_rng(1)
+    n = 200_000
+    ph = np.zeros((9, n), np.float32)
+    ph[0:3] = rng.random((3, n), dtype=np.float32)
+    ph[0:3, : n // 2] = 0.123  # one hot cell -> density re-size branch
+    _check_grid(ctx, ph, (0, 0, 0, 1, 1, 1), 0.0, desired=1000)
+    _check_grid(ctx, ph[:, n // 2:].copy(), (0, 0, 0, 1, 1, 1), 0.0, desired=1000)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_grid_synthetic(ctx, layout):
+    ph, q, qn, aabb = gather_microbench(seed=11, n_photons=500_000, n_queries=10, n_clusters=16)
+    _check_grid(ctx, ph, aabb, 0.01, layout=layout)
+    # photons outside the AABB are clamped into boundary cells
+    ph2 = ph.copy()
+    ph2[0, :1000] = -0.5
+    ph2[1, 1000:2000] = 7.0
+    _check_grid(ctx, ph2, aabb, 0.02, layout=layout)
+
+
+def _compare_knn(gi, gd, gc, oi, od, oc):
+    gi = _np(gi, np.uint32)
+    gd = _np(gd)
+    gc = _np(gc, np.uint32)
+    assert np.array_equal(gc, oc), f"counts differ at {np.nonzero(gc != oc)[0][:10]}"
+    bad = np.nonzero(np.any(gi != oi, axis=1))[0]
+    assert bad.size == 0, f"index sets differ for queries {bad[:10]}"
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+KNN_CASES = [
+    # cell, r, k, layout
+    (0.01, 0.01, 64, 0),
+    (0.01, 0.01, 64, 1),
+    (0.02, 0.02, 100, 0),
+    (0.05, 0.05, 50, 0),
+    (0.005, 0.01, 1, 0),
+    (0.02, 0.03, 256, 0),
+    (0.01, 0.025, 17, 1),
+]
+
+
+@pytest.mark.parametrize("cell,r,k,layout", KNN_CASES)
+def test_knn_parity(ctx, cell, r, k, layout):
+    ph, q, qn, aabb = gather_microbench(seed=5, n_photons=400_000, n_queries=6_000, n_clusters=16)
+    q[:, 0] = ph[0:3, -1]  # on a cluster photon
+    q[:, 1] = (1.4, 0.5, 0.5)  # outside the box: clamped cell, no photons in r
+    g = ctx.grid_build(ph, aabb, cell_size=cell, layout=layout)
+    og = O.OracleGrid(ph, aabb, cell_size=cell, layout=layout)
+    oi, od, oc = og.knn(q, r, k)
+    for flags in (0, pm355.KNN_SORT_QUERIES):
+        gi, gd, gc = ctx.knn(g, q, r, k, flags=flags)
+        ctx.sync()
+        _compare_knn(gi, gd, gc, oi, od, oc)
+    assert oc[1] == 0
+
+
+def test_knn_duplicates_force_refinement(ctx):
+    # 3000 photons at one point: the nearest bin holds more keys than the
+    # compaction buffer, so the selection refines down to index order.
+    rng = np.random.default_rng(9)
+    n = 50_000
+    ph = np.zeros((9, n), np.float32)
+    ph[0:3] = rng.random((3, n), dtype=np.float32)
+    ph[0:3, 1000:4000] = np.float32(0.5)
+    ph[0:3, 4000:9000] = (0.5 + rng.standard_normal((3, 5000)) * 1e-6).astype(np.float32)
+    q = np.array([[0.5, 0.5, 0.3, 0.5], [0.5, 0.5, 0.3, 0.5000001], [0.5, 0.5, 0.3, 0.5]], np.float32)
+    g = ctx.grid_build(ph, (0, 0, 0, 1, 1, 1), cell_size=0.05)
+    og = O.OracleGrid(ph, (0, 0, 0, 1, 1, 1), cell_size=0.05)
+    for k in (1, 64, 200, 256):
+        oi, od, oc = og.knn(q, 0.05, k)
+        gi, gd, gc = ctx.knn(g, q, 0.05, k)
+        ctx.sync()
+        _compare_knn(gi, gd, gc, oi, od, oc)
+
+
+def test_knn_matches_brute_force_small(ctx):
+    rng = np.random.default_rng(4)
+    ph = np.zeros((9, 3000), np.float32)
+    ph[0:3] = rng.random((3, 3000), dtype=np.float32)
+    q = rng.random((3, 500), dtype=np.float32)
+    g = ctx.grid_build(ph, (0, 0, 0, 1, 1, 1), cell_size=0.1)
+    _, _, gph, _ = g.to_host()
+    bi, bd, bc = O.knn_brute(gph[0:3], q, 0.15, 32)
+    gi, gd, gc = ctx.knn(g, q, 0.15, 32)
+    ctx.sync()
+    _compare_knn(gi, gd, gc, bi, bd, bc)
+
+
+@pytest.mark.parametrize("cell,r,k", [(0.01, 0.01, 64), (0.02, 0.02, 100), (0.05, 0.05, 50)])
+def test_radiance_parity(ctx, cell, r, k):
+    ph, q, qn, aabb = gather_microbench(seed=8, n_photons=300_000, n_queries=4_000, n_clusters=16)
+    g = ctx.grid_build(ph, aabb, cell_size=cell)
+    og = O.OracleGrid(ph, aabb, cell_size=cell)
+    oi, od, oc = og.knn(q, r, k)
+    o32, o64 = O.radiance(oi, od, oc, og.photons, qn, k)
+    gi, gd, gc = ctx.knn(g, q, r, k)
+    rgb_sep = ctx.radiance(g, gi, gd, gc, qn, k)
+    rgb_fused, cnt = ctx.gather_radiance(g, q, qn, r, k, want_count=True)
+    rgb_sorted = ctx.gather_radiance(g, q, qn, r, k, flags=pm355.KNN_SORT_QUERIES)
+    ctx.sync()
+    assert np.array_equal(_np(cnt, np.uint32), oc)
+    scale = np.maximum(np.abs(o64), 1e-30)
+    for rgb in (rgb_sep, rgb_fused, rgb_sorted):
+        got = _np(rgb).astype(np.float64)
+        rel = np.abs(got - o64) / scale
+        rel[o64 == 0] = np.abs(got[o64 == 0])
+        assert rel.max() <= RADIANCE_RTOL, rel.max()
+    assert np.array_equal(_np(rgb_fused), _np(rgb_sorted))
+
+
+def test_box_stats_match_oracle(ctx):
+    ph, q, qn, aabb = gather_microbench(seed=2, n_photons=200_000, n_queries=3_000, n_clusters=8)
+    g = ctx.grid_build(ph, aabb, cell_size=0.01)
+    og = O.OracleGrid(ph, aabb, cell_size=0.01)
+    og.knn(q, 0.01, 64)
+    cells, cands = ctx.box_stats(g, q, 0.01)
+    assert cells == og.last_stats["cells"]
+    assert cands == og.last_stats["cands"]
+
+
+def test_full_size_gather_properties(ctx):
+    """16M photons / 4M queries at k=64, r=0.01: size-independent checks
+    plus oracle parity on a seeded sample of the queries."""
+    import torch
+
+    ph, q, qn, aabb = gather_microbench(seed=1234)
+    g = ctx.grid_build(ph, aabb, cell_size=0.01)
+    starts, sizes, photons, perm = g.to_host()
+    assert int(sizes.astype(np.uint64).sum()) == ph.shape[1]
+    assert np.array_equal(starts[1:], np.cumsum(sizes, dtype=np.uint64)[:-1].astype(np.uint32))
+    assert np.array_equal(np.sort(perm), np.arange(ph.shape[1], dtype=np.uint32))
+    qd = torch.from_numpy(q).cuda()
+    gi, gd, gc = ctx.knn(g, qd, 0.01, 64, flags=pm355.KNN_SORT_QUERIES)
+    ctx.sync()
+    gd_h = _np(gd)
+    gc_h = _np(gc, np.uint32)
+    assert np.all(gc_h <= 64)
+    kept = np.arange(64)[None, :] < gc_h[:, None]
+    # ascending, within radius, sentinels past count
+    d = np.where(kept, gd_h, np.inf)
+    assert np.all(np.diff(d, axis=1)[kept[:, 1:]] >= 0)
+    assert np.all(gd_h[kept] <= np.float32(0.01))
+    assert np.all(_np(gi, np.uint32)[~kept] == 0xFFFFFFFF)
+    # oracle parity on 3000 sampled queries (oracle grid built independently)
+    og = O.OracleGrid(ph, aabb, cell_size=0.01)
+    sel = np.random.default_rng(0).choice(q.shape[1], 3000, replace=False)
+    oi, od, oc = og.knn(q[:, sel], 0.01, 64)
+    gi_h = _np(gi, np.uint32)[sel]
+    assert np.array_equal(gc_h[sel], oc)
+    assert np.array_equal(gi_h, oi)
+    assert np.array_equal(gd_h[sel].view(np.uint32), od.view(np.uint32))
