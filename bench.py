@@ -97,18 +97,43 @@ def main():
     d_q = torch.from_numpy(q).to(dev)
     d_n = torch.from_numpy(qn).to(dev)
     nq = q.shape[1]
-    flags = pm355.KNN_SORT_QUERIES if args.sort_queries else 0
     stream = torch.cuda.current_stream(dev)
 
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    # Pick the gather variant (autotune): each variant's kernel alone on the
+    # cell-ordered query block, timed with HIP events on the launch stream.
+    g = ctx.grid_build(d_ph, aabb, cell_size=r)
+    order = ctx.query_order(g, d_q).long()
+    d_qs = d_q[:, order].contiguous()
+    d_ns = d_n[:, order].contiguous()
+    torch.cuda.synchronize()
+    log("autotune gather variant")
+    tune = {}
+    for name, vf in pm355.KNN_VARIANTS.items():
+        ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vf)
+        tune[name] = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vf), 2)
+    best = min(tune, key=tune.get)
+    vflags = pm355.KNN_VARIANTS[best]
+    flags = (pm355.KNN_SORT_QUERIES if args.sort_queries else 0) | vflags
+    log(f"variant {best}: {tune}")
+
     def step():
-        g = ctx.grid_build(d_ph, aabb, cell_size=r)
-        rgb, cnt = ctx.gather_radiance(g, d_q, d_n, r, k, flags=flags, want_count=True)
-        return g, rgb, cnt
+        gg = ctx.grid_build(d_ph, aabb, cell_size=r)
+        rgb, cnt = ctx.gather_radiance(gg, d_q, d_n, r, k, flags=flags, want_count=True)
+        return gg, rgb, cnt
 
     log("warmup")
     for _ in range(args.warmup):
-        g, rgb, cnt = step()
-        g.close()
+        gg, rgb, cnt = step()
+        gg.close()
     torch.cuda.synchronize()
 
     log(f"timing {args.steps} steps")
@@ -116,43 +141,25 @@ def main():
     t0 = time.perf_counter()
     grids = []
     for _ in range(args.steps):
-        g, rgb, cnt = step()
-        grids.append(g)
+        gg, rgb, cnt = step()
+        grids.append(gg)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for g in grids:
-        g.close()
+    for gg in grids:
+        gg.close()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"{ms_per_step:.3f} ms/step")
 
     # Component timing with HIP events on the stream the kernels run on.
-    g = ctx.grid_build(d_ph, aabb, cell_size=r)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
     reps = max(3, args.steps)
-    ev[0].record(stream)
-    for _ in range(reps):
-        gg = ctx.grid_build(d_ph, aabb, cell_size=r)
-        gg.close()
-    ev[1].record(stream)
-    for _ in range(reps):
-        ctx.gather_radiance(g, d_q, d_n, r, k, flags=0)
-    ev[2].record(stream)
-    for _ in range(reps):
-        ctx.gather_radiance(g, d_q, d_n, r, k, flags=pm355.KNN_SORT_QUERIES)
-    ev[3].record(stream)
-    for _ in range(reps):
-        idx, dist, cnt2 = ctx.knn(g, d_q, r, k, flags=flags)
-    ev[4].record(stream)
-    torch.cuda.synchronize()
-    build_ms = ev[0].elapsed_time(ev[1]) / reps
-    gather_ms = ev[1].elapsed_time(ev[2]) / reps       # k_knn alone (no query sort)
-    gather_sorted_ms = ev[2].elapsed_time(ev[3]) / reps
-    knn_only_ms = ev[3].elapsed_time(ev[4]) / reps
+    build_ms = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r).close(), reps)
+    order_ms = timed(lambda: ctx.query_order(g, d_q), reps)
+    gather_ms = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags), reps)
 
-    # Algorithmic bytes of one fused gather launch (k_knn):
+    # Algorithmic bytes of one fused gather launch:
     # per query 12 (query) + 8*C_q (start+size) + 12*P_q (candidate positions)
     # + 24*kept (power+dir of kept photons) + 12 (normal) + 12 (RGB out).
-    rgb, cnt = ctx.gather_radiance(g, d_q, d_n, r, k, flags=0, want_count=True)
+    rgb, cnt = ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags, want_count=True)
     torch.cuda.synchronize()
     kept = int(cnt.to(torch.int64).sum().item())
     cells, cands = ctx.box_stats(g, d_q, r)
@@ -181,6 +188,7 @@ def main():
             "k": k,
             "radius": r,
             "sort_queries": bool(args.sort_queries),
+            "gather_variant": best,
         },
         "roofline": {
             "bound": "hbm",
@@ -189,17 +197,17 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_knn (fused gather + radiance, queries in input order)",
+            "kernel": f"k_knn_{best} (fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
             "alg_bytes": alg_bytes,
         },
         "components_ms": {
             "grid_build": build_ms,
+            "query_order": order_ms,
             "gather_radiance": gather_ms,
-            "gather_radiance_sorted_queries": gather_sorted_ms,
-            "knn_only_with_index_outputs": knn_only_ms,
         },
-        "gather_only_queries_per_s": nq / (min(gather_ms, gather_sorted_ms) * 1e-3),
+        "autotune_gather_ms": tune,
+        "gather_only_queries_per_s": nq / (gather_ms * 1e-3),
         "box_stats": {"cells": cells, "candidates": cands, "kept": kept},
     }
     g.close()

@@ -1,0 +1,388 @@
+// Photon kd-tree build (cpm::PhotonMap, PhotonMap.cuh:189-283) and the
+// per-model AABB of create_kd_tree (KdTree.cu:395-464).
+//
+// Photon kd-tree, level by level instead of the reference's sequential LIFO
+// of quicksorts:
+//   * three presorted orders, one per axis, each a stable radix sort of the
+//     coordinate (so ties are ordered by photon index, the total order the
+//     tree is defined on), stored as 16-byte records {x, y, z, id};
+//   * at every level each node's segment [from, to) has its median at
+//     mid = from + (to - from)/2 in the order of the node's split axis;
+//     the other two orders are stable-partitioned around that median
+//     (flag -> scan -> scatter), so every order stays sorted inside every
+//     child segment;
+//   * a child's split plane is the argmax extent of its points, read from
+//     the first and last record of its segment in each axis order.
+// The implicit heap (children 2i+1, 2i+2) is stored with 2^L - 1 slots so no
+// node is dropped (the reference's Tree drops indices >= n, Tree.cuh:10-13).
+#include <cfloat>
+#include <cmath>
+
+#include "pm_internal.hpp"
+
+struct pm_kdtree {
+    uint32_t n = 0, cap = 0, levels = 0;
+    void* mem = nullptr;
+    uint32_t* node_idx = nullptr;  // [cap]
+    int8_t* node_plane = nullptr;  // [cap]
+    float* photons = nullptr;      // photons9 in node order [9][cap] (empty slots hold +inf positions)
+};
+
+namespace {
+
+struct Rec {
+    float c[3];
+    uint32_t id;
+};
+
+constexpr uint32_t kFinal = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t orderable(float f) {
+    f = f + 0.0f;  // -0 -> +0 so that equal coordinates tie on the index
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void k_axis_keys(const float* __restrict__ p, uint32_t n, uint32_t* __restrict__ keys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = orderable(p[i]);
+}
+
+__global__ __launch_bounds__(256) void k_gather_recs(const float* __restrict__ px, const float* __restrict__ py,
+                                                    const float* __restrict__ pz, const uint32_t* __restrict__ order,
+                                                    uint32_t n, Rec* __restrict__ out) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t id = order[j];
+    Rec r;
+    r.c[0] = px[id];
+    r.c[1] = py[id];
+    r.c[2] = pz[id];
+    r.id = id;
+    out[j] = r;
+}
+
+// PhotonMap::find_largest_plane (PhotonMap.cuh:161-178) from the extreme
+// records of a non-empty segment in each axis order.
+__device__ __forceinline__ int8_t plane_of(const Rec* A0, const Rec* A1, const Rec* A2, uint32_t f, uint32_t t) {
+    float d0 = fabsf(A0[t - 1].c[0] - A0[f].c[0]);
+    float d1 = fabsf(A1[t - 1].c[1] - A1[f].c[1]);
+    float d2 = fabsf(A2[t - 1].c[2] - A2[f].c[2]);
+    int8_t p = 0;
+    float mx = d0;
+    if (d1 > mx) {
+        mx = d1;
+        p = 1;
+    }
+    if (d2 > mx) p = 2;
+    return p;
+}
+
+__global__ void k_root(const Rec* __restrict__ A0, const Rec* __restrict__ A1, const Rec* __restrict__ A2, uint32_t n,
+                       uint2* __restrict__ seg, int8_t* __restrict__ plane) {
+    seg[0] = make_uint2(0, n);
+    plane[0] = plane_of(A0, A1, A2, 0, n);
+}
+
+// Per node of the level: record the median, its key, and the children's
+// segments.
+__global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
+                                                    const Rec* __restrict__ A2, uint32_t first, uint32_t count,
+                                                    uint32_t cap, uint2* __restrict__ seg,
+                                                    const int8_t* __restrict__ plane, uint32_t* __restrict__ node_idx,
+                                                    int8_t* __restrict__ node_plane, uint2* __restrict__ med) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint32_t node = first + t;
+    uint2 s = seg[node];
+    uint32_t l = 2 * node + 1, r = 2 * node + 2;
+    if (s.y <= s.x) {  // empty
+        if (l < cap) seg[l] = make_uint2(0, 0);
+        if (r < cap) seg[r] = make_uint2(0, 0);
+        return;
+    }
+    int8_t p = plane[node];
+    const Rec* A = p == 0 ? A0 : (p == 1 ? A1 : A2);
+    uint32_t mid = s.x + (s.y - s.x) / 2;
+    Rec m = A[mid];
+    node_idx[node] = m.id;
+    node_plane[node] = p;
+    med[node] = make_uint2(__float_as_uint(m.c[p]), m.id);
+    if (s.y - s.x == 1) {
+        if (l < cap) seg[l] = make_uint2(0, 0);
+        if (r < cap) seg[r] = make_uint2(0, 0);
+    } else {
+        if (l < cap) seg[l] = make_uint2(s.x, mid);
+        if (r < cap) seg[r] = make_uint2(mid + 1, s.y);
+    }
+}
+
+// side of record e relative to node's median on plane p: 0 left, 1 median, 2 right
+__device__ __forceinline__ int side_of(const Rec& e, int p, uint2 m) {
+    if (e.id == m.y) return 1;
+    float a = e.c[p], b = __uint_as_float(m.x);
+    bool less = a < b || (a == b && e.id < m.y);
+    return less ? 0 : 2;
+}
+
+// flags[a*n + j] = (record j of order a goes left); mpos[node*3 + a] = j of the median.
+__global__ __launch_bounds__(256) void k_level_flags(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
+                                                    const Rec* __restrict__ A2, uint32_t n,
+                                                    const uint32_t* __restrict__ node_of,
+                                                    const int8_t* __restrict__ plane, const uint2* __restrict__ med,
+                                                    uint32_t* __restrict__ flags, uint32_t* __restrict__ mpos) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = node_of[j];
+    if (s == kFinal) {
+        flags[j] = 0;
+        flags[n + j] = 0;
+        flags[2 * (size_t)n + j] = 0;
+        return;
+    }
+    int p = plane[s];
+    uint2 m = med[s];
+    const Rec* As[3] = {A0, A1, A2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        Rec e = As[a][j];
+        int sd = side_of(e, p, m);
+        flags[(size_t)a * n + j] = sd == 0 ? 1u : 0u;
+        if (sd == 1) mpos[(size_t)s * 3 + a] = j;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
+                                                      const Rec* __restrict__ A2, Rec* __restrict__ B0,
+                                                      Rec* __restrict__ B1, Rec* __restrict__ B2, uint32_t n,
+                                                      uint32_t* __restrict__ node_of, const uint2* __restrict__ seg,
+                                                      const uint32_t* __restrict__ flags,
+                                                      const uint32_t* __restrict__ pref,
+                                                      const uint32_t* __restrict__ mpos) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = node_of[j];
+    const Rec* As[3] = {A0, A1, A2};
+    Rec* Bs[3] = {B0, B1, B2};
+    if (s == kFinal) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Bs[a][j] = As[a][j];
+        return;
+    }
+    uint2 sg = seg[s];
+    uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        size_t o = (size_t)a * n;
+        uint32_t before_l = pref[o + j] - pref[o + from];
+        uint32_t dst;
+        if (flags[o + j]) {
+            dst = from + before_l;
+        } else {
+            uint32_t mp = mpos[(size_t)s * 3 + a];
+            if (mp == j)
+                dst = mid;
+            else
+                dst = mid + 1 + ((j - from) - before_l - (mp < j ? 1u : 0u));
+        }
+        Bs[a][dst] = As[a][j];
+    }
+    uint32_t nn;
+    if (sg.y - sg.x <= 1 || j == mid)
+        nn = kFinal;
+    else
+        nn = j < mid ? 2 * s + 1 : 2 * s + 2;
+    node_of[j] = nn;
+}
+
+__global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
+                                                     const Rec* __restrict__ A2, uint32_t first, uint32_t count,
+                                                     const uint2* __restrict__ seg, int8_t* __restrict__ plane) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint32_t node = first + t;
+    uint2 s = seg[node];
+    // PhotonMap.cuh:252-273: an empty child keeps +-FLT_MAX corners -> plane 0
+    plane[node] = s.y > s.x ? plane_of(A0, A1, A2, s.x, s.y) : (int8_t)0;
+}
+
+__global__ __launch_bounds__(256) void k_tree_photons(const float* __restrict__ in9, uint32_t n,
+                                                     const uint32_t* __restrict__ node_idx, uint32_t cap,
+                                                     float* __restrict__ out9) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    uint32_t id = node_idx[i];
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+        out9[(size_t)p * cap + i] = id != kFinal ? in9[(size_t)p * n + id] : (p < 3 ? __builtin_inff() : 0.f);
+}
+
+// ---- model AABB (the live effect of create_kd_tree, KdTree.cu:395-464) ----
+// One workgroup per model: min/max over the model's vertices with wave
+// reductions. Replaces per-triangle AABBs -> 32-triangle chunks -> pairwise
+// segmented reduction (KdTree.cu:82-327).
+__global__ __launch_bounds__(256) void k_model_aabb(const float* __restrict__ tris9, const uint32_t* __restrict__ tb,
+                                                   float* __restrict__ out) {
+    __shared__ float s[4][6];
+    const uint32_t m = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t v0 = tb[m] * 3, v1 = tb[m + 1] * 3;  // vertex range
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t v = v0 + t; v < v1; v += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float p = tris9[(size_t)v * 3 + a];
+            mn[a] = fminf(mn[a], p);
+            mx[a] = fmaxf(mx[a], p);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], d, 64));
+        }
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            s[wave][a] = mn[a];
+            s[wave][a + 3] = mx[a];
+        }
+    __syncthreads();
+    if (t < 6) {
+        float r = s[0][t];
+        for (int w = 1; w < 4; ++w) r = t < 3 ? fminf(r, s[w][t]) : fmaxf(r, s[w][t]);
+        out[(size_t)m * 6 + t] = r;
+    }
+}
+
+}  // namespace
+
+extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, pm_kdtree** out) {
+    if (!ctx || !out) return PM_ERR_INVALID;
+    *out = nullptr;
+    if (n == 0 || !d_photons9) return pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_build: no photons");
+    if (n > (1u << 30)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_kdtree_build: n too large");
+    uint32_t L = 0;
+    while (((1ull << L) - 1) < (uint64_t)n) ++L;
+    const uint32_t cap = (uint32_t)((1ull << L) - 1);
+
+    pm_kdtree* T = new pm_kdtree();
+    T->n = n;
+    T->cap = cap;
+    T->levels = L;
+    size_t bytes = pm::carve_size({(size_t)cap * 4, (size_t)cap, (size_t)cap * 36});
+    hipError_t e = hipMalloc(&T->mem, bytes);
+    if (e != hipSuccess) {
+        delete T;
+        return pm::hip_fail(ctx, e, "hipMalloc(kdtree)");
+    }
+    pm::Carve tc{static_cast<char*>(T->mem)};
+    T->node_idx = tc.take<uint32_t>(cap);
+    T->node_plane = tc.take<int8_t>(cap);
+    T->photons = tc.take<float>((size_t)cap * 9);
+    auto bail = [&](pm_status s) {
+        (void)hipFree(T->mem);
+        delete T;
+        return s;
+    };
+
+    size_t rs = pm::radix_sort_scratch_bytes(n);
+    size_t need = pm::carve_size({(size_t)n * 16 * 6, (size_t)n * 4 * 3, (size_t)n * 4 * 3, (size_t)n * 4,
+                                  (size_t)cap * 8, (size_t)cap, (size_t)cap * 8, (size_t)cap * 12,
+                                  (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs});
+    void* work = nullptr;
+    pm_status st = pm::scratch_reserve(ctx, need, &work);
+    if (st != PM_OK) return bail(st);
+    pm::Carve cv{static_cast<char*>(work)};
+    Rec* A[3];
+    Rec* B[3];
+    for (int a = 0; a < 3; ++a) A[a] = cv.take<Rec>(n);
+    for (int a = 0; a < 3; ++a) B[a] = cv.take<Rec>(n);
+    uint32_t* flags = cv.take<uint32_t>((size_t)n * 3);
+    uint32_t* pref = cv.take<uint32_t>((size_t)n * 3);
+    uint32_t* node_of = cv.take<uint32_t>(n);
+    uint2* seg = cv.take<uint2>(cap);
+    int8_t* plane = cv.take<int8_t>(cap);
+    uint2* med = cv.take<uint2>(cap);
+    uint32_t* mpos = cv.take<uint32_t>((size_t)cap * 3);
+    uint32_t* keys = cv.take<uint32_t>(n);
+    uint32_t* skeys = cv.take<uint32_t>(n);
+    uint32_t* order = cv.take<uint32_t>(n);
+    void* rw = cv.take<char>(rs);
+
+    const uint32_t nb = pm::div_up(n, 256);
+    hipStream_t s = ctx->stream;
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(k_axis_keys, dim3(nb), dim3(256), 0, s, d_photons9 + (size_t)a * n, n, keys);
+        if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, 32, rw)) != PM_OK) return bail(st);
+        hipLaunchKernelGGL(k_gather_recs, dim3(nb), dim3(256), 0, s, d_photons9, d_photons9 + n,
+                           d_photons9 + 2 * (size_t)n, order, n, A[a]);
+    }
+    if ((st = pm::fill_u32(ctx, T->node_idx, kFinal, cap)) != PM_OK) return bail(st);
+    // node_plane = -1; the carve pads every block to 256 bytes, so whole
+    // words past cap stay inside the allocation.
+    if ((st = pm::fill_u32(ctx, reinterpret_cast<uint32_t*>(T->node_plane), 0xFFFFFFFFu, (cap + 3) / 4)) != PM_OK)
+        return bail(st);
+    if ((st = pm::fill_u32(ctx, node_of, 0u, n)) != PM_OK) return bail(st);
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, A[0], A[1], A[2], n, seg, plane);
+
+    for (uint32_t lvl = 0; lvl < L; ++lvl) {
+        uint32_t first = (1u << lvl) - 1, count = 1u << lvl;
+        hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(count, 256)), dim3(256), 0, s, A[0], A[1], A[2], first,
+                           count, cap, seg, plane, T->node_idx, T->node_plane, med);
+        if (lvl + 1 == L) break;  // every segment of the last level has size <= 1
+        hipLaunchKernelGGL(k_level_flags, dim3(nb), dim3(256), 0, s, A[0], A[1], A[2], n, node_of, plane, med, flags,
+                           mpos);
+        if ((st = pm::scan_u32(ctx, flags, pref, 3 * n, false, nullptr)) != PM_OK) return bail(st);
+        hipLaunchKernelGGL(k_level_scatter, dim3(nb), dim3(256), 0, s, A[0], A[1], A[2], B[0], B[1], B[2], n, node_of,
+                           seg, flags, pref, mpos);
+        for (int a = 0; a < 3; ++a) std::swap(A[a], B[a]);
+        uint32_t cf = (1u << (lvl + 1)) - 1, cc = 1u << (lvl + 1);
+        hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
+                           plane);
+    }
+    hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, d_photons9, n, T->node_idx, cap,
+                       T->photons);
+    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "pm_kdtree_build: launch failed"));
+    *out = T;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_kdtree_describe(const pm_kdtree* t, pm_kdtree_desc* out) {
+    if (!t || !out) return PM_ERR_INVALID;
+    out->n_photons = t->n;
+    out->n_slots = t->cap;
+    out->levels = t->levels;
+    out->d_node_idx = t->node_idx;
+    out->d_node_plane = t->node_plane;
+    out->d_photons = t->photons;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* t, uint32_t* h_node_idx,
+                                            int8_t* h_node_plane) {
+    if (!ctx || !t) return PM_ERR_INVALID;
+    if (h_node_idx) PM_HIP(ctx, hipMemcpyAsync(h_node_idx, t->node_idx, (size_t)t->cap * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_node_plane) PM_HIP(ctx, hipMemcpyAsync(h_node_plane, t->node_plane, t->cap, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PM_OK;
+}
+
+extern "C" pm_status pm_kdtree_destroy(pm_kdtree* t) {
+    if (!t) return PM_OK;
+    if (t->mem) (void)hipFree(t->mem);
+    delete t;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_model_aabb(pm_ctx* ctx, const float* d_tris9, const uint32_t* d_tri_begin, int32_t n_models,
+                                   float* d_aabb6) {
+    if (!ctx) return PM_ERR_INVALID;
+    if (n_models < 0 || (n_models > 0 && (!d_tris9 || !d_tri_begin || !d_aabb6)))
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_model_aabb: bad arguments");
+    if (n_models == 0) return PM_OK;
+    hipLaunchKernelGGL(k_model_aabb, dim3(n_models), dim3(256), 0, ctx->stream, d_tris9, d_tri_begin, d_aabb6);
+    PM_CHECK_LAUNCH(ctx, "k_model_aabb");
+    return PM_OK;
+}

@@ -5,19 +5,21 @@
 // (Raytracer.cuh:110-138). One wave64 serves one query:
 //   1. the candidate box (query cell +- ceil(r/cell), clamped) is split into
 //      contiguous photon runs (one per (y,z) row of cells in the linear
-//      layout, one per cell in the Morton layout); lane s loads run s and a
-//      wave scan turns the run lengths into a flat candidate index space;
-//   2. pass 1 streams the candidates 64 at a time (coalesced SoA loads),
-//      computes d2 = (dx*dx + dy*dy) + dz*dz exactly as the reference host
-//      does, and histograms the in-radius d2 into 64 bins given by the top
-//      float bits (two bins per octave below the radius threshold) in LDS;
-//   3. a wave scan of the histogram gives the smallest bin edge that keeps
-//      at least k photons (rarely, a refinement pass narrows a crowded bin);
-//   4. pass 2 compacts the keys (d2 bits << 32 | grid slot) below that edge
-//      into an LDS buffer with ballot + mbcnt;
-//   5. a register bitonic sort across the wave orders them; the first k are
-//      the answer — the exact k smallest by (d2, slot);
-//   6. optionally the radiance estimate runs on those k in the same wave.
+//      layout, one per cell in the Morton layout), one run per lane;
+//   2. the candidates stream 64 at a time (coalesced SoA loads, next step
+//      prefetched); d2 = (dx*dx + dy*dy) + dz*dz exactly as the reference
+//      host computes it, radius test d2 <= T with T the largest float whose
+//      sqrtf is <= r (identical to the reference's sqrtf(d2) <= r);
+//   3. in-radius keys (d2 bits << 32 | grid slot) are appended to an LDS
+//      buffer with ballot + mbcnt; when it fills, a histogram over the top
+//      float bits of d2 (two bins per octave) finds the smallest bin edge
+//      keeping >= k keys, the buffer is compacted to it and the acceptance
+//      bound tightens (a refinement splits a crowded bin 64 ways);
+//   4. a register bitonic sort across the wave orders the <= 256 survivors;
+//      the first k are the answer — the exact k smallest by (d2, slot);
+//   5. optionally the radiance estimate runs on those k in the same wave.
+// A two-pass variant (histogram pass, then collect pass) is kept selectable
+// for tuning.
 #include <cmath>
 #include <cstring>
 
@@ -27,9 +29,7 @@ namespace {
 
 using pmd::GridGeom;
 
-constexpr int kWaves = 4;    // queries per 256-thread workgroup
-constexpr int kCap = 512;    // compaction buffer (keys) per wave
-constexpr int kMaxSeg = 64;  // photon runs handled per batch
+constexpr int kWaves = 4;  // queries per 256-thread workgroup
 
 struct KnnParams {
     GridGeom G;
@@ -59,11 +59,10 @@ struct KnnParams {
     int remap;
 };
 
+template <int CAP>
 struct WaveLds {
-    uint32_t seg_begin[kMaxSeg];
-    uint32_t seg_off[kMaxSeg + 1];
     uint32_t hist[64];
-    uint64_t buf[kCap];
+    uint64_t buf[CAP];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -87,6 +86,10 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint32_t mbcnt(uint64_t bal) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+}
+
 // Squared distance in the reference's host order, never contracted to FMA.
 __device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
     float dx = px - qx, dy = py - qy, dz = pz - qz;
@@ -94,20 +97,26 @@ __device__ __forceinline__ float dist2(float px, float py, float pz, float qx, f
     return __fadd_rn(__fadd_rn(xx, yy), zz);
 }
 
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t slot) {
+    return ((uint64_t)__float_as_uint(d2) << 32) | slot;
+}
+
 struct Box {
     int lo[3], hi[3];
 };
 
 // Visit every candidate of the box, 64 per step, calling
-// f(valid, slot, d2) on all lanes (valid = lane holds a candidate).
+// f(valid, slot, d2) on all lanes (valid = lane holds a candidate). Runs of
+// photons (rows) are held one per lane; a scalar cursor walks them, and the
+// next step's positions are loaded before the current step is processed.
 template <class F>
-__device__ __forceinline__ void for_each_candidate(const KnnParams& P, WaveLds& L, uint32_t lane, const Box& B,
-                                                   float qx, float qy, float qz, F&& f) {
+__device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
+                                     F&& f) {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
     const int nseg = linear ? ny * nz : nx * ny * nz;
-    for (int sbase = 0; sbase < nseg; sbase += kMaxSeg) {
-        int s = sbase + (int)lane;
+    for (int sbase = 0; sbase < nseg; sbase += 64) {
+        const int s = sbase + (int)lane;
         uint32_t b = 0, len = 0;
         if (s < nseg) {
             if (linear) {
@@ -124,27 +133,127 @@ __device__ __forceinline__ void for_each_candidate(const KnnParams& P, WaveLds& 
                 len = P.sizes[c];
             }
         }
-        uint32_t incl = wave_incl_scan(len, lane);
-        uint32_t M = __shfl(incl, 63, 64);
-        L.seg_begin[lane] = b;
-        L.seg_off[lane] = incl - len;
-        if (lane == 63) L.seg_off[kMaxSeg] = incl;
-        wave_sync();
-        int seg = 0;
-        for (uint32_t base = 0; base < M; base += 64) {
-            uint32_t fi = base + lane;
-            bool valid = fi < M;
-            uint32_t slot = 0;
-            float d2 = 0.f;
-            if (valid) {
-                while (L.seg_off[seg + 1] <= fi) ++seg;
-                slot = L.seg_begin[seg] + (fi - L.seg_off[seg]);
-                d2 = dist2(P.px[slot], P.py[slot], P.pz[slot], qx, qy, qz);
+        const int ns = min(64, nseg - sbase);
+        // Non-empty rows as a lane mask; the cursor visits them in order.
+        uint64_t rows = __ballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
+        if (!rows) continue;
+        int cs = __builtin_ctzll(rows);
+        rows &= rows - 1;
+        uint32_t rb = __builtin_amdgcn_readlane(b, cs), rl = __builtin_amdgcn_readlane(len, cs), cc = 0;
+        bool v0 = lane < rl;
+        uint32_t slot0 = rb + lane;
+        float x0 = 0.f, y0 = 0.f, z0 = 0.f;
+        if (v0) {
+            x0 = P.px[slot0];
+            y0 = P.py[slot0];
+            z0 = P.pz[slot0];
+        }
+        for (;;) {
+            // next step: same row if it has more, else the next non-empty row
+            bool has_next = true;
+            uint32_t ncc = cc + 64, nrb = rb, nrl = rl;
+            if (ncc >= rl) {
+                if (!rows) {
+                    has_next = false;
+                } else {
+                    int ncs = __builtin_ctzll(rows);
+                    rows &= rows - 1;
+                    nrb = __builtin_amdgcn_readlane(b, ncs);
+                    nrl = __builtin_amdgcn_readlane(len, ncs);
+                    ncc = 0;
+                }
             }
-            f(valid, slot, d2);
+            bool v1 = false;
+            uint32_t slot1 = 0;
+            float x1 = 0.f, y1 = 0.f, z1 = 0.f;
+            if (has_next) {
+                v1 = ncc + lane < nrl;
+                slot1 = nrb + ncc + lane;
+                if (v1) {
+                    x1 = P.px[slot1];
+                    y1 = P.py[slot1];
+                    z1 = P.pz[slot1];
+                }
+            }
+            f(v0, slot0, v0 ? dist2(x0, y0, z0, qx, qy, qz) : 0.f);
+            if (!has_next) break;
+            cc = ncc;
+            rb = nrb;
+            rl = nrl;
+            v0 = v1;
+            slot0 = slot1;
+            x0 = x1;
+            y0 = y1;
+            z0 = z1;
+        }
+    }
+}
+
+// Narrow the bucket of keys kept from LDS buffer entries [0, cnt) — all of
+// which are < khi — to the smallest bin edge that keeps >= k keys and at most
+// `target` when the keys allow, then compact the buffer to those keys.
+// Round 0 bins by the top float bits of d2 (two bins per octave below the
+// radius threshold); later rounds split the crossing bin 64 ways.
+template <int CAP>
+__device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uint32_t k, uint32_t target, uint32_t bT,
+                               uint64_t& khi_io) {
+    if (cnt <= target || cnt <= k) return cnt;
+    L.hist[lane] = 0;
+    wave_sync();
+    for (uint32_t e = lane; e < cnt; e += 64) {
+        uint32_t h = bT - (uint32_t)(L.buf[e] >> 54);
+        atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
+    }
+    wave_sync();
+    uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
+    int ls = __builtin_ctzll(__ballot(cum >= k));
+    uint32_t hstar = 63u - (uint32_t)ls;
+    uint32_t A = __shfl(cum, ls, 64);
+    uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
+    uint64_t khi = (uint64_t)(bT - hstar + 1u) << 54, klo = 0;
+    uint32_t below = 0;
+    if (hstar < 63u) {
+        klo = (uint64_t)(bT - hstar) << 54;
+        below = ls > 0 ? prev : 0u;
+    }
+    if (khi > khi_io) khi = khi_io;
+    while (A > target) {
+        uint64_t width = khi - klo;
+        int s = width <= 64 ? 0 : 64 - __builtin_clzll(width - 1) - 6;
+        L.hist[lane] = 0;
+        wave_sync();
+        for (uint32_t e = lane; e < cnt; e += 64) {
+            uint64_t key = L.buf[e];
+            if (key >= klo && key < khi) atomicAdd(&L.hist[(uint32_t)((key - klo) >> s)], 1u);
         }
         wave_sync();
+        uint32_t cu = wave_incl_scan(L.hist[lane], lane);
+        uint32_t need = k - below;
+        int js = __builtin_ctzll(__ballot(cu >= need));
+        uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
+        uint32_t at = __shfl(cu, js, 64);
+        uint64_t nlo = klo + ((uint64_t)js << s);
+        uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
+        if (nhi > khi) nhi = khi;
+        A = below + at;
+        below += js > 0 ? pv : 0u;
+        klo = nlo;
+        khi = nhi;
     }
+    // in-place compaction, one row of 64 at a time (writes never pass reads)
+    uint32_t out = 0;
+    for (uint32_t base = 0; base < cnt; base += 64) {
+        uint32_t e = base + lane;
+        uint64_t key = e < cnt ? L.buf[e] : ~0ull;
+        wave_sync();
+        bool keep = e < cnt && key < khi;
+        uint64_t bal = __ballot(keep);
+        if (keep) L.buf[out + mbcnt(bal)] = key;
+        out += (uint32_t)__popcll(bal);
+        wave_sync();
+    }
+    khi_io = khi;
+    return out;
 }
 
 template <int R>
@@ -198,7 +307,7 @@ __device__ __forceinline__ void radiance_epilogue(const KnnParams& P, const uint
         float dm = 0.f;
 #pragma unroll
         for (int j = 0; j < R; ++j)
-            if (j == jm) dm = __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32)));
+            if (j == jm) dm = sqrtf(__uint_as_float((uint32_t)(v[j] >> 32)));
         rmax = __shfl(dm, (int)((m - 1) & 63), 64);
         const float filter_r = __fmul_rn(rmax, rmax);
         const float denom = 1.f - expf(-beta);
@@ -211,7 +320,7 @@ __device__ __forceinline__ void radiance_epilogue(const KnnParams& P, const uint
                 float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
                 float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
                 if (cosNL > 0.f) {
-                    float d = __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32)));
+                    float d = sqrtf(__uint_as_float((uint32_t)(v[j] >> 32)));
                     float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
                     float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
                     res0 += P.power[idx] * w;
@@ -236,8 +345,8 @@ __device__ __forceinline__ void radiance_epilogue(const KnnParams& P, const uint
     }
 }
 
-template <int R>
-__device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds& L, uint32_t lane, uint32_t q,
+template <int R, int CAP>
+__device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q,
                                               uint32_t cnt, uint32_t m) {
     uint64_t v[R];
 #pragma unroll
@@ -254,119 +363,140 @@ __device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds& L, ui
                 bool keep = e < m;
                 P.out_idx[(size_t)q * P.k + e] = keep ? (uint32_t)v[j] : 0xFFFFFFFFu;
                 P.out_dist[(size_t)q * P.k + e] =
-                    keep ? __fsqrt_rn(__uint_as_float((uint32_t)(v[j] >> 32))) : __builtin_inff();
+                    keep ? sqrtf(__uint_as_float((uint32_t)(v[j] >> 32))) : __builtin_inff();
             }
+        }
+        // slots beyond the sorted registers (k > 64*R >= cnt >= m) are empty
+        for (uint32_t e = 64u * R + lane; e < (uint32_t)P.k; e += 64) {
+            P.out_idx[(size_t)q * P.k + e] = 0xFFFFFFFFu;
+            P.out_dist[(size_t)q * P.k + e] = __builtin_inff();
         }
     }
     if (P.out_count && lane == 0) P.out_count[q] = m;
     if (P.rgb) radiance_epilogue<R>(P, v, m, lane, q);
 }
 
-__global__ __launch_bounds__(256) void k_knn(KnnParams P) {
-    __shared__ WaveLds lds[kWaves];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+template <int CAP>
+__device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q, uint32_t cnt,
+                                       uint32_t found, uint64_t khi) {
+    const uint32_t k = (uint32_t)P.k;
+    const uint32_t m = found < k ? found : k;
+    // the sort holds at most 256 keys
+    cnt = buf_select<CAP>(L, lane, cnt, k, 256u, P.bT, khi);
+    wave_sync();
+    if (cnt <= 64)
+        sort_and_emit<1, CAP>(P, L, lane, q, cnt, m);
+    else if (cnt <= 128)
+        sort_and_emit<2, CAP>(P, L, lane, q, cnt, m);
+    else
+        sort_and_emit<4, CAP>(P, L, lane, q, cnt, m);
+}
+
+__device__ __forceinline__ bool query_box(const KnnParams& P, uint32_t& q, Box& B, float& qx, float& qy, float& qz) {
+    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t ql = blk * kWaves + wave;
-    if (ql >= P.nq) return;  // wave-uniform; no workgroup barriers below
-    const uint32_t q = P.order ? P.order[ql] : ql;
-    WaveLds& L = lds[wave];
-
-    const float qx = P.qx[q], qy = P.qy[q], qz = P.qz[q];
+    if (ql >= P.nq) return false;
+    q = P.order ? P.order[ql] : ql;
+    qx = P.qx[q];
+    qy = P.qy[q];
+    qz = P.qz[q];
     const float qv[3] = {qx, qy, qz};
-    Box B;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         int c = pmd::cell_coord(qv[a], P.G.mn[a], P.G.cell, P.G.g[a]);
         B.lo[a] = max(0, c - P.rc);
         B.hi[a] = min(P.G.g[a] - 1, c + P.rc);
     }
+    return true;
+}
 
-    // Pass 1: histogram of in-radius d2 over 64 float-bit bins.
+// Single pass: in-radius keys stream into an LDS buffer of CAP keys; when it
+// would overflow, buf_select keeps only the smallest bin range holding >= k
+// keys and tightens the acceptance bound. Exact: a dropped key has >= k kept
+// keys below it.
+template <int CAP>
+__global__ __launch_bounds__(256) void k_knn_stream(KnnParams P) {
+    __shared__ WaveLds<CAP> lds[kWaves];
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t q;
+    Box B;
+    float qx, qy, qz;
+    if (!query_box(P, q, B, qx, qy, qz)) return;  // wave-uniform; no workgroup barriers below
+    WaveLds<CAP>& L = lds[threadIdx.x >> 6];
+    const float T = P.T;
+    const uint32_t k = (uint32_t)P.k;
+    uint32_t cnt = 0, found = 0;
+    uint64_t khi = ~0ull;
+    walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+        bool in = valid && d2 <= T;
+        found += (uint32_t)__popcll(__ballot(in));
+        uint64_t key = make_key(d2, slot);
+        bool acc = in && key < khi;
+        uint64_t bal = __ballot(acc);
+        uint32_t na = (uint32_t)__popcll(bal);
+        if (cnt + na > (uint32_t)CAP) {
+            wave_sync();
+            cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
+            acc = acc && key < khi;
+            bal = __ballot(acc);
+            na = (uint32_t)__popcll(bal);
+        }
+        if (acc) L.buf[cnt + mbcnt(bal)] = key;
+        cnt += na;
+    });
+    wave_sync();
+    finish<CAP>(P, L, lane, q, cnt, found, khi);
+}
+
+// Two passes: histogram of in-radius d2 over the float-bit bins, then the
+// keys below the bin edge that keeps >= k are compacted into the buffer.
+__global__ __launch_bounds__(256) void k_knn_two_pass(KnnParams P) {
+    constexpr int CAP = 512;
+    __shared__ WaveLds<CAP> lds[kWaves];
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t q;
+    Box B;
+    float qx, qy, qz;
+    if (!query_box(P, q, B, qx, qy, qz)) return;
+    WaveLds<CAP>& L = lds[threadIdx.x >> 6];
+    const float T = P.T;
+    const uint32_t bT = P.bT, k = (uint32_t)P.k;
     L.hist[lane] = 0;
     wave_sync();
-    const float T = P.T;
-    const uint32_t bT = P.bT;
-    for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t, float d2) {
+    walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t, float d2) {
         if (valid && d2 <= T) {
             uint32_t h = bT - (__float_as_uint(d2) >> 22);
             atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
         }
     });
     wave_sync();
-    uint32_t c = L.hist[63 - lane];  // lane 0 = nearest bin
-    uint32_t cum = wave_incl_scan(c, lane);
+    uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
     const uint32_t found = __shfl(cum, 63, 64);
-    const uint32_t k = (uint32_t)P.k;
-    const uint32_t m = found < k ? found : k;
-
-    uint64_t khi = ~0ull, klo = 0;
-    uint32_t below = 0, A = found;
+    uint64_t khi = ~0ull;
     if (found > k) {
-        uint64_t ge = __ballot(cum >= k);
-        int ls = __builtin_ctzll(ge);
-        uint32_t hstar = 63u - (uint32_t)ls;
-        A = __shfl(cum, ls, 64);
-        uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
-        khi = (uint64_t)(bT - hstar + 1u) << 54;
-        if (hstar < 63u) {
-            klo = (uint64_t)(bT - hstar) << 54;
-            below = ls > 0 ? prev : 0u;
-        }
-        // Rare: a crowded bin. Split [klo, khi) into 64 sub-ranges until the
-        // kept set fits the buffer (terminates: keys are unique).
-        while (A > (uint32_t)kCap) {
-            uint64_t width = khi - klo;
-            int s = width <= 64 ? 0 : 64 - __builtin_clzll(width - 1) - 6;
-            L.hist[lane] = 0;
-            wave_sync();
-            const uint64_t lo_k = klo, hi_k = khi;
-            for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
-                if (valid && d2 <= T) {
-                    uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | slot;
-                    if (key >= lo_k && key < hi_k) atomicAdd(&L.hist[(uint32_t)((key - lo_k) >> s)], 1u);
-                }
-            });
-            wave_sync();
-            uint32_t cc = L.hist[lane];
-            uint32_t cu = wave_incl_scan(cc, lane);
-            uint32_t need = k - below;
-            uint64_t g2 = __ballot(cu >= need);
-            int js = __builtin_ctzll(g2);
-            uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
-            uint32_t at = __shfl(cu, js, 64);
-            uint64_t nlo = klo + ((uint64_t)js << s);
-            uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
-            if (nhi > khi) nhi = khi;
-            A = below + at;
-            below = below + (js > 0 ? pv : 0u);
-            klo = nlo;
-            khi = nhi;
-        }
+        int ls = __builtin_ctzll(__ballot(cum >= k));
+        khi = (uint64_t)(bT - (63u - (uint32_t)ls) + 1u) << 54;
     }
-
-    // Pass 2: compact keys below khi into the LDS buffer.
+    // collect; if the bin edge keeps more than CAP keys, tighten on the fly
     uint32_t cnt = 0;
-    const uint64_t kh = khi;
-    for_each_candidate(P, L, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
-        uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | slot;
-        bool acc = valid && d2 <= T && key < kh;
+    walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+        uint64_t key = make_key(d2, slot);
+        bool acc = valid && d2 <= T && key < khi;
         uint64_t bal = __ballot(acc);
-        if (acc) {
-            uint32_t pos = cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-            L.buf[pos] = key;
+        uint32_t na = (uint32_t)__popcll(bal);
+        if (cnt + na > (uint32_t)CAP) {
+            wave_sync();
+            cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, bT, khi);
+            acc = acc && key < khi;
+            bal = __ballot(acc);
+            na = (uint32_t)__popcll(bal);
         }
-        cnt += (uint32_t)__popcll(bal);
+        if (acc) L.buf[cnt + mbcnt(bal)] = key;
+        cnt += na;
     });
     wave_sync();
-
-    if (cnt <= 64)
-        sort_and_emit<1>(P, L, lane, q, cnt, m);
-    else if (cnt <= 128)
-        sort_and_emit<2>(P, L, lane, q, cnt, m);
-    else if (cnt <= 256)
-        sort_and_emit<4>(P, L, lane, q, cnt, m);
-    else
-        sort_and_emit<8>(P, L, lane, q, cnt, m);
+    finish<CAP>(P, L, lane, q, cnt, found, khi);
 }
 
 // Radiance from a stored kNN result (pm_radiance): one wave per query.
@@ -518,28 +648,49 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     return PM_OK;
 }
 
+pm_status query_order(pm_ctx* ctx, const pm_grid* grid, const KnnParams& P, uint32_t* order_out) {
+    size_t rs = pm::radix_sort_scratch_bytes(P.nq);
+    void* work = nullptr;
+    pm_status st = pm::scratch_reserve(ctx, pm::carve_size({(size_t)P.nq * 4, (size_t)P.nq * 4, (size_t)P.nq * 4, rs}),
+                                       &work);
+    if (st != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    uint32_t* keys = cv.take<uint32_t>(P.nq);
+    uint32_t* skeys = cv.take<uint32_t>(P.nq);
+    uint32_t* own = cv.take<uint32_t>(P.nq);
+    uint32_t* order = order_out ? order_out : own;
+    void* rw = cv.take<char>(rs);
+    hipLaunchKernelGGL(k_query_keys, dim3(pm::div_up(P.nq, 256)), dim3(256), 0, ctx->stream, P.G, P.qx, P.qy, P.qz,
+                       P.nq, keys);
+    PM_CHECK_LAUNCH(ctx, "k_query_keys");
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < (uint64_t)grid->desc.n_cells) ++bits;
+    return pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, P.nq, bits, rw);
+}
+
 pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags) {
     if (P.nq == 0) return PM_OK;
     if (flags & PM_KNN_SORT_QUERIES) {
+        // the order lives in the scratch block carved by query_order
         size_t rs = pm::radix_sort_scratch_bytes(P.nq);
         void* work = nullptr;
         pm_status st = pm::scratch_reserve(ctx, pm::carve_size({(size_t)P.nq * 4, (size_t)P.nq * 4, (size_t)P.nq * 4, rs}),
                                            &work);
         if (st != PM_OK) return st;
         pm::Carve cv{static_cast<char*>(work)};
-        uint32_t* keys = cv.take<uint32_t>(P.nq);
-        uint32_t* skeys = cv.take<uint32_t>(P.nq);
+        cv.take<uint32_t>(P.nq);
+        cv.take<uint32_t>(P.nq);
         uint32_t* order = cv.take<uint32_t>(P.nq);
-        void* rw = cv.take<char>(rs);
-        hipLaunchKernelGGL(k_query_keys, dim3(pm::div_up(P.nq, 256)), dim3(256), 0, ctx->stream, P.G, P.qx, P.qy, P.qz,
-                           P.nq, keys);
-        PM_CHECK_LAUNCH(ctx, "k_query_keys");
-        int bits = 1;
-        while (bits < 32 && (1ull << bits) < (uint64_t)grid->desc.n_cells) ++bits;
-        if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, P.nq, bits, rw)) != PM_OK) return st;
+        if ((st = query_order(ctx, grid, P, order)) != PM_OK) return st;
         P.order = order;
     }
-    hipLaunchKernelGGL(k_knn, dim3(pm::div_up(P.nq, kWaves)), dim3(256), 0, ctx->stream, P);
+    const dim3 grid_dim(pm::div_up(P.nq, kWaves)), block(256);
+    if (flags & PM_KNN_TWO_PASS)
+        hipLaunchKernelGGL(k_knn_two_pass, grid_dim, block, 0, ctx->stream, P);
+    else if (flags & PM_KNN_SMALL_BUFFER)
+        hipLaunchKernelGGL(k_knn_stream<512>, grid_dim, block, 0, ctx->stream, P);
+    else
+        hipLaunchKernelGGL(k_knn_stream<1024>, grid_dim, block, 0, ctx->stream, P);
     PM_CHECK_LAUNCH(ctx, "k_knn");
     return PM_OK;
 }
@@ -620,4 +771,15 @@ extern "C" pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const f
     if (h_sum_cells) *h_sum_cells = h[0];
     if (h_sum_cands) *h_sum_cands = h[1];
     return PM_OK;
+}
+
+extern "C" pm_status pm_grid_query_order(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
+                                         uint32_t* d_order) {
+    if (!ctx) return PM_ERR_INVALID;
+    KnnParams P;
+    pm_status st = setup(ctx, grid, d_queries3, nq, 0.f, 1, 0, P, "pm_grid_query_order");
+    if (st != PM_OK) return st;
+    if (nq == 0) return PM_OK;
+    if (!d_order) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_query_order: null output");
+    return query_order(ctx, grid, P, d_order);
 }

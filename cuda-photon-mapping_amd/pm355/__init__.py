@@ -29,6 +29,9 @@ LAYOUT_LINEAR = 0
 LAYOUT_MORTON = 1
 KNN_SORT_QUERIES = 1
 KNN_NO_XCD_REMAP = 2
+KNN_TWO_PASS = 1 << 4
+KNN_SMALL_BUFFER = 1 << 5
+KNN_VARIANTS = {"stream1024": 0, "stream512": KNN_SMALL_BUFFER, "two_pass": KNN_TWO_PASS}
 
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -72,8 +75,25 @@ _SIGS = {
     "pm_grid_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _U32, _P, _P, _P]),
     "pm_radiance": (_I32, [_P, _P, _P, _P, _P, _P, _U32, _I32, _P]),
     "pm_grid_gather_radiance": (_I32, [_P, _P, _P, _P, _U32, _F32, _I32, _U32, _P, _P]),
+    "pm_grid_query_order": (_I32, [_P, _P, _P, _U32, _P]),
     "pm_grid_box_stats": (_I32, [_P, _P, _P, _U32, _F32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pm_kdtree_build": (_I32, [_P, _P, _U32, C.POINTER(_P)]),
+    "pm_kdtree_describe": (_I32, [_P, _P]),
+    "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
+    "pm_kdtree_destroy": (_I32, [_P]),
+    "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
 }
+
+
+class KdTreeDesc(C.Structure):
+    _fields_ = [
+        ("n_photons", C.c_uint32),
+        ("n_slots", C.c_uint32),
+        ("levels", C.c_uint32),
+        ("d_node_idx", C.c_void_p),
+        ("d_node_plane", C.c_void_p),
+        ("d_photons", C.c_void_p),
+    ]
 
 _lib = None
 
@@ -209,6 +229,32 @@ class Context:
                                                     int(flags), _ptr(rgb), _ptr(cnt)))
         return (rgb, cnt) if want_count else rgb
 
+    # -- kd-tree / model AABB ---------------------------------------------------
+    def kdtree_build(self, photons9) -> "KdTree":
+        ph = _f32_dev(photons9, self.device)
+        assert ph.dim() == 2 and ph.shape[0] == 9
+        t = _P()
+        self.check(self.lib.pm_kdtree_build(self.h, _ptr(ph), ph.shape[1], C.byref(t)))
+        return KdTree(self, t, keep=ph)
+
+    def model_aabb(self, tris9, tri_begin):
+        import torch
+
+        tr = _f32_dev(tris9, self.device)
+        tb = torch.from_numpy(np.ascontiguousarray(tri_begin, dtype=np.uint32).view(np.int32)).to(self.device)
+        nm = tb.numel() - 1
+        out = torch.empty((nm, 6), dtype=torch.float32, device=self.device)
+        self.check(self.lib.pm_model_aabb(self.h, _ptr(tr), _ptr(tb), nm, _ptr(out)))
+        return out
+
+    def query_order(self, grid: "Grid", queries3):
+        import torch
+
+        q = _f32_dev(queries3, self.device)
+        order = torch.empty((q.shape[1],), dtype=torch.int32, device=self.device)
+        self.check(self.lib.pm_grid_query_order(self.h, grid.h, _ptr(q), q.shape[1], _ptr(order)))
+        return order
+
     def box_stats(self, grid: "Grid", queries3, radius: float):
         q = _f32_dev(queries3, self.device)
         c = C.c_uint64()
@@ -251,6 +297,34 @@ class Grid:
     def close(self):
         if self.h:
             self.ctx.lib.pm_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KdTree:
+    def __init__(self, ctx: Context, h, keep=None):
+        self.ctx = ctx
+        self.h = h
+        self._keep = keep
+        d = KdTreeDesc()
+        ctx.check(ctx.lib.pm_kdtree_describe(h, C.byref(d)))
+        self.desc = d
+
+    def to_host(self):
+        d = self.desc
+        ni = np.empty(d.n_slots, np.uint32)
+        npl = np.empty(d.n_slots, np.int8)
+        self.ctx.check(self.ctx.lib.pm_kdtree_copy_to_host(self.ctx.h, self.h, ni.ctypes.data, npl.ctypes.data))
+        return ni, npl
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.pm_kdtree_destroy(self.h)
             self.h = None
 
     def __del__(self):

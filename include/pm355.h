@@ -46,7 +46,9 @@ enum { PM_LAYOUT_LINEAR = 0, PM_LAYOUT_MORTON = 1 };
 /* pm_grid_knn / pm_grid_gather_radiance flags */
 enum {
     PM_KNN_SORT_QUERIES = 1u << 0, /* process queries in cell order (outputs stay in caller order) */
-    PM_KNN_NO_XCD_REMAP = 1u << 1  /* disable the XCD-aware block mapping */
+    PM_KNN_NO_XCD_REMAP = 1u << 1, /* disable the XCD-aware block mapping */
+    PM_KNN_TWO_PASS = 1u << 4,     /* tuning: histogram pass + collect pass instead of one streaming pass */
+    PM_KNN_SMALL_BUFFER = 1u << 5  /* tuning: 512-key LDS buffer per query (higher occupancy) */
 };
 
 typedef struct pm_ctx pm_ctx;
@@ -121,10 +123,46 @@ pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, const float*
                                   const float* d_normals3, uint32_t nq, float radius, int32_t k, uint32_t flags,
                                   float* d_rgb3, uint32_t* d_count);
 
+/* Cell-coherent processing order of a query block: d_order[i] = index of the
+ * i-th query when queries are sorted (stably) by grid cell. Gathering the
+ * queries in this order lets neighbouring waves share cells in cache; the
+ * renderer's eye rays have this coherence naturally, the reference gets it
+ * from its pixel order (Raytracer.cuh:267). */
+pm_status pm_grid_query_order(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
+                              uint32_t* d_order);
+
 /* Per-query candidate statistics of the gather box (cells visited C_q and
  * photons in them P_q, summed), used for the algorithmic-byte count. */
 pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
                             float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands);
+
+/* ---- photon kd-tree: cpm::PhotonMap(Type, Photon*, size_t) ->
+ * PhotonMap::fill_balanced (PhotonMap.cuh:189-283, 373-375). Same split
+ * rule: plane = argmax extent (lowest axis on ties), median at
+ * mid = from + (to-from)/2 of the segment sorted on that plane, children in
+ * heap slots 2i+1 / 2i+2. The sort order is made total with the photon
+ * index as tie-break. n_slots = 2^L - 1 with L = ceil(log2(n+1)), so no node
+ * is dropped (the reference's Tree drops slots >= n, Tree.cuh:10-13). */
+typedef struct {
+    uint32_t n_photons;
+    uint32_t n_slots;
+    uint32_t levels;
+    const uint32_t* d_node_idx; /* [n_slots] input photon index, 0xFFFFFFFF = empty */
+    const int8_t* d_node_plane; /* [n_slots] split axis 0..2, -1 = empty */
+    const float* d_photons;     /* photons9 in slot order, [9][n_slots] */
+} pm_kdtree_desc;
+
+pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, pm_kdtree** out);
+pm_status pm_kdtree_describe(const pm_kdtree* tree, pm_kdtree_desc* out);
+pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* tree, uint32_t* h_node_idx, int8_t* h_node_plane);
+pm_status pm_kdtree_destroy(pm_kdtree* tree);
+
+/* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
+ * only live effect is the model's bounding box (min/max over its vertices).
+ * d_tris9: one triangle per 9 floats (v0.xyz v1.xyz v2.xyz); model m spans
+ * triangles [d_tri_begin[m], d_tri_begin[m+1]). d_aabb6: [n_models][6]. */
+pm_status pm_model_aabb(pm_ctx* ctx, const float* d_tris9, const uint32_t* d_tri_begin, int32_t n_models,
+                        float* d_aabb6);
 
 #ifdef __cplusplus
 }

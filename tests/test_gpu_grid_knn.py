@@ -127,10 +127,11 @@ def test_knn_parity(ctx, cell, r, k, layout):
     g = ctx.grid_build(ph, aabb, cell_size=cell, layout=layout)
     og = O.OracleGrid(ph, aabb, cell_size=cell, layout=layout)
     oi, od, oc = og.knn(q, r, k)
-    for flags in (0, pm355.KNN_SORT_QUERIES):
-        gi, gd, gc = ctx.knn(g, q, r, k, flags=flags)
-        ctx.sync()
-        _compare_knn(gi, gd, gc, oi, od, oc)
+    for variant in pm355.KNN_VARIANTS.values():
+        for flags in (0, pm355.KNN_SORT_QUERIES, pm355.KNN_NO_XCD_REMAP):
+            gi, gd, gc = ctx.knn(g, q, r, k, flags=flags | variant)
+            ctx.sync()
+            _compare_knn(gi, gd, gc, oi, od, oc)
     assert oc[1] == 0
 
 
@@ -148,9 +149,10 @@ def test_knn_duplicates_force_refinement(ctx):
     og = O.OracleGrid(ph, (0, 0, 0, 1, 1, 1), cell_size=0.05)
     for k in (1, 64, 200, 256):
         oi, od, oc = og.knn(q, 0.05, k)
-        gi, gd, gc = ctx.knn(g, q, 0.05, k)
-        ctx.sync()
-        _compare_knn(gi, gd, gc, oi, od, oc)
+        for variant in pm355.KNN_VARIANTS.values():
+            gi, gd, gc = ctx.knn(g, q, 0.05, k, flags=variant)
+            ctx.sync()
+            _compare_knn(gi, gd, gc, oi, od, oc)
 
 
 def test_knn_matches_brute_force_small(ctx):
@@ -186,6 +188,17 @@ def test_radiance_parity(ctx, cell, r, k):
         rel[o64 == 0] = np.abs(got[o64 == 0])
         assert rel.max() <= RADIANCE_RTOL, rel.max()
     assert np.array_equal(_np(rgb_fused), _np(rgb_sorted))
+
+
+def test_query_order_is_stable_cell_sort(ctx):
+    ph, q, qn, aabb = gather_microbench(seed=6, n_photons=50_000, n_queries=20_000, n_clusters=4)
+    g = ctx.grid_build(ph, aabb, cell_size=0.02)
+    order = _np(ctx.query_order(g, q), np.uint32)
+    og = O.OracleGrid(ph, aabb, cell_size=0.02)
+    from test_oracle import _numpy_cell_ids
+
+    cid = _numpy_cell_ids(q, og.aabb, og.cell, og.dims)
+    assert np.array_equal(order, np.argsort(cid, kind="stable"))
 
 
 def test_box_stats_match_oracle(ctx):
