@@ -166,6 +166,15 @@ def main():
     alg_bytes = nq * (12 + 12 + 12) + 8 * cells + 12 * cands + 24 * kept
     achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
 
+    # HBM bytes per dispatch of this kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary (scaled by a 1 GiB copy probe).
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_hbm_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("variant") == best:
+            traffic = t["hbm_read_bytes_per_dispatch"] + t["hbm_write_bytes_per_dispatch"]
+
     value = nq / (ms_per_step * 1e-3)
     res = {
         "metric": METRIC,
@@ -196,7 +205,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": f"k_knn_{best} (fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
             "alg_bytes": alg_bytes,

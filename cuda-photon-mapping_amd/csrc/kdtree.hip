@@ -376,6 +376,15 @@ extern "C" pm_status pm_kdtree_destroy(pm_kdtree* t) {
     return PM_OK;
 }
 
+namespace pm {
+pm_status model_aabb_launch(pm_ctx* ctx, const float* tris9, const uint32_t* tb, int n_models, float* out6) {
+    if (n_models <= 0) return PM_OK;
+    hipLaunchKernelGGL(k_model_aabb, dim3(n_models), dim3(256), 0, ctx->stream, tris9, tb, out6);
+    PM_CHECK_LAUNCH(ctx, "k_model_aabb");
+    return PM_OK;
+}
+}  // namespace pm
+
 extern "C" pm_status pm_model_aabb(pm_ctx* ctx, const float* d_tris9, const uint32_t* d_tri_begin, int32_t n_models,
                                    float* d_aabb6) {
     if (!ctx) return PM_ERR_INVALID;

@@ -1,0 +1,190 @@
+// Device geometry shared by photon tracing and the eye-ray renderer: FP32
+// vector helpers in the reference's host evaluation order (the library is
+// built with -ffp-contract=off, and division / sqrtf are IEEE correctly
+// rounded), ray-triangle and ray-box tests, the scene hit, and the per-path
+// Philox4x32-10 stream.
+#pragma once
+#include <cfloat>
+
+#include "pm_internal.hpp"
+
+namespace pmg {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, float f) { return V3{a.x * f, a.y * f, a.z * f}; }
+__device__ __forceinline__ V3 mulv(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+// vec3::dot host branch (vec3.cuh:258-274)
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// vec3::cross host branch (vec3.cuh:276-300)
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ bool is_null(V3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+// static vec3::normalize: division by the length (vec3.cuh:226-240)
+__device__ __forceinline__ V3 normalize_div(V3 a) {
+    if (is_null(a)) return a;
+    float l = length(a);
+    return V3{a.x / l, a.y / l, a.z / l};
+}
+// member vec3::normalize: multiply by 1/length (vec3.cuh:211-224, :156-169)
+__device__ __forceinline__ V3 normalize_rcp(V3 a) {
+    if (is_null(a)) return a;
+    float u = 1.0f / length(a);
+    return V3{a.x * u, a.y * u, a.z * u};
+}
+__device__ __forceinline__ float maxc(V3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+
+struct Ray {
+    V3 o, d;
+};
+
+// Per-model record on the device (material of Material.cuh:4-31, the
+// triangle range of the model, its AABB and id).
+struct DevModel {
+    uint32_t t0, t1;
+    float kd[3], ks[3], ke[3];
+    float ior, opaque, shininess;
+    int32_t id, pad;
+    float box[6];
+};
+
+__device__ __forceinline__ V3 load_v(const float* __restrict__ t9, uint32_t t, int k) {
+    const float* p = t9 + 9 * (size_t)t + 3 * k;
+    return v3(p[0], p[1], p[2]);
+}
+
+// AABB::intersects_with_ray (AABB.cuh:103-111, update_tmin_tmax :115-127)
+__device__ __forceinline__ bool box_hit(const float* c, const Ray& r) {
+    float tmin = 0.0f, tmax = __builtin_inff();
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        bool sign = __builtin_signbit(d[i]);
+        float bmin = sign ? c[3 + i] : c[i];
+        float bmax = sign ? c[i] : c[3 + i];
+        float dmin = (bmin - o[i]) / d[i];
+        float dmax = (bmax - o[i]) / d[i];
+        tmin = fmaxf(dmin, tmin);
+        tmax = fminf(dmax, tmax);
+    }
+    return tmin <= tmax;
+}
+
+// traingle_intersection (Model.cuh:69-120)
+__device__ __forceinline__ bool tri_hit(const Ray& r, V3 v0, V3 v1, V3 v2, float& t, V3& uvw) {
+    V3 N = cross(sub(v1, v0), sub(v2, v0));
+    float denom = 1.0f / dot(N, N);
+    float nd = dot(N, r.d);
+    if (fabsf(nd) < 1e-6f) return false;
+    t = (dot(N, v0) - dot(N, r.o)) / nd;
+    if (t < 0) return false;
+    V3 P = add(r.o, mul(r.d, t));
+    float w = dot(N, cross(sub(v1, v0), sub(P, v0)));
+    if (w < 0) return false;
+    float v = dot(N, cross(sub(v0, v2), sub(P, v2)));
+    if (v < 0) return false;
+    w *= denom;
+    v *= denom;
+    float u = 1.f - v - w;
+    if (u < 0) return false;
+    uvw = v3(u, v, w);
+    return true;
+}
+
+// Nearest hit over all models (find_intersection, PhotonTracing.cu:178-213 /
+// Raytracer.cuh:66-93) with Model::intersection (Model.cuh:366-446): per
+// model an AABB test, every triangle in order keeping the first of equal t,
+// normal = normalised barycentric blend of the vertex normals; a model's hit
+// replaces the current one if the current t is 0 or the new t is smaller.
+// Triangle data is read with wave-uniform indices (scalar loads).
+__device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_models,
+                                          const float* __restrict__ t9, const float* __restrict__ n9, const Ray& r,
+                                          float& t_out, V3& n_out, int& m_out) {
+    float best = 0.f;
+    int bm = -1;
+    V3 bn = v3(0, 0, 0);
+    for (int m = 0; m < n_models; ++m) {
+        const bool in_box = box_hit(M[m].box, r);
+        if (!__builtin_amdgcn_ballot_w64(in_box)) continue;
+        float mt = FLT_MAX;
+        bool found = false;
+        V3 uvw = v3(0, 0, 0);
+        uint32_t ti = 0;
+        const uint32_t t0 = M[m].t0, t1 = M[m].t1;
+        for (uint32_t t = t0; t < t1; ++t) {
+            float pt;
+            V3 pu;
+            if (in_box && tri_hit(r, load_v(t9, t, 0), load_v(t9, t, 1), load_v(t9, t, 2), pt, pu) && pt < mt) {
+                mt = pt;
+                uvw = pu;
+                ti = t;
+                found = true;
+            }
+        }
+        if (!found) continue;
+        V3 n = add(add(mul(load_v(n9, ti, 0), uvw.x), mul(load_v(n9, ti, 1), uvw.y)), mul(load_v(n9, ti, 2), uvw.z));
+        n = normalize_rcp(n);
+        if (bm < 0 || best == 0.f || mt < best) {
+            best = mt;
+            bm = m;
+            bn = n;
+        }
+    }
+    if (bm < 0) return false;
+    t_out = best;
+    n_out = bn;
+    m_out = bm;
+    return true;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11); one stream per
+// (seed, stream id, path id), four uniforms per block, u in (0, 1].
+struct Stream {
+    uint32_t k0, k1, path, sid, blk, pos;
+    uint32_t buf[4];
+    __device__ __forceinline__ void init(uint64_t seed, uint32_t stream_id, uint32_t path_id) {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        path = path_id;
+        sid = stream_id;
+        blk = 0;
+        pos = 4;
+    }
+    __device__ __forceinline__ void refill() {
+        uint32_t c0 = path, c1 = blk, c2 = sid, c3 = 0x7068u;
+        uint32_t a = k0, b = k1;
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+            uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+            uint32_t n0 = hi1 ^ c1 ^ a, n2 = hi0 ^ c3 ^ b;
+            c0 = n0;
+            c1 = lo1;
+            c2 = n2;
+            c3 = lo0;
+            a += 0x9E3779B9u;
+            b += 0xBB67AE85u;
+        }
+        buf[0] = c0;
+        buf[1] = c1;
+        buf[2] = c2;
+        buf[3] = c3;
+        ++blk;
+        pos = 0;
+    }
+    __device__ __forceinline__ float next() {
+        if (pos == 4) refill();
+        uint32_t x = pos == 0 ? buf[0] : pos == 1 ? buf[1] : pos == 2 ? buf[2] : buf[3];
+        ++pos;
+        return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+    }
+};
+
+}  // namespace pmg

@@ -86,6 +86,9 @@ pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
 
+// Per-model AABB over triangle ranges [tb[m], tb[m+1]) (device arrays).
+pm_status model_aabb_launch(pm_ctx* ctx, const float* tris9, const uint32_t* tb, int n_models, float* out6);
+
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace pm

@@ -1,0 +1,410 @@
+// Photon emission and bounce tracing: trace_photons / emit / trace
+// (PhotonTracing.cu:37-55, 178-422).
+//
+// One lane traces one light path. Paths are numbered; path p draws from its
+// own Philox stream (seed, 0, p), so the photons a path stores do not depend
+// on which lane, wave or launch traced it. The global and caustic maps take
+// the first n photons of each kind in (path, event) order: a batch of paths
+// stages its photons per path, two wave64 scans over the per-path counts give
+// each photon its map slot, and a compaction kernel writes the SoA blocks.
+// The reference instead appends with atomicAdd until both maps are full
+// (PhotonArray.cuh:31-46), which makes the stored set launch-dependent.
+//
+// The medium stack of PhotonTracing.cu:127-176 (photon_trace_stack,
+// PhotonTraceStack.cuh:37-116) is a per-lane array of max_bounces entries;
+// its strided global-memory layout is gone.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pm_geom.hpp"
+
+namespace {
+
+using namespace pmg;
+
+constexpr int kMaxBounces = 16;
+enum : uint8_t { DIF = 0, SPEC = 1, ABSORB = 2, REFR = 3, NONE = 4 };
+
+struct CaEntry {
+    float n1, n2, ca;
+};
+
+struct TraceParams {
+    const float* tris;
+    const float* nrms;
+    const DevModel* models;
+    int n_models;
+    int light;
+    float pp[3];
+    float def_ior;
+    const CaEntry* ca;
+    int n_ca;
+    uint64_t seed;
+    int max_b;
+    uint64_t path_base;
+    uint32_t n_paths;
+    float* stage;     // [9][max_b][n_paths]
+    uint8_t* kind;    // [max_b][n_paths]: 0 global, 1 caustic
+    uint32_t* cnt_g;  // [n_paths]
+    uint32_t* cnt_c;  // [n_paths]
+};
+
+struct Medium {
+    float n[kMaxBounces];
+    int id[kMaxBounces];
+    int size;
+};
+
+__device__ __forceinline__ bool ca_lookup(const TraceParams& P, float n1, float n2, float& ca) {
+    for (int i = 0; i < P.n_ca; ++i)
+        if (P.ca[i].n1 == n1 && P.ca[i].n2 == n2) {
+            ca = P.ca[i].ca;
+            return true;
+        }
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n_paths) return;
+    const uint32_t path = (uint32_t)(P.path_base + i);
+    Stream rng;
+    rng.init(P.seed, 0u, path);
+    const DevModel& L = P.models[P.light];
+    const uint32_t lt = L.t1 - L.t0;
+    const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
+
+    // emit (PhotonTracing.cu:345-370), gpu_get_random_point_with_normal
+    // (Model.cuh:202-243): roundf triangle pick, barycentrics x = u,
+    // y = u' (1 - x), origin offset 1e-4 along the (unnormalised) normal.
+    int tsel = (int)roundf(0 + rng.next() * (float)((3 * (int)lt - 3) / 3));
+    uint32_t t = L.t0 + (uint32_t)tsel;
+    float ux = rng.next();
+    float uy = 0.f + rng.next() * ((1.f - ux) - 0.f);
+    float uz = 1.f - ux - uy;
+    V3 point = add(add(mul(load_v(P.tris, t, 0), ux), mul(load_v(P.tris, t, 1), uy)), mul(load_v(P.tris, t, 2), uz));
+    V3 lnrm = add(add(mul(load_v(P.nrms, t, 0), ux), mul(load_v(P.nrms, t, 1), uy)), mul(load_v(P.nrms, t, 2), uz));
+    Ray ray;
+    ray.o = add(point, mul(lnrm, 0.0001f));
+    float prob, dt;
+    do {
+        ray.d.x = -1.f + rng.next() * (1.f - -1.f);
+        ray.d.y = -1.f + rng.next() * (1.f - -1.f);
+        ray.d.z = -1.f + rng.next() * (1.f - -1.f);
+        ray.d = normalize_rcp(ray.d);
+        prob = rng.next();
+        dt = dot(ray.d, lnrm);
+        if (dt < 0) {
+            dt = -dt;
+            ray.d = mul(ray.d, -1.f);
+        }
+    } while (prob > dt);
+
+    Medium ms;  // photon_trace_stack: (refractive index, model id) entries
+    ms.size = 0;
+    int diffuse_surfs = 0;
+    uint8_t last = NONE;
+    uint32_t ng = 0, nc = 0, ev = 0;
+
+    for (int b = 0; b < P.max_b; ++b) {
+        // trace (PhotonTracing.cu:278-326)
+        float th;
+        V3 n;
+        int hm;
+        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, ray, th, n, hm)) break;
+        V3 ip = add(mul(ray.d, th), ray.o);
+        if (dot(ray.d, n) > 0) n = mul(n, -1.f);
+        const DevModel& mo = P.models[hm];
+        const float cosNL = dot(neg(ray.d), n);
+        const float ior = mo.ior;
+        const int mid = mo.id;
+        uint8_t dest = 0xFF;
+        // refract_destity (PhotonTracing.cu:220-243) with get_refraction_pair
+        if (mo.opaque != 1.f) {
+            float n1, n2;
+            if (ms.size == 0) {
+                n1 = P.def_ior;
+                n2 = ior;
+            } else if (ms.id[ms.size - 1] == mid) {
+                n1 = ms.n[ms.size - 1];
+                n2 = ms.size == 1 ? P.def_ior : ms.n[ms.size - 2];
+            } else {
+                n1 = ms.n[ms.size - 1];
+                n2 = ior;
+            }
+            float ca;
+            bool tir = ca_lookup(P, n1, n2, ca) && cosNL <= ca;
+            if (!tir) {
+                float f = (n1 - n2) / (n1 + n2);
+                float f0 = f * f;
+                float x = 1.f - cosNL;
+                float x2 = x * x;
+                float x5 = (x2 * x2) * x;
+                float refr_p = 1.f - (f0 + (1.f - f0) * x5);
+                float e = rng.next();
+                if (!(e > refr_p)) dest = REFR;
+            }
+        }
+        // destiny: Russian roulette with e in [0, 2) (PhotonTracing.cu:245-276)
+        if (dest == 0xFF) {
+            const V3 kd = v3(mo.kd[0], mo.kd[1], mo.kd[2]), ks = v3(mo.ks[0], mo.ks[1], mo.ks[2]);
+            float max_lp = maxc(pp);
+            float e = 0.f + rng.next() * (2.f - 0.f);
+            float pd = maxc(mulv(kd, pp)) / max_lp;
+            if (e <= pd) {
+                dest = DIF;
+            } else {
+                float ps = maxc(mulv(ks, pp)) / max_lp;
+                dest = (e <= pd + ps) ? SPEC : ABSORB;
+            }
+        }
+        Ray nr{v3(0, 0, 0), v3(0, 0, 0)};
+        bool end = false;
+        if (dest == REFR) {
+            // get_update_refraction_pair (PhotonTracing.cu:127-151)
+            float n1, n2;
+            if (ms.size != 0 && ms.id[ms.size - 1] == mid) {
+                n1 = ms.n[ms.size - 1];
+                n2 = ms.size == 1 ? P.def_ior : ms.n[ms.size - 2];
+                --ms.size;
+            } else {
+                n1 = ms.size == 0 ? P.def_ior : ms.n[ms.size - 1];
+                n2 = ior;
+                if (ms.size < P.max_b) {
+                    ms.n[ms.size] = ior;
+                    ms.id[ms.size] = mid;
+                    ++ms.size;
+                }
+            }
+            // Ray::refract (Ray.cuh:85-96)
+            float eta = n1 / n2;
+            float c1 = -dot(ray.d, n);
+            float w = eta * c1;
+            float c2m = (w - eta) * (w + eta);
+            if (c2m < -1.f) {
+                end = true;
+            } else {
+                nr.d = add(mul(ray.d, eta), mul(n, w - sqrtf(1.f + c2m)));
+                nr.o = add(ip, mul(neg(n), 0.01f));
+            }
+        } else {
+            bool ks_one = fabsf(mo.ks[0] - 1.f) < 1e-3f && fabsf(mo.ks[1] - 1.f) < 1e-3f && fabsf(mo.ks[2] - 1.f) < 1e-3f;
+            if (dest == DIF || (dest == ABSORB && !ks_one)) {
+                // PathOperator::response (PathOperator.cuh:28-37)
+                const bool caustic = (last == SPEC || last == REFR) && diffuse_surfs == 0;
+                const float v[9] = {ip.x, ip.y, ip.z, pp.x, pp.y, pp.z, ray.d.x, ray.d.y, ray.d.z};
+#pragma unroll
+                for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + ev) * P.n_paths + i] = v[c];
+                P.kind[(size_t)ev * P.n_paths + i] = caustic ? 1 : 0;
+                ++ev;
+                if (caustic)
+                    ++nc;
+                else
+                    ++ng;
+            } else if (dest == SPEC) {
+                // Ray::reflect (Ray.cuh:64-71)
+                float dnd = dot(ray.d, n);
+                nr.d = normalize_div(sub(ray.d, mul(mul(n, 2.f), dnd)));
+                nr.o = add(ip, mul(n, 0.01f));
+            }
+            if (dest == DIF) {
+                // Ray::reflect_spherical (Ray.cuh:16-56)
+                V3 d;
+                do {
+                    float x1, x2, s1, s2;
+                    do {
+                        x1 = -1.f + rng.next() * (1.f - -1.f);
+                        x2 = -1.f + rng.next() * (1.f - -1.f);
+                        s1 = x1 * x1;
+                        s2 = x2 * x2;
+                    } while (s1 + s2 >= 1);
+                    float f = sqrtf(1.f - s1 - s2);
+                    d = v3(2.f * x1 * f, 2.f * x2 * f, 1.f - 2.f * (s1 + s2));
+                } while (dot(d, n) < 0);
+                nr.d = normalize_div(d);
+                nr.o = add(ip, mul(nr.d, 0.0001f));
+            }
+        }
+        if (dest == DIF) ++diffuse_surfs;
+        last = dest;
+        if (dest == ABSORB || end) break;
+        ray = nr;
+    }
+    P.cnt_g[i] = ng;
+    P.cnt_c[i] = nc;
+}
+
+// Write each staged photon to its map slot: base + exclusive-scan offset of
+// its path + its rank among the path's photons of the same kind.
+__global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__ stage, const uint8_t* __restrict__ kind,
+                                                      const uint32_t* __restrict__ cnt_g,
+                                                      const uint32_t* __restrict__ cnt_c,
+                                                      const uint32_t* __restrict__ off_g,
+                                                      const uint32_t* __restrict__ off_c, uint32_t n_paths, int max_b,
+                                                      uint64_t base_g, uint64_t base_c, float* __restrict__ out_g,
+                                                      uint32_t cap_g, float* __restrict__ out_c, uint32_t cap_c) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths) return;
+    uint32_t ne = cnt_g[i] + cnt_c[i];
+    uint64_t sg = base_g + off_g[i], sc = base_c + off_c[i];
+    for (uint32_t e = 0; e < ne; ++e) {
+        bool c = kind[(size_t)e * n_paths + i] != 0;
+        uint64_t s = c ? sc++ : sg++;
+        uint32_t cap = c ? cap_c : cap_g;
+        float* out = c ? out_c : out_g;
+        if (s < cap) {
+#pragma unroll
+            for (int p = 0; p < 9; ++p) out[(size_t)p * cap + s] = stage[((size_t)p * max_b + e) * n_paths + i];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, uint32_t n_global,
+                                      uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,
+                                      float* d_global9, float* d_caustic9, pm_trace_stats* stats) {
+    if (!ctx || !scene) return PM_ERR_INVALID;
+    if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: empty scene");
+    if (scene->light_model < 0 || scene->light_model >= scene->n_models)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: bad light model");
+    if (max_bounces < 1 || max_bounces > kMaxBounces)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_trace_photons: max_bounces outside [1, %d]", kMaxBounces);
+    if (n_global == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: n_global must be >= 1");
+    if ((n_global && !d_global9) || (n_caustic && !d_caustic9))
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: null output map");
+    const int nm = scene->n_models;
+    std::vector<DevModel> hm(nm);
+    for (int i = 0; i < nm; ++i) {
+        const pm_model& s = scene->h_models[i];
+        if (s.tri_end > scene->n_tris || s.tri_begin > s.tri_end)
+            return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: model %d triangle range", i);
+        DevModel& d = hm[i];
+        d.t0 = s.tri_begin;
+        d.t1 = s.tri_end;
+        for (int c = 0; c < 3; ++c) {
+            d.kd[c] = s.kd[c];
+            d.ks[c] = s.ks[c];
+            d.ke[c] = s.ke[c];
+        }
+        d.ior = s.refr_index;
+        d.opaque = s.opaque;
+        d.shininess = s.shininess;
+        d.id = s.id;
+        d.pad = 0;
+    }
+    if (hm[scene->light_model].t1 == hm[scene->light_model].t0)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: light model has no triangles");
+    // compute_critical_angles (PhotonTracing.cu:71-125): cos(asin(n2/n1)) for
+    // every ordered pair with n2/n1 < 1, against each other and the default.
+    std::vector<CaEntry> ca;
+    auto add_pair = [&](float e1, float e2) {
+        float eta = e2 / e1;
+        if (!(eta < 1.f)) return;
+        for (auto& c : ca)
+            if (c.n1 == e1 && c.n2 == e2) return;
+        ca.push_back(CaEntry{e1, e2, std::cos(std::asin(eta))});
+    };
+    for (int i = 0; i + 1 < nm; ++i)
+        for (int j = i + 1; j < nm; ++j) {
+            add_pair(hm[i].ior, hm[j].ior);
+            add_pair(hm[j].ior, hm[i].ior);
+        }
+    for (int i = 0; i < nm; ++i) {
+        add_pair(scene->default_refr_index, hm[i].ior);
+        add_pair(hm[i].ior, scene->default_refr_index);
+    }
+
+    uint64_t want = (uint64_t)n_global + n_caustic;
+    uint32_t batch = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 65536), 4u << 20);
+    if (max_paths < batch) batch = (uint32_t)std::max<uint64_t>(max_paths, 1);
+    const int B = max_bounces;
+    size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
+                                      (size_t)nm * 8, (size_t)nm * 24, (size_t)batch * B * 36,
+                                      (size_t)batch * B, (size_t)batch * 4 * 4, 64});
+    void* work = nullptr;
+    pm_status st = pm::scratch_reserve(ctx, rs_bytes, &work);
+    if (st != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    DevModel* d_models = cv.take<DevModel>(nm);
+    CaEntry* d_ca = cv.take<CaEntry>(ca.size() + 1);
+    uint32_t* d_tb = cv.take<uint32_t>(2 * (size_t)nm);
+    float* d_box = cv.take<float>((size_t)nm * 6);
+    float* stage = cv.take<float>((size_t)batch * B * 9);
+    uint8_t* kind = cv.take<uint8_t>((size_t)batch * B);
+    uint32_t* cnt_g = cv.take<uint32_t>(batch);
+    uint32_t* cnt_c = cv.take<uint32_t>(batch);
+    uint32_t* off_g = cv.take<uint32_t>(batch);
+    uint32_t* off_c = cv.take<uint32_t>(batch);
+    uint64_t* tot = cv.take<uint64_t>(2);
+
+    // model AABBs on the device (the live effect of create_kd_tree): one
+    // reduction per model over its own triangle range
+    std::vector<uint32_t> tb(2 * (size_t)nm);
+    for (int i = 0; i < nm; ++i) {
+        tb[2 * i] = hm[i].t0;
+        tb[2 * i + 1] = hm[i].t1;
+    }
+    PM_HIP(ctx, hipMemcpyAsync(d_tb, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    for (int i = 0; i < nm; ++i)
+        if ((st = pm::model_aabb_launch(ctx, scene->d_tris9, d_tb + 2 * i, 1, d_box + 6 * i)) != PM_OK) return st;
+    std::vector<float> boxes((size_t)nm * 6);
+    PM_HIP(ctx, hipMemcpyAsync(boxes.data(), d_box, boxes.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nm; ++i)
+        for (int c = 0; c < 6; ++c) hm[i].box[c] = boxes[(size_t)i * 6 + c];
+    PM_HIP(ctx, hipMemcpyAsync(d_models, hm.data(), nm * sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    if (!ca.empty())
+        PM_HIP(ctx, hipMemcpyAsync(d_ca, ca.data(), ca.size() * sizeof(CaEntry), hipMemcpyHostToDevice, ctx->stream));
+
+    TraceParams P{};
+    P.tris = scene->d_tris9;
+    P.nrms = scene->d_normals9;
+    P.models = d_models;
+    P.n_models = nm;
+    P.light = scene->light_model;
+    for (int c = 0; c < 3; ++c) P.pp[c] = scene->light_intensity[c] / (float)n_global;
+    P.def_ior = scene->default_refr_index;
+    P.ca = d_ca;
+    P.n_ca = (int)ca.size();
+    P.seed = seed;
+    P.max_b = B;
+    P.stage = stage;
+    P.kind = kind;
+    P.cnt_g = cnt_g;
+    P.cnt_c = cnt_c;
+
+    uint64_t paths = 0, got_g = 0, got_c = 0;
+    while ((got_g < n_global || got_c < n_caustic) && paths < max_paths) {
+        uint32_t nb = (uint32_t)std::min<uint64_t>(batch, max_paths - paths);
+        P.path_base = paths;
+        P.n_paths = nb;
+        hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P);
+        PM_CHECK_LAUNCH(ctx, "k_trace_paths");
+        if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
+        if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
+        hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind, cnt_g,
+                           cnt_c, off_g, off_c, nb, B, got_g, got_c, d_global9, n_global, d_caustic9, n_caustic);
+        PM_CHECK_LAUNCH(ctx, "k_trace_compact");
+        uint64_t h[2];
+        PM_HIP(ctx, hipMemcpyAsync(h, tot, 16, hipMemcpyDeviceToHost, ctx->stream));
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        got_g += h[0];
+        got_c += h[1];
+        paths += nb;
+        // size the next batch for the map that is still short
+        double rate_g = (double)got_g / (double)paths, rate_c = (double)got_c / (double)paths;
+        double need = 0;
+        if (got_g < n_global) need = std::max(need, (n_global - got_g) / std::max(rate_g, 1e-9));
+        if (got_c < n_caustic) need = std::max(need, (n_caustic - got_c) / std::max(rate_c, 1e-9));
+        batch = (uint32_t)std::min<double>(std::max<double>(need * 1.05 + 1024, 65536), (double)batch);
+    }
+    if (stats) {
+        stats->paths = paths;
+        stats->n_global = (uint32_t)std::min<uint64_t>(got_g, n_global);
+        stats->n_caustic = (uint32_t)std::min<uint64_t>(got_c, n_caustic);
+    }
+    return PM_OK;
+}

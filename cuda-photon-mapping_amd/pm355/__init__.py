@@ -82,7 +82,24 @@ _SIGS = {
     "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
     "pm_kdtree_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
+    "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
 }
+
+
+class ModelRec(C.Structure):
+    _fields_ = [("tri_begin", C.c_uint32), ("tri_end", C.c_uint32), ("kd", C.c_float * 3), ("ks", C.c_float * 3),
+                ("ke", C.c_float * 3), ("refr_index", C.c_float), ("opaque", C.c_float), ("shininess", C.c_float),
+                ("id", C.c_int32), ("pad", C.c_int32), ("aabb", C.c_float * 6)]
+
+
+class SceneRec(C.Structure):
+    _fields_ = [("d_tris9", C.c_void_p), ("d_normals9", C.c_void_p), ("n_tris", C.c_uint32),
+                ("h_models", C.c_void_p), ("n_models", C.c_int32), ("light_model", C.c_int32),
+                ("light_intensity", C.c_float * 3), ("default_refr_index", C.c_float)]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("n_global", C.c_uint32), ("n_caustic", C.c_uint32)]
 
 
 class KdTreeDesc(C.Structure):
@@ -255,6 +272,21 @@ class Context:
         self.check(self.lib.pm_grid_query_order(self.h, grid.h, _ptr(q), q.shape[1], _ptr(order)))
         return order
 
+    # -- scene / photon tracing ------------------------------------------------
+    def upload_scene(self, scene) -> "DeviceScene":
+        return DeviceScene(self, scene)
+
+    def trace_photons(self, dscene: "DeviceScene", seed: int, n_global: int, n_caustic: int,
+                      max_bounces: int = 10, max_paths: int = 1 << 40):
+        import torch
+
+        g = torch.empty((9, max(n_global, 1)), dtype=torch.float32, device=self.device)
+        c = torch.empty((9, max(n_caustic, 1)), dtype=torch.float32, device=self.device)
+        st = TraceStats()
+        self.check(self.lib.pm_trace_photons(self.h, C.byref(dscene.rec), int(seed), int(n_global), int(n_caustic),
+                                             int(max_bounces), int(max_paths), _ptr(g), _ptr(c), C.byref(st)))
+        return g[:, :n_global], c[:, :n_caustic], (int(st.paths), int(st.n_global), int(st.n_caustic))
+
     def box_stats(self, grid: "Grid", queries3, radius: float):
         q = _f32_dev(queries3, self.device)
         c = C.c_uint64()
@@ -304,6 +336,30 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+class DeviceScene:
+    """A scene's triangles and normals resident in HBM plus its model table."""
+
+    def __init__(self, ctx: Context, scene):
+        import torch
+
+        self.scene = scene
+        self.tris = torch.from_numpy(np.ascontiguousarray(scene.tris9, np.float32)).to(ctx.device)
+        self.nrms = torch.from_numpy(np.ascontiguousarray(scene.normals9, np.float32)).to(ctx.device)
+        self.models = np.ascontiguousarray(scene.models)
+        assert self.models.dtype.itemsize == C.sizeof(ModelRec)
+        r = SceneRec()
+        r.d_tris9 = self.tris.data_ptr()
+        r.d_normals9 = self.nrms.data_ptr()
+        r.n_tris = scene.tris9.shape[0]
+        r.h_models = self.models.ctypes.data
+        r.n_models = len(self.models)
+        r.light_model = int(scene.light_model)
+        for i in range(3):
+            r.light_intensity[i] = float(scene.light_intensity[i])
+        r.default_refr_index = float(scene.default_refr_index)
+        self.rec = r
 
 
 class KdTree:

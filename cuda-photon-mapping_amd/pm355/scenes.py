@@ -1,0 +1,163 @@
+"""Synthetic scenes for the photon pipeline (host numpy arrays).
+
+Scene arrays follow the pm355 C ABI (include/pm355.h):
+  tris9    float32[T, 9]   triangle vertices v0.xyz v1.xyz v2.xyz
+  normals9 float32[T, 9]   per-vertex normals n0 n1 n2
+  models   PM_MODEL_DTYPE[M] (tri range, material, id)
+A model is a contiguous range of triangles with one material, like the
+reference's Model + Material (Model.cuh:24-30, Material.cuh:4-31).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PM_MODEL_DTYPE = np.dtype([
+    ("tri_begin", "<u4"), ("tri_end", "<u4"),
+    ("kd", "<f4", 3), ("ks", "<f4", 3), ("ke", "<f4", 3),
+    ("refr_index", "<f4"), ("opaque", "<f4"), ("shininess", "<f4"),
+    ("id", "<i4"), ("pad", "<i4"),
+    ("aabb", "<f4", 6),
+])
+
+PM_CAMERA_DTYPE = np.dtype([
+    ("position", "<f4", 3), ("forward", "<f4", 3), ("right", "<f4", 3), ("up", "<f4", 3),
+    ("viewport_width", "<f4"), ("viewport_height", "<f4"), ("focal_length", "<f4"),
+    ("width", "<i4"), ("height", "<i4"), ("pad", "<i4"),
+])
+
+
+@dataclass
+class Scene:
+    tris9: np.ndarray
+    normals9: np.ndarray
+    models: np.ndarray
+    light_model: int
+    light_intensity: tuple = (1.0, 1.0, 1.0)
+    default_refr_index: float = 1.0
+    names: list = field(default_factory=list)
+
+    @property
+    def tri_begin(self) -> np.ndarray:
+        return np.concatenate([self.models["tri_begin"], self.models["tri_end"][-1:]]).astype(np.uint32)
+
+    def aabb(self) -> tuple:
+        """Scene AABB = union of model AABBs (Initializer.cu:197-200)."""
+        v = self.tris9.reshape(-1, 3)
+        return tuple(v.min(0).tolist()) + tuple(v.max(0).tolist())
+
+
+def _quad(p0, p1, p2, p3, n):
+    """Two triangles (p0 p1 p2), (p0 p2 p3) with a flat normal."""
+    t = np.array([np.concatenate([p0, p1, p2]), np.concatenate([p0, p2, p3])], np.float32)
+    nn = np.array([np.tile(n, 3), np.tile(n, 3)], np.float32)
+    return t, nn
+
+
+def uv_sphere(center, radius, slices=24, stacks=21):
+    """Tessellated sphere with 2*slices*(stacks-1) triangles and smooth
+    (radial) vertex normals; slices=24, stacks=21 gives 960 triangles."""
+    c = np.asarray(center, np.float64)
+    tris, nrms = [], []
+
+    def vert(i, j):
+        th = np.pi * i / stacks
+        ph = 2 * np.pi * j / slices
+        d = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+        return c + radius * d, d
+
+    for i in range(stacks):
+        for j in range(slices):
+            a, na = vert(i, j)
+            b, nb = vert(i + 1, j)
+            cc, nc = vert(i + 1, j + 1)
+            d, nd = vert(i, j + 1)
+            if i != 0:
+                tris.append(np.concatenate([a, b, d]))
+                nrms.append(np.concatenate([na, nb, nd]))
+            if i != stacks - 1:
+                tris.append(np.concatenate([d, b, cc]))
+                nrms.append(np.concatenate([nd, nb, nc]))
+    return np.array(tris, np.float32), np.array(nrms, np.float32)
+
+
+def _model(tb, te, kd=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ior=1.0, opaque=1.0, shininess=1.0, mid=0):
+    m = np.zeros((), PM_MODEL_DTYPE)
+    m["tri_begin"], m["tri_end"] = tb, te
+    m["kd"], m["ks"], m["ke"] = kd, ks, ke
+    m["refr_index"], m["opaque"], m["shininess"] = ior, opaque, shininess
+    m["id"] = mid
+    return m
+
+
+def cornell_box(sphere_slices=24, sphere_stacks=21) -> Scene:
+    """The small Cornell-box scene of the benchmark configs: unit cube with 5
+    diffuse walls (10 triangles; red and green side walls albedo 0.75, others
+    white 0.75) and an open front (z = 1); a 0.25 x 0.25 square area light at
+    the ceiling centre; a mirror sphere r = 0.2 and a glass sphere r = 0.2
+    (IOR 1.5), 960 triangles each."""
+    P = lambda *a: np.array(a, np.float32)
+    parts = []  # (name, tris, normals, material kwargs)
+    w = 0.75
+    parts.append(("floor", *_quad(P(0, 0, 0), P(0, 0, 1), P(1, 0, 1), P(1, 0, 0), P(0, 1, 0)), dict(kd=(w, w, w))))
+    parts.append(("ceiling", *_quad(P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1), P(0, -1, 0)), dict(kd=(w, w, w))))
+    parts.append(("backWall", *_quad(P(0, 0, 0), P(1, 0, 0), P(1, 1, 0), P(0, 1, 0), P(0, 0, 1)), dict(kd=(w, w, w))))
+    parts.append(("leftWall", *_quad(P(0, 0, 0), P(0, 1, 0), P(0, 1, 1), P(0, 0, 1), P(1, 0, 0)), dict(kd=(w, 0, 0))))
+    parts.append(("rightWall", *_quad(P(1, 0, 0), P(1, 0, 1), P(1, 1, 1), P(1, 1, 0), P(-1, 0, 0)), dict(kd=(0, w, 0))))
+    ms, mn = uv_sphere((0.3, 0.2, 0.35), 0.2, sphere_slices, sphere_stacks)
+    parts.append(("mirrorSphere", ms, mn, dict(ks=(1, 1, 1), shininess=0.0)))
+    gs, gn = uv_sphere((0.7, 0.2, 0.62), 0.2, sphere_slices, sphere_stacks)
+    parts.append(("glassSphere", gs, gn, dict(ks=(1, 1, 1), opaque=0.0, ior=1.5)))
+    y = np.float32(0.999)
+    lo, hi = np.float32(0.375), np.float32(0.625)
+    parts.append(("light", *_quad(P(lo, y, lo), P(hi, y, lo), P(hi, y, hi), P(lo, y, hi), P(0, -1, 0)),
+                  dict(kd=(0.78, 0.78, 0.78), ke=(1, 1, 1))))
+    tris, nrms, models, names = [], [], [], []
+    t0 = 0
+    for i, (name, t, n, mat) in enumerate(parts):
+        tris.append(t)
+        nrms.append(n)
+        models.append(_model(t0, t0 + len(t), mid=i, **mat))
+        names.append(name)
+        t0 += len(t)
+    models = np.array(models, PM_MODEL_DTYPE)
+    return Scene(np.concatenate(tris), np.concatenate(nrms), models, light_model=names.index("light"), names=names)
+
+
+def camera(position, direction, up, vfov_deg, width, height) -> np.ndarray:
+    """Camera(pos, dir, up, fov, W, H) (Camera.cuh:43-66) evaluated in float32
+    in the reference's host order."""
+    f32 = np.float32
+    pos = np.asarray(position, f32)
+    d = np.asarray(direction, f32)
+    u = np.asarray(up, f32)
+    aspect = f32(width) / f32(height)
+    theta = f32(f32(vfov_deg) * f32(3.141592653589793)) / f32(180.0)
+    h = f32(np.tan(theta / f32(2)))
+    vh = f32(2.0) * h
+    vw = aspect * vh
+
+    def norm(v):
+        if not np.any(v):
+            return v
+        ln = f32(np.sqrt(f32(f32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])))
+        return (v / ln).astype(f32)
+
+    def cross(a, b):
+        return np.array([a[1] * b[2] - a[2] * b[1], -(a[0] * b[2] - a[2] * b[0]), a[0] * b[1] - a[1] * b[0]], f32)
+
+    fwd = norm(d)
+    right = norm(cross(fwd, u))
+    upv = cross(right, fwd)
+    c = np.zeros((), PM_CAMERA_DTYPE)
+    c["position"], c["forward"], c["right"], c["up"] = pos, fwd, right, upv
+    c["viewport_width"], c["viewport_height"], c["focal_length"] = vw, vh, 1.0
+    c["width"], c["height"] = width, height
+    return c
+
+
+def cornell_camera(width=320, height=240) -> np.ndarray:
+    """The reference camera (kernel.cu:29-30: (-0.01, 0.795, 2.35) looking
+    down -z, 60 degree vertical field of view) mapped to the unit box."""
+    return camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, width, height)

@@ -157,6 +157,54 @@ pm_status pm_kdtree_describe(const pm_kdtree* tree, pm_kdtree_desc* out);
 pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* tree, uint32_t* h_node_idx, int8_t* h_node_plane);
 pm_status pm_kdtree_destroy(pm_kdtree* tree);
 
+/* ---- scene (Scene.cuh:7-25, Model.cuh:24-30, Material.cuh:4-31,
+ * Light.cuh:7-31). A model is a contiguous range of triangles with one
+ * material; the light source is the model whose triangles emit (the
+ * reference picks the OBJ group named "light", Initializer.cu:62-101). */
+typedef struct {
+    uint32_t tri_begin, tri_end; /* triangles [tri_begin, tri_end) */
+    float kd[3];                 /* Material::diffuse */
+    float ks[3];                 /* Material::specular */
+    float ke[3];                 /* Material::emission */
+    float refr_index;            /* Material::refr_index */
+    float opaque;                /* Material::opaque (1 = opaque) */
+    float shininess;             /* Material::shininess */
+    int32_t id;                  /* model id (refraction medium tracking) */
+    int32_t pad;
+    float aabb[6];               /* filled by the library where needed */
+} pm_model;
+
+typedef struct {
+    const float* d_tris9;    /* [n_tris][9] vertices v0 v1 v2 */
+    const float* d_normals9; /* [n_tris][9] vertex normals n0 n1 n2 */
+    uint32_t n_tris;
+    const pm_model* h_models; /* host array [n_models] */
+    int32_t n_models;
+    int32_t light_model;
+    float light_intensity[3]; /* LightSource::intensity */
+    float default_refr_index; /* GlobalParams::default_refractive_index */
+} pm_scene;
+
+typedef struct {
+    uint64_t paths;     /* light paths traced */
+    uint32_t n_global;  /* photons stored in the global map */
+    uint32_t n_caustic; /* photons stored in the caustic map */
+} pm_trace_stats;
+
+/* ---- photon emission and bounce tracing: trace_photons(gpu_scene,
+ * cpu_scene, diffuse_num, specular_num) (PhotonTracing.cu:379-422) with emit
+ * / trace / destiny (:245-375). Paths are numbered and path p draws from its
+ * own Philox4x32-10 stream keyed by (seed, p); the maps take the first
+ * n_global / n_caustic photons in (path, event) order, so the result does not
+ * depend on launch shape. Tracing stops when both maps are full or after
+ * max_paths paths (the reference spins forever when a map cannot fill,
+ * PhotonTracing.cu:344). Output blocks are photons9 with plane stride
+ * n_global / n_caustic. Photon power = intensity / n_global for both maps
+ * (PhotonTracing.cu:369). 1 <= max_bounces <= 16 (reference: 10). */
+pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, uint32_t n_global,
+                           uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths, float* d_global9,
+                           float* d_caustic9, pm_trace_stats* stats);
+
 /* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
  * only live effect is the model's bounding box (min/max over its vertices).
  * d_tris9: one triangle per 9 floats (v0.xyz v1.xyz v2.xyz); model m spans

@@ -79,6 +79,16 @@ int orc_model_aabb(const float* verts9, const uint32_t* tri_begin, int32_t n_mod
  * reference's distance test). */
 float orc_radius_threshold(float r);
 
+/* Photon emission and bounce tracing (PhotonTracing.cu:278-375): paths in
+ * index order, each with its own Philox stream (seed, 0, path); the maps
+ * take the first n_global / n_caustic photons in (path, event) order.
+ * models_raw points at n_models pm_model records (include/pm355.h).
+ * out_stats = {paths traced, global stored, caustic stored}. */
+int orc_trace_photons(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
+                      int32_t n_models, int32_t light_model, const float intensity[3], float default_ior,
+                      uint64_t seed, uint32_t n_global, uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,
+                      float* out_global9, float* out_caustic9, uint64_t* out_stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -173,3 +173,48 @@ def model_aabb(verts9, tri_begin):
 
 def radius_threshold(r):
     return float(np.float32(lib().orc_radius_threshold(float(r))))
+
+
+def trace_photons(scene, seed, n_global, n_caustic, max_bounces=10, max_paths=1 << 40):
+    """Oracle photon tracing on a pm355.scenes.Scene; returns (global9,
+    caustic9, stats) with stats = (paths, n_global_stored, n_caustic_stored)."""
+    L = lib()
+    if not getattr(L, "_trace_sig", False):
+        L.orc_trace_photons.argtypes = [_P, _P, _U32, _P, _I32, _I32, _P, _F32, C.c_uint64, _U32, _U32, _I32,
+                                        C.c_uint64, _P, _P, _P]
+        L.orc_trace_photons.restype = _I32
+        L._trace_sig = True
+    tris = _c(scene.tris9, np.float32)
+    nrms = _c(scene.normals9, np.float32)
+    models = np.ascontiguousarray(scene.models)
+    inten = _c(scene.light_intensity, np.float32)
+    g = np.zeros((9, max(n_global, 1)), np.float32)
+    c = np.zeros((9, max(n_caustic, 1)), np.float32)
+    st = np.zeros(3, np.uint64)
+    rc = L.orc_trace_photons(tris.ctypes.data, nrms.ctypes.data, tris.shape[0], models.ctypes.data, len(models),
+                             int(scene.light_model), inten.ctypes.data, float(scene.default_refr_index), int(seed),
+                             int(n_global), int(n_caustic), int(max_bounces), int(max_paths), g.ctypes.data,
+                             c.ctypes.data, st.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"orc_trace_photons rc={rc}")
+    return g[:, :n_global], c[:, :n_caustic], tuple(int(v) for v in st)
+
+
+def scene_hit(scene, rays6):
+    """Nearest hit of each ray (6 planes: origin xyz, direction xyz):
+    (t, model, normal3); t = -1 and model = -1 on a miss."""
+    L = lib()
+    L.orc_scene_hit.argtypes = [_P, _P, _U32, _P, _I32, _P, _U32, _P, _P, _P]
+    L.orc_scene_hit.restype = _I32
+    tris = _c(scene.tris9, np.float32)
+    nrms = _c(scene.normals9, np.float32)
+    models = np.ascontiguousarray(scene.models)
+    r = _c(rays6, np.float32)
+    nr = r.shape[1]
+    t = np.empty(nr, np.float32)
+    m = np.empty(nr, np.int32)
+    n = np.empty((3, nr), np.float32)
+    rc = L.orc_scene_hit(tris.ctypes.data, nrms.ctypes.data, tris.shape[0], models.ctypes.data, len(models),
+                         r.ctypes.data, nr, t.ctypes.data, m.ctypes.data, n.ctypes.data)
+    assert rc == 0
+    return t, m, n
