@@ -5,6 +5,7 @@
 // Philox4x32-10 stream.
 #pragma once
 #include <cfloat>
+#include <vector>
 
 #include "pm_internal.hpp"
 
@@ -142,6 +143,54 @@ __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_
     n_out = bn;
     m_out = bm;
     return true;
+}
+
+// Host side: copy the scene's model table to the device as DevModel records
+// with their AABBs (computed on the device over each model's triangles, the
+// live effect of create_kd_tree, KdTree.cu:395-464). d_tmp needs
+// 32 * n_models bytes. On return host_out holds the same records.
+inline pm_status upload_models(pm_ctx* ctx, const pm_scene* scene, DevModel* d_models, void* d_tmp,
+                               std::vector<DevModel>& host_out) {
+    const int nm = scene->n_models;
+    std::vector<DevModel>& hm = host_out;
+    hm.assign(nm, DevModel{});
+    for (int i = 0; i < nm; ++i) {
+        const pm_model& s = scene->h_models[i];
+        if (s.tri_end > scene->n_tris || s.tri_begin > s.tri_end)
+            return pm::fail(ctx, PM_ERR_INVALID, "scene: model %d triangle range", i);
+        DevModel& d = hm[i];
+        d.t0 = s.tri_begin;
+        d.t1 = s.tri_end;
+        for (int c = 0; c < 3; ++c) {
+            d.kd[c] = s.kd[c];
+            d.ks[c] = s.ks[c];
+            d.ke[c] = s.ke[c];
+        }
+        d.ior = s.refr_index;
+        d.opaque = s.opaque;
+        d.shininess = s.shininess;
+        d.id = s.id;
+        d.pad = 0;
+    }
+    uint32_t* d_tb = static_cast<uint32_t*>(d_tmp);
+    float* d_box = reinterpret_cast<float*>(static_cast<char*>(d_tmp) + 8 * (size_t)nm);
+    std::vector<uint32_t> tb(2 * (size_t)nm);
+    for (int i = 0; i < nm; ++i) {
+        tb[2 * i] = hm[i].t0;
+        tb[2 * i + 1] = hm[i].t1;
+    }
+    PM_HIP(ctx, hipMemcpyAsync(d_tb, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    pm_status st;
+    for (int i = 0; i < nm; ++i)
+        if ((st = pm::model_aabb_launch(ctx, scene->d_tris9, d_tb + 2 * i, 1, d_box + 6 * i)) != PM_OK) return st;
+    std::vector<float> boxes((size_t)nm * 6);
+    PM_HIP(ctx, hipMemcpyAsync(boxes.data(), d_box, boxes.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nm; ++i)
+        for (int c = 0; c < 6; ++c) hm[i].box[c] = boxes[(size_t)i * 6 + c];
+    PM_HIP(ctx, hipMemcpyAsync(d_models, hm.data(), nm * sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PM_OK;
 }
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11); one stream per

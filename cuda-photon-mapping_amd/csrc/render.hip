@@ -1,0 +1,416 @@
+// Eye-ray frame: Raytracer::render_gpu / render_trace (Raytracer.cuh:140-279)
+// with RaytracePlanner's deferred refraction rays (RaytracePlanner.cuh:30-69).
+//
+// The reference runs each pixel's whole ray tree in one thread, calling the
+// kNN gather from inside it through a global-memory heap per thread
+// (Drawer.cu:87-101). Here the frame is a wavefront:
+//   1. k_render_rays: one lane per pixel (8x8 pixel tiles per wave) walks the
+//      ray tree — primary ray, mirror reflections, a LIFO of deferred
+//      refraction rays — and records every hit (point, normal, model, ray
+//      coefficient, flags) in per-pixel slots, in the reference's visit order;
+//   2. the hits are compacted (wave64 scan) into one query block;
+//   3. the global and caustic gathers run as two batched, cell-ordered
+//      k-nearest + radiance launches (the gather kernel of knn.hip);
+//   4. k_render_combine: one lane per pixel folds its hits in order —
+//      4·Kd·opaque·L_global (outside objects, Kd != 0) + emission +
+//      opaque·L_caustic (when the caustic set is full), clamped to 1, times the
+//      ray coefficient — then HDR, clamp and the uchar3 conversion.
+// This is exact because the ray tree depends only on geometry and materials,
+// never on the photon estimates.
+//
+// Media: the reference's MediumManager (MediumManager.cuh:30-88) reads inner
+// stack entries it never wrote, so its refraction indices are undefined; here
+// refraction follows the photon tracer's medium rule (PhotonTracing.cu:127-151)
+// with one stack of (ior, id) per deferred ray, depth (max_depth + 1) / 2.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pm_geom.hpp"
+
+namespace {
+
+using namespace pmg;
+
+constexpr int kMaxDepth = 6;
+constexpr int kMaxMedium = (kMaxDepth + 1) / 2;
+
+struct Cam {
+    float pos[3], fwd[3], right[3], up[3];
+    float vw, vh, focal;
+    int32_t w, h, pad;
+};
+
+struct RaysParams {
+    const float* tris;
+    const float* nrms;
+    const DevModel* models;
+    int n_models;
+    float def_ior;
+    Cam cam;
+    int max_depth;
+    int hmax;
+    // per-pixel hit slots [hmax][npix]
+    float* hp;   // 3 planes: hit point
+    float* hn;   // 3 planes: normal
+    float* hc;   // 3 planes: ray coefficient
+    int* hm;     // model index | flags << 16 (bit 0: outside objects)
+    uint32_t* count;  // [npix]
+    uint32_t npix;
+};
+
+struct Plan {
+    V3 o, d;
+    V3 coef;
+    int depth;
+    int ms;
+    float mn[kMaxMedium];
+    int mid[kMaxMedium];
+};
+
+__global__ __launch_bounds__(256) void k_render_rays(RaysParams P) {
+    // 8x8 pixel tile per wave: coherent rays share model AABB outcomes
+    const uint32_t lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int W = P.cam.w, H = P.cam.h;
+    const int tiles_x = (W + 7) / 8;
+    const int tx = (int)(wave % (uint32_t)tiles_x), ty = (int)(wave / (uint32_t)tiles_x);
+    const int x = tx * 8 + (int)(lane & 7), y = ty * 8 + (int)(lane >> 3);
+    if (ty * 8 >= H) return;  // wave-uniform
+    const bool active = x < W && y < H;
+    const uint32_t pix = active ? (uint32_t)(y * W + x) : 0u;
+
+    // Camera::generate_ray_direction (Camera.cuh:67-81); Ray ctor normalises
+    const Cam& c = P.cam;
+    const float u = ((float)x + 0.5f) / W, v = ((float)y + 0.5f) / H;
+    const V3 pos = v3(c.pos[0], c.pos[1], c.pos[2]);
+    const V3 horiz = mul(v3(c.right[0], c.right[1], c.right[2]), c.vw);
+    const V3 vert = mul(v3(c.up[0], c.up[1], c.up[2]), c.vh);
+    const V3 ll = sub(sub(mul(add(pos, v3(c.fwd[0], c.fwd[1], c.fwd[2])), c.focal), mul(horiz, 0.5f)), mul(vert, 0.5f));
+    Ray ray;
+    ray.o = pos;
+    ray.d = normalize_div(sub(add(add(ll, mul(horiz, u)), mul(vert, v)), pos));
+
+    Plan plans[kMaxDepth];
+    int nplans = 0;
+    int new_depth = 0, cur_depth = -1;
+    V3 coef = v3(1, 1, 1);
+    bool in_obj = false;
+    int ms = 0;
+    float mn[kMaxMedium];
+    int mid[kMaxMedium];
+    uint32_t nh = 0;
+    const int max_medium = (P.max_depth + 1) / 2;
+    bool alive = active;
+    // render_trace's loop (Raytracer.cuh:147-236); the wave runs until every
+    // lane's tree is done (each lane's loop is bounded by 2^max_depth rays)
+    for (int guard = 0; guard < (1 << kMaxDepth) + 4; ++guard) {
+        if (alive && !(cur_depth < P.max_depth && cur_depth != new_depth)) {
+            if (nplans == 0) {
+                alive = false;
+            } else {
+                const Plan& p = plans[--nplans];
+                ray.o = p.o;
+                ray.d = p.d;
+                new_depth = p.depth;
+                coef = p.coef;
+                ms = p.ms;
+                in_obj = ms > 0;
+                for (int s = 0; s < kMaxMedium; ++s) {
+                    mn[s] = p.mn[s];
+                    mid[s] = p.mid[s];
+                }
+            }
+        }
+        if (!__builtin_amdgcn_ballot_w64(alive)) break;
+        if (alive) cur_depth = new_depth;
+        float th = 0.f;
+        V3 n = v3(0, 0, 0);
+        int hmod = -1;
+        // all lanes join the (wave-uniform) triangle loops; dead lanes shoot a
+        // null ray that hits nothing
+        Ray rr = ray;
+        if (!alive) rr.d = v3(0, 0, 0);
+        bool hit = scene_hit(P.models, P.n_models, P.tris, P.nrms, rr, th, n, hmod) && alive;
+        if (!hit) continue;
+        // Raytracer::find_intersection: point = dir*t + origin; normal turned
+        // toward the ray only inside objects
+        V3 ip = add(mul(ray.d, th), ray.o);
+        if (in_obj && dot(ray.d, n) > 0) n = mul(n, -1.f);
+        const DevModel& mat = P.models[hmod];
+        if (nh < (uint32_t)P.hmax) {
+            size_t s = (size_t)nh * P.npix + pix, plane = (size_t)P.hmax * P.npix;
+            P.hp[s] = ip.x;
+            P.hp[plane + s] = ip.y;
+            P.hp[2 * plane + s] = ip.z;
+            P.hn[s] = n.x;
+            P.hn[plane + s] = n.y;
+            P.hn[2 * plane + s] = n.z;
+            P.hc[s] = coef.x;
+            P.hc[plane + s] = coef.y;
+            P.hc[2 * plane + s] = coef.z;
+            P.hm[s] = hmod | ((in_obj ? 0 : 1) << 16);
+            ++nh;
+        }
+        if (mat.opaque < 1.f && cur_depth != P.max_depth - 1) {
+            float n1, n2;
+            int nms = ms;
+            float nmn[kMaxMedium];
+            int nmid[kMaxMedium];
+            for (int s = 0; s < kMaxMedium; ++s) {
+                nmn[s] = mn[s];
+                nmid[s] = mid[s];
+            }
+            if (ms > 0 && mid[ms - 1] == mat.id) {
+                n1 = mn[ms - 1];
+                n2 = ms >= 2 ? mn[ms - 2] : P.def_ior;
+                nms = ms - 1;
+            } else {
+                n1 = ms > 0 ? mn[ms - 1] : P.def_ior;
+                n2 = mat.ior;
+                if (ms < max_medium) {
+                    nmn[ms] = mat.ior;
+                    nmid[ms] = mat.id;
+                    nms = ms + 1;
+                }
+            }
+            // Ray::refract (Ray.cuh:85-96)
+            float eta = n1 / n2;
+            float c1 = -dot(ray.d, n);
+            float w = eta * c1;
+            float c2m = (w - eta) * (w + eta);
+            if (!(c2m < -1.f) && nplans < P.max_depth) {
+                Plan& p = plans[nplans++];
+                p.d = add(mul(ray.d, eta), mul(n, w - sqrtf(1.f + c2m)));
+                p.o = add(ip, mul(neg(n), 0.01f));
+                p.depth = cur_depth + 1;
+                p.coef = mul(coef, 1.f - mat.opaque);
+                p.ms = nms;
+                for (int s = 0; s < kMaxMedium; ++s) {
+                    p.mn[s] = nmn[s];
+                    p.mid[s] = nmid[s];
+                }
+            }
+        }
+        if (!in_obj && !(mat.ks[0] == 0.f && mat.ks[1] == 0.f && mat.ks[2] == 0.f) && cur_depth != P.max_depth - 1) {
+            // Ray::reflect (Ray.cuh:64-71) and the Phong-style coefficient
+            float dnd = dot(ray.d, n);
+            Ray nr;
+            nr.d = normalize_div(sub(ray.d, mul(mul(n, 2.f), dnd)));
+            nr.o = add(ip, mul(n, 0.01f));
+            float cc = powf(fmaxf(dot(nr.d, neg(ray.d)), 0.0f), mat.shininess);
+            coef = mulv(coef, mul(mul(v3(mat.ks[0], mat.ks[1], mat.ks[2]), mat.opaque), cc));
+            ray = nr;
+            new_depth = cur_depth + 1;
+        }
+    }
+    if (active) P.count[pix] = nh;
+}
+
+// Slot (pixel, h) -> compacted query q = off[pixel] + h: query point,
+// normal, and the inverse map for the combine pass.
+__global__ __launch_bounds__(256) void k_render_compact(const float* __restrict__ hp, const float* __restrict__ hn,
+                                                       const uint32_t* __restrict__ count,
+                                                       const uint32_t* __restrict__ off, uint32_t npix, int hmax,
+                                                       uint32_t nq, float* __restrict__ q3, float* __restrict__ n3) {
+    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    uint32_t c = count[pix], o = off[pix];
+    size_t plane = (size_t)hmax * npix;
+    for (uint32_t h = 0; h < c; ++h) {
+        size_t s = (size_t)h * npix + pix;
+        uint32_t q = o + h;
+        q3[q] = hp[s];
+        q3[nq + q] = hp[plane + s];
+        q3[2 * (size_t)nq + q] = hp[2 * plane + s];
+        n3[q] = hn[s];
+        n3[nq + q] = hn[plane + s];
+        n3[2 * (size_t)nq + q] = hn[2 * plane + s];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_render_combine(const DevModel* __restrict__ models,
+                                                       const float* __restrict__ hc, const int* __restrict__ hm,
+                                                       const uint32_t* __restrict__ count,
+                                                       const uint32_t* __restrict__ off, uint32_t npix, int hmax,
+                                                       uint32_t nq, const float* __restrict__ lg,
+                                                       const float* __restrict__ lc,
+                                                       const uint32_t* __restrict__ cc_count, int k_caustic,
+                                                       int have_caustic, float* __restrict__ rgb3,
+                                                       uint8_t* __restrict__ rgb8) {
+    uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    uint32_t c = count[pix], o = off[pix];
+    size_t plane = (size_t)hmax * npix;
+    V3 acc = v3(0, 0, 0);
+    for (uint32_t h = 0; h < c; ++h) {
+        size_t s = (size_t)h * npix + pix;
+        uint32_t q = o + h;
+        int code = hm[s];
+        const DevModel& mat = models[code & 0xFFFF];
+        bool outside = (code >> 16) & 1;
+        V3 cur = v3(0, 0, 0);
+        if (outside && !(mat.kd[0] == 0.f && mat.kd[1] == 0.f && mat.kd[2] == 0.f)) {
+            V3 di_op = mul(v3(mat.kd[0], mat.kd[1], mat.kd[2]), mat.opaque);
+            V3 re = v3(lg[q], lg[nq + q], lg[2 * (size_t)nq + q]);
+            cur = add(cur, mul(mulv(re, di_op), 4.f));
+        }
+        cur = add(cur, v3(mat.ke[0], mat.ke[1], mat.ke[2]));
+        if (have_caustic && cc_count[q] == (uint32_t)k_caustic) {
+            V3 t = v3(lc[q], lc[nq + q], lc[2 * (size_t)nq + q]);
+            cur = add(cur, mul(t, mat.opaque));
+        }
+        cur = v3(fminf(cur.x, 1.f), fminf(cur.y, 1.f), fminf(cur.z, 1.f));
+        acc = add(acc, mulv(cur, v3(hc[s], hc[plane + s], hc[2 * plane + s])));
+    }
+    // HDR (Raytracer.cuh:95-107), clamp_min(1), uchar3 (:238-239, :277)
+    const float t = 1.0f / 0.6f;  // pow(exposure, -1) with exposure = 0.6f
+    const float rgam = 1.0f / 2.2f;
+    const float o3[3] = {acc.x, acc.y, acc.z};
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float e = expf(-t * o3[ch]);
+        float mapped = fminf(powf(1.0f - e, rgam), 1.f);
+        if (rgb3) rgb3[(size_t)ch * npix + pix] = mapped;
+        if (rgb8) rgb8[3 * (size_t)pix + ch] = (uint8_t)(mapped * 255);
+    }
+}
+
+}  // namespace
+
+extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* global_grid,
+                                     const pm_grid* caustic_grid, const pm_camera* camera,
+                                     const pm_render_params* params, float* d_rgb3, uint8_t* d_rgb8,
+                                     pm_frame_stats* stats) {
+    if (!ctx || !scene || !camera || !params) return PM_ERR_INVALID;
+    if (!global_grid) return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: global grid required");
+    if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: empty scene");
+    const pm_render_params& R = *params;
+    if (R.max_depth < 1 || R.max_depth > kMaxDepth)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: max_depth outside [1, %d]", kMaxDepth);
+    if (camera->width <= 0 || camera->height <= 0)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: empty canvas");
+    const uint32_t W = (uint32_t)camera->width, H = (uint32_t)camera->height;
+    const uint64_t npix64 = (uint64_t)W * H;
+    if (npix64 > (1ull << 26)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: canvas too large");
+    const uint32_t npix = (uint32_t)npix64;
+    const int hmax = (1 << R.max_depth) - 1;
+    const int nm = scene->n_models;
+
+    size_t slot = (size_t)hmax * npix;
+    size_t bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), (size_t)nm * 32, slot * 12, slot * 12, slot * 12,
+                                   slot * 4, (size_t)npix * 4, (size_t)npix * 4, 64});
+    void* work = nullptr;
+    pm_status st = pm::frame_reserve(ctx, bytes, &work);
+    if (st != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    DevModel* d_models = cv.take<DevModel>(nm);
+    void* d_tmp = cv.take<char>((size_t)nm * 32);
+    float* hp = cv.take<float>(slot * 3);
+    float* hn = cv.take<float>(slot * 3);
+    float* hc = cv.take<float>(slot * 3);
+    int* hm = cv.take<int>(slot);
+    uint32_t* count = cv.take<uint32_t>(npix);
+    uint32_t* off = cv.take<uint32_t>(npix);
+    uint64_t* total = cv.take<uint64_t>(1);
+
+    std::vector<DevModel> host_models;
+    if ((st = upload_models(ctx, scene, d_models, d_tmp, host_models)) != PM_OK) return st;
+
+    RaysParams P{};
+    P.tris = scene->d_tris9;
+    P.nrms = scene->d_normals9;
+    P.models = d_models;
+    P.n_models = nm;
+    P.def_ior = scene->default_refr_index;
+    std::memcpy(&P.cam, camera, sizeof(Cam));
+    P.max_depth = R.max_depth;
+    P.hmax = hmax;
+    P.hp = hp;
+    P.hn = hn;
+    P.hc = hc;
+    P.hm = hm;
+    P.count = count;
+    P.npix = npix;
+    const uint32_t tiles = ((W + 7) / 8) * ((H + 7) / 8);
+    hipLaunchKernelGGL(k_render_rays, dim3(pm::div_up(tiles, 4)), dim3(256), 0, ctx->stream, P);
+    PM_CHECK_LAUNCH(ctx, "k_render_rays");
+    if ((st = pm::scan_u32(ctx, count, off, npix, false, total)) != PM_OK) return st;
+    uint64_t nq64 = 0;
+    PM_HIP(ctx, hipMemcpyAsync(&nq64, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t nq = (uint32_t)nq64;
+
+    // query block + gather outputs live in a second carve of the frame arena
+    size_t qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
+                                    (size_t)nq * 4, (size_t)nq * 4 * 256, (size_t)nq * 4 * 256});
+    // (index/distance buffers are only needed for the reference-caustic mode)
+    const bool quirk = (R.flags & PM_RENDER_REFERENCE_CAUSTIC) != 0 && caustic_grid;
+    if (!quirk) qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
+                                         (size_t)nq * 4});
+    // grow the arena if needed, keeping the hit slots: recompute as one block
+    size_t all = bytes + qbytes;
+    if (all > ctx->frame_bytes) {
+        // the hit slots must survive: allocate a fresh arena and move them
+        void* old = ctx->frame;
+        size_t old_bytes = ctx->frame_bytes;
+        void* fresh = nullptr;
+        PM_HIP(ctx, hipMalloc(&fresh, all + all / 8));
+        PM_HIP(ctx, hipMemcpyAsync(fresh, old, old_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PM_HIP(ctx, hipFree(old));
+        ctx->frame = fresh;
+        ctx->frame_bytes = all + all / 8;
+        // re-carve the same layout
+        pm::Carve c2{static_cast<char*>(ctx->frame)};
+        d_models = c2.take<DevModel>(nm);
+        c2.take<char>((size_t)nm * 32);
+        hp = c2.take<float>(slot * 3);
+        hn = c2.take<float>(slot * 3);
+        hc = c2.take<float>(slot * 3);
+        hm = c2.take<int>(slot);
+        count = c2.take<uint32_t>(npix);
+        off = c2.take<uint32_t>(npix);
+        c2.take<uint64_t>(1);
+        cv = c2;
+    }
+    float* q3 = cv.take<float>((size_t)nq * 3);
+    float* n3 = cv.take<float>((size_t)nq * 3);
+    float* lg = cv.take<float>((size_t)nq * 3);
+    float* lc = cv.take<float>((size_t)nq * 3);
+    uint32_t* ccount = cv.take<uint32_t>(nq);
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_render_compact, dim3(pm::div_up(npix, 256)), dim3(256), 0, ctx->stream, hp, hn, count,
+                           off, npix, hmax, nq, q3, n3);
+        PM_CHECK_LAUNCH(ctx, "k_render_compact");
+        const uint32_t gflags = R.gather_flags | PM_KNN_SORT_QUERIES;
+        if ((st = pm_grid_gather_radiance(ctx, global_grid, q3, n3, nq, R.r_global, R.k_global, gflags, lg, nullptr)) !=
+            PM_OK)
+            return st;
+        if (caustic_grid) {
+            if (!quirk) {
+                if ((st = pm_grid_gather_radiance(ctx, caustic_grid, q3, n3, nq, R.r_caustic, R.k_caustic, gflags, lc,
+                                                  ccount)) != PM_OK)
+                    return st;
+            } else {
+                // Raytracer.cuh:203-204: caustic indices, global map arrays
+                uint32_t* kidx = cv.take<uint32_t>((size_t)nq * R.k_caustic);
+                float* kdist = cv.take<float>((size_t)nq * R.k_caustic);
+                if ((st = pm_grid_knn(ctx, caustic_grid, q3, nq, R.r_caustic, R.k_caustic, gflags, kidx, kdist,
+                                      ccount)) != PM_OK)
+                    return st;
+                if ((st = pm_radiance(ctx, global_grid, kidx, kdist, ccount, n3, nq, R.k_caustic, lc)) != PM_OK)
+                    return st;
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_render_combine, dim3(pm::div_up(npix, 256)), dim3(256), 0, ctx->stream, d_models, hc, hm,
+                       count, off, npix, hmax, nq, lg, lc, ccount, R.k_caustic, caustic_grid ? 1 : 0, d_rgb3, d_rgb8);
+    PM_CHECK_LAUNCH(ctx, "k_render_combine");
+    if (stats) {
+        stats->hits = nq;
+        stats->pixels = npix;
+    }
+    return PM_OK;
+}

@@ -82,6 +82,7 @@ _SIGS = {
     "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
     "pm_kdtree_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
+    "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
 }
 
@@ -100,6 +101,19 @@ class SceneRec(C.Structure):
 
 class TraceStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("n_global", C.c_uint32), ("n_caustic", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("max_depth", C.c_int32), ("k_global", C.c_int32), ("r_global", C.c_float),
+                ("k_caustic", C.c_int32), ("r_caustic", C.c_float), ("flags", C.c_uint32),
+                ("gather_flags", C.c_uint32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("hits", C.c_uint64), ("pixels", C.c_uint32)]
+
+
+RENDER_REFERENCE_CAUSTIC = 1
 
 
 class KdTreeDesc(C.Structure):
@@ -286,6 +300,22 @@ class Context:
         self.check(self.lib.pm_trace_photons(self.h, C.byref(dscene.rec), int(seed), int(n_global), int(n_caustic),
                                              int(max_bounces), int(max_paths), _ptr(g), _ptr(c), C.byref(st)))
         return g[:, :n_global], c[:, :n_caustic], (int(st.paths), int(st.n_global), int(st.n_caustic))
+
+    def render_frame(self, dscene: "DeviceScene", grid_g: "Grid", grid_c, camera, max_depth=4, k_global=50,
+                     r_global=0.05, k_caustic=50, r_caustic=0.05, flags=0, gather_flags=0, want_rgb8=True):
+        """Render one frame; returns (rgb3 float32 [3, H*W], rgb8 uint8 [H*W*3] or None, hits)."""
+        import torch
+
+        cam = np.ascontiguousarray(camera)
+        W, H = int(cam["width"].item()), int(cam["height"].item())
+        rgb = torch.empty((3, W * H), dtype=torch.float32, device=self.device)
+        rgb8 = torch.empty((W * H * 3,), dtype=torch.uint8, device=self.device) if want_rgb8 else None
+        p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic),
+                         int(flags), int(gather_flags))
+        st = FrameStats()
+        self.check(self.lib.pm_render_frame(self.h, C.byref(dscene.rec), grid_g.h, grid_c.h if grid_c else None,
+                                            cam.ctypes.data, C.byref(p), _ptr(rgb), _ptr(rgb8), C.byref(st)))
+        return rgb, rgb8, int(st.hits)
 
     def box_stats(self, grid: "Grid", queries3, radius: float):
         q = _f32_dev(queries3, self.device)

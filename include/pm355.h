@@ -205,6 +205,45 @@ pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, ui
                            uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths, float* d_global9,
                            float* d_caustic9, pm_trace_stats* stats);
 
+/* ---- eye-ray frame: Raytracer::render_gpu / render_cpu / render_trace
+ * (Raytracer.cuh:140-299), Camera (Camera.cuh:43-81), RaytracePlanner
+ * (RaytracePlanner.cuh:9-73). Camera fields are those of the reference's
+ * Camera after construction (forward/right/up normalised, viewport sizes). */
+typedef struct {
+    float position[3], forward[3], right[3], up[3];
+    float viewport_width, viewport_height, focal_length;
+    int32_t width, height, pad;
+} pm_camera;
+
+enum {
+    /* Raytracer.cuh:203-204 reads the global map's power/direction with the
+     * caustic map's indices; this flag reproduces that (requires
+     * n_caustic <= n_global). Default: the caustic map's own arrays. */
+    PM_RENDER_REFERENCE_CAUSTIC = 1u << 0
+};
+
+typedef struct {
+    int32_t max_depth;     /* GlobalParams::max_depth (kernel.cu:23: 4), <= 6 */
+    int32_t k_global;      /* GlobalParams::global_photon_num (kernel.cu:25) */
+    float r_global;        /* Raytracer.cuh:187 (0.1) */
+    int32_t k_caustic;     /* GlobalParams::caustic_photon_num (kernel.cu:26) */
+    float r_caustic;       /* Raytracer.cuh:200 (0.2) */
+    uint32_t flags;        /* PM_RENDER_* */
+    uint32_t gather_flags; /* PM_KNN_* tuning flags for the gathers */
+} pm_render_params;
+
+typedef struct {
+    uint64_t hits;   /* eye-ray hits shaded (gather queries per map) */
+    uint32_t pixels;
+} pm_frame_stats;
+
+/* Renders one frame: d_rgb3 (nullable) = 3 planes of W*H HDR-mapped floats,
+ * d_rgb8 (nullable) = W*H uchar3 as the reference's canvas (row y, column x
+ * at y*W + x). caustic_grid may be NULL (no caustic term). */
+pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* global_grid,
+                          const pm_grid* caustic_grid, const pm_camera* camera, const pm_render_params* params,
+                          float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
+
 /* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
  * only live effect is the model's bounding box (min/max over its vertices).
  * d_tris9: one triangle per 9 floats (v0.xyz v1.xyz v2.xyz); model m spans

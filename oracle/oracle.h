@@ -84,6 +84,25 @@ float orc_radius_threshold(float r);
  * take the first n_global / n_caustic photons in (path, event) order.
  * models_raw points at n_models pm_model records (include/pm355.h).
  * out_stats = {paths traced, global stored, caustic stored}. */
+/* Nearest hit for each ray of a 6-plane ray block (origin, direction):
+ * t = -1, model = -1 on a miss. */
+int orc_scene_hit(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
+                  int32_t n_models, const float* rays6, uint32_t nr, float* out_t, int32_t* out_model,
+                  float* out_n3);
+
+/* Eye-ray frame: Raytracer::render_cpu / render_trace (Raytracer.cuh:140-299)
+ * over an oracle global grid and an optional caustic grid (c_n = 0: none).
+ * camera_raw = pm_camera; iparams = {max_depth, k_global, k_caustic, flags},
+ * fparams = {r_global, r_caustic}. out_rgb3: 3 planes of W*H HDR-mapped
+ * floats; out_rgb8: W*H*3 bytes (uchar3). out_stats[0] = eye-ray hits. */
+int orc_render(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
+               int32_t n_models, float default_ior, const uint32_t* g_counts, const uint32_t* g_starts,
+               const float* g_photons9, uint32_t g_n, const float* g_aabb, float g_cell, const int32_t* g_dims,
+               int32_t g_layout, const uint32_t* c_counts, const uint32_t* c_starts, const float* c_photons9,
+               uint32_t c_n, const float* c_aabb, float c_cell, const int32_t* c_dims, int32_t c_layout,
+               const void* camera_raw, const int32_t* iparams, const float* fparams, int32_t threads,
+               float* out_rgb3, uint8_t* out_rgb8, uint64_t* out_stats);
+
 int orc_trace_photons(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
                       int32_t n_models, int32_t light_model, const float intensity[3], float default_ior,
                       uint64_t seed, uint32_t n_global, uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,

@@ -218,3 +218,39 @@ def scene_hit(scene, rays6):
                          r.ctypes.data, nr, t.ctypes.data, m.ctypes.data, n.ctypes.data)
     assert rc == 0
     return t, m, n
+
+
+def render(scene, og, oc, camera, max_depth=4, k_global=50, r_global=0.05, k_caustic=50, r_caustic=0.05,
+           flags=0, threads=0):
+    """Oracle frame (render_cpu semantics) on OracleGrid og (global) and oc
+    (caustic, or None). Returns (rgb3 float32[3, H*W], rgb8 uint8[H*W*3], hits)."""
+    L = lib()
+    L.orc_render.argtypes = [_P, _P, _U32, _P, _I32, _F32, _P, _P, _P, _U32, _P, _F32, _P, _I32,
+                             _P, _P, _P, _U32, _P, _F32, _P, _I32, _P, _P, _P, _I32, _P, _P, _P]
+    L.orc_render.restype = _I32
+    tris = _c(scene.tris9, np.float32)
+    nrms = _c(scene.normals9, np.float32)
+    models = np.ascontiguousarray(scene.models)
+    cam = np.ascontiguousarray(camera)
+    W, H = int(cam["width"].item()), int(cam["height"].item())
+    ip = np.array([max_depth, k_global, k_caustic, flags], np.int32)
+    fp = np.array([r_global, r_caustic], np.float32)
+    rgb = np.empty((3, W * H), np.float32)
+    rgb8 = np.empty(W * H * 3, np.uint8)
+    st = np.zeros(1, np.uint64)
+    z = np.zeros(8, np.uint32)
+    if oc is None:
+        c_args = (z.ctypes.data, z.ctypes.data, z.ctypes.data, 0, og.aabb.ctypes.data, 1.0, og.dims.ctypes.data, 0)
+    else:
+        c_args = (oc.counts.ctypes.data, oc.starts.ctypes.data, oc.photons.ctypes.data, oc.n, oc.aabb.ctypes.data,
+                  oc.cell, oc.dims.ctypes.data, oc.layout)
+    if oc is None:
+        c_args = (None, None, None, 0) + c_args[4:]
+    rc = L.orc_render(tris.ctypes.data, nrms.ctypes.data, tris.shape[0], models.ctypes.data, len(models),
+                      float(scene.default_refr_index), og.counts.ctypes.data, og.starts.ctypes.data,
+                      og.photons.ctypes.data, og.n, og.aabb.ctypes.data, og.cell, og.dims.ctypes.data, og.layout,
+                      *c_args, cam.ctypes.data, ip.ctypes.data, fp.ctypes.data, int(threads), rgb.ctypes.data,
+                      rgb8.ctypes.data, st.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"orc_render rc={rc}")
+    return rgb, rgb8, int(st[0])

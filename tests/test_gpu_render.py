@@ -1,0 +1,63 @@
+"""GPU parity: full frames vs the oracle renderer on the same photon maps.
+
+The ray trees, hit points and kNN sets are bit-exact; the radiance sums
+differ only in summation order (<= 1e-5 relative) and expf / powf may differ
+by an ulp between the device and host libraries, so the HDR float image is
+compared with a per-pixel RMSE bound and a max-abs bound, and the uchar3
+canvas may differ by at most one step on a small fraction of channels.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pm355.scenes import cornell_box, cornell_camera
+
+pytestmark = pytest.mark.gpu
+
+RMSE_MAX = 1e-5      # HDR-mapped float image, per-pixel RMSE over the frame
+ABS_MAX = 1e-4       # any channel
+U8_STEP_FRAC = 1e-3  # fraction of uchar channels allowed to differ by 1
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    scene = cornell_box()
+    ds = ctx.upload_scene(scene)
+    g, c, _ = ctx.trace_photons(ds, 4, 100_000, 10_000, 8)
+    aabb = scene.aabb()
+    gh, ch = g.cpu().numpy(), c.cpu().numpy()
+    return scene, ds, aabb, g, c, gh, ch
+
+
+@pytest.mark.parametrize("max_depth,k,r,cell,flags", [(4, 50, 0.05, 0.05, 0), (4, 50, 0.05, 0.05, 1),
+                                                      (1, 20, 0.08, 0.04, 0), (6, 100, 0.02, 0.02, 0)])
+def test_frame_parity(ctx, setup, max_depth, k, r, cell, flags):
+    scene, ds, aabb, g, c, gh, ch = setup
+    gg = ctx.grid_build(g, aabb, cell_size=cell)
+    gc = ctx.grid_build(c, aabb, cell_size=cell)
+    og = O.OracleGrid(gh, aabb, cell_size=cell)
+    oc = O.OracleGrid(ch, aabb, cell_size=cell)
+    cam = cornell_camera(320, 240)
+    rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, max_depth, k, r, k, r, flags=flags)
+    ctx.sync()
+    orgb, orgb8, ohits = O.render(scene, og, oc, cam, max_depth, k, r, k, r, flags=flags)
+    assert hits == ohits
+    got = rgb.cpu().numpy()
+    rmse = np.sqrt(np.mean((got.astype(np.float64) - orgb) ** 2))
+    assert rmse <= RMSE_MAX, rmse
+    assert np.abs(got - orgb).max() <= ABS_MAX
+    g8 = rgb8.cpu().numpy().astype(np.int32)
+    d8 = np.abs(g8 - orgb8.astype(np.int32))
+    assert d8.max() <= 1 and np.mean(d8 > 0) <= U8_STEP_FRAC
+
+
+def test_frame_without_caustic_map(ctx, setup):
+    scene, ds, aabb, g, c, gh, ch = setup
+    gg = ctx.grid_build(g, aabb, cell_size=0.05)
+    og = O.OracleGrid(gh, aabb, cell_size=0.05)
+    cam = cornell_camera(96, 64)
+    rgb, _, hits = ctx.render_frame(ds, gg, None, cam, 4, 50, 0.05, 50, 0.05)
+    ctx.sync()
+    orgb, _, ohits = O.render(scene, og, None, cam, 4, 50, 0.05, 50, 0.05)
+    assert hits == ohits
+    assert np.abs(rgb.cpu().numpy() - orgb).max() <= ABS_MAX
