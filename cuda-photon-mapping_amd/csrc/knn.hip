@@ -23,11 +23,12 @@
 #include <cmath>
 #include <cstring>
 
-#include "pm_internal.hpp"
+#include "pm_select.hpp"
 
 namespace {
 
 using pmd::GridGeom;
+using namespace pms;
 
 constexpr int kWaves = 4;  // queries per 256-thread workgroup
 
@@ -58,48 +59,6 @@ struct KnnParams {
     float* rgb;  // 3 planes of nq (radiance only)
     int remap;
 };
-
-template <int CAP>
-struct WaveLds {
-    uint32_t hist[64];
-    uint64_t buf[CAP];
-};
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += t;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
-    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t bal) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-}
-
-// Squared distance in the reference's host order, never contracted to FMA.
-__device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
-    float dx = px - qx, dy = py - qy, dz = pz - qz;
-    float xx = __fmul_rn(dx, dx), yy = __fmul_rn(dy, dy), zz = __fmul_rn(dz, dz);
-    return __fadd_rn(__fadd_rn(xx, yy), zz);
-}
-
-__device__ __forceinline__ uint64_t make_key(float d2, uint32_t slot) {
-    return ((uint64_t)__float_as_uint(d2) << 32) | slot;
-}
 
 struct Box {
     int lo[3], hi[3];
@@ -185,110 +144,6 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
             x0 = x1;
             y0 = y1;
             z0 = z1;
-        }
-    }
-}
-
-// Narrow the bucket of keys kept from LDS buffer entries [0, cnt) — all of
-// which are < khi — to the smallest bin edge that keeps >= k keys and at most
-// `target` when the keys allow, then compact the buffer to those keys.
-// Round 0 bins by the top float bits of d2 (two bins per octave below the
-// radius threshold); later rounds split the crossing bin 64 ways.
-template <int CAP>
-__device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uint32_t k, uint32_t target, uint32_t bT,
-                               uint64_t& khi_io) {
-    if (cnt <= target || cnt <= k) return cnt;
-    L.hist[lane] = 0;
-    wave_sync();
-    for (uint32_t e = lane; e < cnt; e += 64) {
-        uint32_t h = bT - (uint32_t)(L.buf[e] >> 54);
-        atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
-    }
-    wave_sync();
-    uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
-    int ls = __builtin_ctzll(__ballot(cum >= k));
-    uint32_t hstar = 63u - (uint32_t)ls;
-    uint32_t A = __shfl(cum, ls, 64);
-    uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
-    uint64_t khi = (uint64_t)(bT - hstar + 1u) << 54, klo = 0;
-    uint32_t below = 0;
-    if (hstar < 63u) {
-        klo = (uint64_t)(bT - hstar) << 54;
-        below = ls > 0 ? prev : 0u;
-    }
-    if (khi > khi_io) khi = khi_io;
-    while (A > target) {
-        uint64_t width = khi - klo;
-        int s = width <= 64 ? 0 : 64 - __builtin_clzll(width - 1) - 6;
-        L.hist[lane] = 0;
-        wave_sync();
-        for (uint32_t e = lane; e < cnt; e += 64) {
-            uint64_t key = L.buf[e];
-            if (key >= klo && key < khi) atomicAdd(&L.hist[(uint32_t)((key - klo) >> s)], 1u);
-        }
-        wave_sync();
-        uint32_t cu = wave_incl_scan(L.hist[lane], lane);
-        uint32_t need = k - below;
-        int js = __builtin_ctzll(__ballot(cu >= need));
-        uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
-        uint32_t at = __shfl(cu, js, 64);
-        uint64_t nlo = klo + ((uint64_t)js << s);
-        uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
-        if (nhi > khi) nhi = khi;
-        A = below + at;
-        below += js > 0 ? pv : 0u;
-        klo = nlo;
-        khi = nhi;
-    }
-    // in-place compaction, one row of 64 at a time (writes never pass reads)
-    uint32_t out = 0;
-    for (uint32_t base = 0; base < cnt; base += 64) {
-        uint32_t e = base + lane;
-        uint64_t key = e < cnt ? L.buf[e] : ~0ull;
-        wave_sync();
-        bool keep = e < cnt && key < khi;
-        uint64_t bal = __ballot(keep);
-        if (keep) L.buf[out + mbcnt(bal)] = key;
-        out += (uint32_t)__popcll(bal);
-        wave_sync();
-    }
-    khi_io = khi;
-    return out;
-}
-
-template <int R>
-__device__ __forceinline__ void bitonic_sort(uint64_t (&v)[R], uint32_t lane) {
-    constexpr int N = 64 * R;
-#pragma unroll
-    for (int size = 2; size <= N; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            if (stride >= 64) {
-                const int rs = stride / 64;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    if ((j & rs) == 0) {
-                        const int jp = j | rs;
-                        const bool asc = (((j * 64) & size) == 0);
-                        uint64_t a = v[j], b = v[jp];
-                        bool sw = asc ? (a > b) : (a < b);
-                        v[j] = sw ? b : a;
-                        v[jp] = sw ? a : b;
-                    }
-                }
-            } else {
-                const bool lower = (lane & (uint32_t)stride) == 0;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const uint32_t e = (uint32_t)j * 64 + lane;
-                    const bool asc = (e & (uint32_t)size) == 0;
-                    uint64_t p = shfl_xor_u64(v[j], stride);
-                    bool take_min = (asc == lower);
-                    uint64_t mn = v[j] < p ? v[j] : p;
-                    uint64_t mx = v[j] < p ? p : v[j];
-                    v[j] = take_min ? mn : mx;
-                }
-            }
         }
     }
 }
@@ -593,19 +448,6 @@ __global__ __launch_bounds__(256) void k_box_stats(GridGeom G, const uint32_t* _
 __global__ void k_zero2(unsigned long long* p) {
     p[0] = 0;
     p[1] = 0;
-}
-
-float radius_threshold(float r) {
-    float t = r * r;
-    while (std::sqrt(t) > r) t = std::nextafter(t, 0.f);
-    for (;;) {
-        float u = std::nextafter(t, INFINITY);
-        if (std::sqrt(u) <= r)
-            t = u;
-        else
-            break;
-    }
-    return t;
 }
 
 pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq, float radius, int32_t k,

@@ -37,6 +37,14 @@ struct pm_grid {
     void* mem = nullptr;  // one allocation backing starts/sizes/photons/perm
 };
 
+struct pm_kdtree {
+    uint32_t n = 0, cap = 0, levels = 0;
+    void* mem = nullptr;
+    uint32_t* node_idx = nullptr;  // [cap]
+    int8_t* node_plane = nullptr;  // [cap]
+    float* photons = nullptr;      // photons9 in node order [9][cap] (empty slots hold +inf positions)
+};
+
 namespace pm {
 
 pm_status fail(pm_ctx* ctx, pm_status code, const char* fmt, ...);

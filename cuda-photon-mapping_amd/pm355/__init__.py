@@ -81,6 +81,7 @@ _SIGS = {
     "pm_kdtree_describe": (_I32, [_P, _P]),
     "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
     "pm_kdtree_destroy": (_I32, [_P]),
+    "pm_kdtree_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _P, _P, _P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
@@ -267,6 +268,18 @@ class Context:
         t = _P()
         self.check(self.lib.pm_kdtree_build(self.h, _ptr(ph), ph.shape[1], C.byref(t)))
         return KdTree(self, t, keep=ph)
+
+    def kdtree_knn(self, tree: "KdTree", queries3, radius: float, k: int):
+        import torch
+
+        q = _f32_dev(queries3, self.device)
+        nq = q.shape[1]
+        idx = torch.empty((nq, k), dtype=torch.int32, device=self.device)
+        dist = torch.empty((nq, k), dtype=torch.float32, device=self.device)
+        cnt = torch.empty((nq,), dtype=torch.int32, device=self.device)
+        self.check(self.lib.pm_kdtree_knn(self.h, tree.h, _ptr(q), nq, float(radius), int(k), _ptr(idx), _ptr(dist),
+                                          _ptr(cnt)))
+        return idx, dist, cnt
 
     def model_aabb(self, tris9, tri_begin):
         import torch

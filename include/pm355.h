@@ -157,6 +157,16 @@ pm_status pm_kdtree_describe(const pm_kdtree* tree, pm_kdtree_desc* out);
 pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* tree, uint32_t* h_node_idx, int8_t* h_node_plane);
 pm_status pm_kdtree_destroy(pm_kdtree* tree);
 
+/* k nearest photons on the kd-tree, the query side of cpm::PhotonMap
+ * (PhotonMap::locate_q, PhotonMap.cuh:289-371, whose debug queue and 10000
+ * sentinel are replaced by an exact search). Same contract as pm_grid_knn:
+ * photons with length(p - q) <= radius, the k smallest by (FP32 d2, photon
+ * index), sorted ascending; d_idx[q*k + j] is the INPUT photon index
+ * (0xFFFFFFFF past the count), d_dist the FP32 distance (+inf past the
+ * count), d_count[q] = min(k, photons within radius). k in [1, 256]. */
+pm_status pm_kdtree_knn(pm_ctx* ctx, const pm_kdtree* tree, const float* d_queries3, uint32_t nq, float radius,
+                        int32_t k, uint32_t* d_idx, float* d_dist, uint32_t* d_count);
+
 /* ---- scene (Scene.cuh:7-25, Model.cuh:24-30, Material.cuh:4-31,
  * Light.cuh:7-31). A model is a contiguous range of triangles with one
  * material; the light source is the model whose triangles emit (the

@@ -1,0 +1,34 @@
+"""The C++ façade (cuda-photon-mapping_amd/host/pm355.hpp) end to end: the
+demo program builds a PhotonGrid, queries find_nearests and builds a
+PhotonMap through the reference's names; its outputs must equal the oracle's
+on the same photons."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import PKG
+
+pytestmark = pytest.mark.gpu
+
+
+def test_facade_demo_matches_oracle(tmp_path):
+    exe = os.path.join(PKG, "facade_demo")
+    n, nq = 60000, 3000
+    r = subprocess.run([exe, str(tmp_path), str(n), str(nq), "5"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rd = lambda name, dt: np.fromfile(os.path.join(tmp_path, name), dtype=dt)
+    ph = rd("photons.f32", np.float32).reshape(9, n)
+    q = rd("queries.f32", np.float32).reshape(3, nq)
+    og = O.OracleGrid(ph, (0, 0, 0, 1, 1, 1), cell_size=0.03)
+    assert np.array_equal(rd("grid_starts.u32", np.uint32), og.starts)
+    assert np.array_equal(rd("grid_sizes.u32", np.uint32), og.counts)
+    oi, od, oc = og.knn(q, 0.05, 32)
+    assert np.array_equal(rd("knn_count.u32", np.uint32), oc)
+    assert np.array_equal(rd("knn_idx.u32", np.uint32).reshape(nq, 32), oi)
+    assert np.array_equal(rd("knn_dist.f32", np.float32).reshape(nq, 32).view(np.uint32), od.view(np.uint32))
+    ki, kp = O.kdtree_build(ph[0:3])
+    assert np.array_equal(rd("kd_idx.u32", np.uint32), ki)
+    assert np.array_equal(rd("kd_plane.i8", np.int8), kp)

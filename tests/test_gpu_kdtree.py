@@ -57,3 +57,48 @@ def test_model_aabb(ctx):
         got = ctx.model_aabb(tris, bounds).cpu().numpy()
         ref = O.model_aabb(tris, bounds)
         assert np.array_equal(got, ref)
+
+
+def _kd_knn_check(ctx, pos, q, r, k):
+    t = ctx.kdtree_build(_photons(pos))
+    gi, gd, gc = ctx.kdtree_knn(t, q, r, k)
+    gi = gi.cpu().numpy().view(np.uint32)
+    gd = gd.cpu().numpy()
+    gc = gc.cpu().numpy().view(np.uint32)
+    ni, npl = O.kdtree_build(pos)
+    oi, od, oc = O.kdtree_knn(pos, ni, npl, q, r, k)
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gi, oi), np.nonzero((gi != oi).any(axis=1))[0][:10]
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+    return gi, gd, gc
+
+
+def test_kdtree_knn_reference_fixture(ctx):
+    # PhotonMapGetClosestTest.cu:21-52,64: query (80, 9, 40), k = 4 returns
+    # photons {3, 4, 5, 2} with distances {0, 50.636, 64.156, 82.444}.
+    q = np.array([[80.0], [9.0], [40.0]], np.float32)
+    gi, gd, gc = _kd_knn_check(ctx, KD_FIXTURE, q, 1e4, 4)
+    assert gc[0] == 4 and sorted(gi[0].tolist()) == [2, 3, 4, 5]
+    assert np.allclose(gd[0], [0.0, 50.636, 64.156, 82.444], atol=1e-3)
+
+
+@pytest.mark.parametrize("n,nq,r,k", [(1, 5, 2.0, 3), (9, 40, 0.5, 4), (5000, 700, 0.05, 16),
+                                      (60000, 2000, 0.03, 64), (60000, 500, 0.08, 256), (200000, 3000, 0.01, 100)])
+def test_kdtree_knn_matches_oracle_and_brute(ctx, n, nq, r, k):
+    rng = np.random.default_rng(n + k)
+    pos = rng.random((3, n), dtype=np.float32)
+    if n > 40:
+        pos[:, 10:40] = pos[:, 9:10]  # exact duplicates: ties on d2 resolved by index
+        pos[1, 100:n // 4] = 0.0  # a plane of photons
+    q = rng.random((3, nq), dtype=np.float32)
+    q[:, 0] = pos[:, min(9, n - 1)]
+    gi, gd, gc = _kd_knn_check(ctx, pos, q, r, k)
+    if n * nq <= 2e8:
+        bi, bd, bc = O.knn_brute(pos, q, r, k)
+        assert np.array_equal(gc, bc) and np.array_equal(gi, bi)
+
+
+def test_kdtree_knn_ties(ctx):
+    pos = np.random.default_rng(4).integers(0, 6, (3, 3000)).astype(np.float32) * 0.01
+    q = np.random.default_rng(5).integers(0, 6, (3, 400)).astype(np.float32) * 0.01
+    _kd_knn_check(ctx, pos, q, 0.02, 50)
