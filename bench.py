@@ -63,6 +63,62 @@ def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
     }
 
 
+def frame_bench(ctx, n_frames):
+    """The per-frame form on the benchmark's Cornell box: 2M global + 500k
+    caustic photons traced, both hash grids (cell = r) and both kd-trees
+    built, then a 1280x720 frame with k = 100 within r = 0.02, max depth 4.
+    Reports photons traced/s and p50/p99 frame latency (wall clock around a
+    synchronized frame) with the per-stage medians (HIP events)."""
+    import numpy as np
+    import torch
+
+    from pm355.scenes import cornell_box, cornell_camera
+
+    scene = cornell_box()
+    ds = ctx.upload_scene(scene, accel=True)
+    aabb = scene.aabb()
+    cam = cornell_camera(1280, 720)
+    stream = torch.cuda.current_stream(ctx.device)
+
+    def frame(seed):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record(stream)
+        g, c, st = ctx.trace_photons(ds, seed, 2_000_000, 500_000, 10, max_paths=200_000_000)
+        ev[1].record(stream)
+        gg = ctx.grid_build(g, aabb, cell_size=0.02)
+        gc = ctx.grid_build(c, aabb, cell_size=0.02)
+        ev[2].record(stream)
+        kg, kc = ctx.kdtree_build(g), ctx.kdtree_build(c)
+        ev[3].record(stream)
+        rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, 4, 100, 0.02, 100, 0.02)
+        ev[4].record(stream)
+        torch.cuda.synchronize()
+        for h in (gg, gc, kg, kc):
+            h.close()
+        return st, [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+
+    frame(1)
+    lat, stages, stored = [], [], 0
+    for i in range(n_frames):
+        t0 = time.perf_counter()
+        st, sm = frame(100 + i)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        stages.append(sm)
+        stored = st[1] + st[2]
+    stages = np.median(np.array(stages), 0)
+    return {
+        "workload": "Cornell box (10 wall + 2x960 sphere + 2 light triangles, per-model BVHs), 2M global + 500k "
+                    "caustic photons (<=10 bounces), grids cell=r and kd-trees built per frame, 1280x720, "
+                    "k=100, r=0.02, max depth 4",
+        "frames": n_frames,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "photons_traced_per_s": stored / (stages[0] * 1e-3),
+        "stage_ms": {"trace": stages[0], "grid_builds": stages[1], "kdtree_builds": stages[2],
+                     "render": stages[3]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +131,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--sort-queries", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -219,7 +276,13 @@ def main():
         "gather_only_queries_per_s": nq / (gather_ms * 1e-3),
         "box_stats": {"cells": cells, "candidates": cands, "kept": kept},
     }
+    # kd-tree build over the same 16M photons (PhotonMap), HIP events
+    kd_ms = timed(lambda: ctx.kdtree_build(d_ph).close(), 3)
+    res["components_ms"]["kdtree_build_16M"] = kd_ms
     g.close()
+    if args.frames > 0:
+        log(f"per-frame form: {args.frames} Cornell-box frames")
+        res["per_frame"] = frame_bench(ctx, args.frames)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)

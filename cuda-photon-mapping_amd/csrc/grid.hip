@@ -30,6 +30,47 @@ __global__ __launch_bounds__(256) void k_grid_hash(const float* __restrict__ px,
     }
 }
 
+// Cell index of every photon (the hash of PhotonGrid.cu:152-174 without the
+// counting atomics: counts come from the sorted keys).
+__global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ px, const float* __restrict__ py,
+                                                  const float* __restrict__ pz, uint32_t n, GridGeom G,
+                                                  uint32_t* __restrict__ keys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = pmd::cell_coord(px[i], G.mn[0], G.cell, G.g[0]);
+    int y = pmd::cell_coord(py[i], G.mn[1], G.cell, G.g[1]);
+    int z = pmd::cell_coord(pz[i], G.mn[2], G.cell, G.g[2]);
+    keys[i] = pmd::cell_index(G, x, y, z);
+}
+
+// starts[c] = first sorted position with key >= c (the exclusive scan of the
+// counts, PhotonGrid.cu:195-196), sizes[c] = starts[c+1] - starts[c].
+__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ skeys, uint32_t n,
+                                                    uint32_t n_cells, uint32_t* __restrict__ starts,
+                                                    uint32_t* __restrict__ sizes) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (skeys[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    uint32_t end = lo;
+    hi = n;
+    while (end < hi) {
+        uint32_t mid = end + ((hi - end) >> 1);
+        if (skeys[mid] <= c)
+            end = mid + 1;
+        else
+            hi = mid;
+    }
+    starts[c] = lo;
+    sizes[c] = end - lo;
+}
+
 // analyze_density_kernel's max (PhotonGrid.cu:70-90), as a reduction.
 __global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out) {
     uint32_t m = 0;
@@ -41,15 +82,17 @@ __global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ a,
 
 // assign_photons_to_cells (PhotonGrid.cu:152-174) with the slot taken from
 // the stable sort instead of an atomic: in-cell order = input order.
+// blockIdx.y is the plane: blocks are dispatched plane by plane, so the
+// randomly gathered source plane (4 B x n) stays resident in the MALL.
 __global__ __launch_bounds__(256) void k_grid_scatter(const float* __restrict__ in9, uint32_t n,
                                                      const uint32_t* __restrict__ order, float* __restrict__ out9,
                                                      uint32_t* __restrict__ perm) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    const uint32_t p = blockIdx.y;
     uint32_t src = order[j];
-    perm[j] = src;
-#pragma unroll
-    for (int p = 0; p < 9; ++p) out9[(size_t)p * n + j] = in9[(size_t)p * n + src];
+    if (p == 0) perm[j] = src;
+    out9[(size_t)p * n + j] = in9[(size_t)p * n + src];
 }
 
 struct HostDims {
@@ -193,13 +236,15 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         delete g;
         return s;
     };
-    if ((st = pm::fill_u32(ctx, sizes, 0u, d.n_cells)) != PM_OK) return bail(st);
-    hipLaunchKernelGGL(k_grid_hash, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys, sizes);
-    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_hash launch"));
-    if ((st = pm::scan_u32(ctx, sizes, starts, d.n_cells, false, nullptr)) != PM_OK) return bail(st);
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys);
+    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_keys launch"));
     if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(d.n_cells), rwork)) != PM_OK)
         return bail(st);
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, order, photons, perm);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(pm::div_up(d.n_cells, 256)), dim3(256), 0, ctx->stream, skeys, n,
+                       d.n_cells, starts, sizes);
+    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_cell_bounds launch"));
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks, 9), dim3(256), 0, ctx->stream, d_photons9, n, order, photons,
+                       perm);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
 
     pm_grid_desc& D = g->desc;

@@ -47,14 +47,31 @@ struct Ray {
 };
 
 // Per-model record on the device (material of Material.cuh:4-31, the
-// triangle range of the model, its AABB and id).
+// triangle range of the model, its AABB and id). bvh is the model's root
+// node in the scene's BVH node array, or -1 to test every triangle.
 struct DevModel {
     uint32_t t0, t1;
     float kd[3], ks[3], ke[3];
     float ior, opaque, shininess;
-    int32_t id, pad;
+    int32_t id, bvh;
     float box[6];
 };
+
+}  // namespace pmg
+
+// Scene acceleration structure: per-model AABBs and triangle BVHs, built once
+// per geometry and shared by photon tracing and eye-ray rendering.
+struct pm_accel {
+    uint32_t n_tris = 0;
+    int32_t n_models = 0;
+    std::vector<uint32_t> ranges;  // [n_models][2] triangle range the BVH was built for
+    std::vector<float> boxes;      // [n_models][6] exact model AABB
+    std::vector<int32_t> roots;    // [n_models] root node, -1 = no BVH (every triangle tested)
+    uint32_t n_nodes = 0;
+    float4* nodes = nullptr;  // [n_nodes][4]
+};
+
+namespace pmg {
 
 __device__ __forceinline__ V3 load_v(const float* __restrict__ t9, uint32_t t, int k) {
     const float* p = t9 + 9 * (size_t)t + 3 * k;
@@ -99,18 +116,111 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, V3 v0, V3 v1, V3 v2, float
     return true;
 }
 
+// ---- per-model triangle BVH ------------------------------------------------
+// A node is 4 float4: child 0 box (lo.xyz, hi.xyz), child 1 box, then the two
+// child codes as int bits (>= 0: node index; < 0: ~triangle index). Boxes
+// are inflated by a margin far above the rounding error of tri_hit, so a
+// triangle that tri_hit accepts always lies inside every box on its path.
+constexpr int kBvhStack = 64;  // a node's key prefix grows down the tree: depth <= 64
+
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, V3 o, V3 inv,
+                                     float tb, float& tn) {
+    const float ax = (lx - o.x) * inv.x, bx = (hx - o.x) * inv.x;
+    const float ay = (ly - o.y) * inv.y, by = (hy - o.y) * inv.y;
+    const float az = (lz - o.z) * inv.z, bz = (hz - o.z) * inv.z;
+    const float tmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.f));
+    const float tmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tb));
+    tn = tmin;
+    return tmin <= tmax;
+}
+
+__device__ __forceinline__ V3 inv_dir(V3 d) {
+    // a zero component becomes a tiny one of the same sign: its slab is then
+    // all of t when the origin lies inside it and empty otherwise
+    return v3(1.0f / (d.x != 0.f ? d.x : copysignf(1e-30f, d.x)), 1.0f / (d.y != 0.f ? d.y : copysignf(1e-30f, d.y)),
+              1.0f / (d.z != 0.f ? d.z : copysignf(1e-30f, d.z)));
+}
+
+// Nearest triangle of one model through its BVH: the smallest t among the
+// triangles tri_hit accepts, the lowest index on ties -- exactly what testing
+// every triangle in index order returns. Triangles with t >= limit may be
+// skipped (the caller discards them). A box is pruned only when its entry t
+// exceeds the bound by a relative 2^-16, far above the slab test's rounding.
+__device__ __forceinline__ bool bvh_hit(const float4* __restrict__ nodes, int root, const float* __restrict__ t9,
+                                        const Ray& r, V3 inv, float limit, float& mt, uint32_t& ti, V3& uvw) {
+    int stack[kBvhStack];
+    float stn[kBvhStack];
+    int sp = 0;
+    int node = root;
+    bool found = false;
+    mt = FLT_MAX;
+    for (;;) {
+        const float bound = fminf(limit, mt);
+        const float tb = bound + bound * 0x1p-16f;
+        if (node >= 0) {
+            const float4* nd = nodes + 4 * (size_t)node;
+            const float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+            float t0, t1;
+            const bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, r.o, inv, tb, t0);
+            const bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r.o, inv, tb, t1);
+            const int c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+            if (h0 && h1) {
+                const bool first0 = t0 <= t1;
+                stack[sp] = first0 ? c1 : c0;
+                stn[sp] = first0 ? t1 : t0;
+                ++sp;
+                node = first0 ? c0 : c1;
+                continue;
+            }
+            if (h0 || h1) {
+                node = h0 ? c0 : c1;
+                continue;
+            }
+        } else {
+            const uint32_t t = ~(uint32_t)node;
+            float pt;
+            V3 pu;
+            if (tri_hit(r, load_v(t9, t, 0), load_v(t9, t, 1), load_v(t9, t, 2), pt, pu) &&
+                (pt < mt || (found && pt == mt && t < ti))) {
+                mt = pt;
+                ti = t;
+                uvw = pu;
+                found = true;
+            }
+        }
+        // next deferred subtree that can still hold a closer hit
+        const float bnd = fminf(limit, mt);
+        const float tcut = bnd + bnd * 0x1p-16f;
+        node = 0x7FFFFFFF;
+        while (sp > 0) {
+            --sp;
+            if (stn[sp] <= tcut) {
+                node = stack[sp];
+                break;
+            }
+        }
+        if (node == 0x7FFFFFFF) break;
+    }
+    return found;
+}
+
 // Nearest hit over all models (find_intersection, PhotonTracing.cu:178-213 /
 // Raytracer.cuh:66-93) with Model::intersection (Model.cuh:366-446): per
-// model an AABB test, every triangle in order keeping the first of equal t,
-// normal = normalised barycentric blend of the vertex normals; a model's hit
-// replaces the current one if the current t is 0 or the new t is smaller.
-// Triangle data is read with wave-uniform indices (scalar loads).
+// model an AABB test, then the model's nearest triangle (the first of equal
+// t in index order), normal = normalised barycentric blend of the vertex
+// normals; a model's hit replaces the current one if the current t is 0 or
+// the new t is smaller. A model with a BVH finds its nearest triangle by
+// traversal (same result); the others test every triangle, with triangle
+// data read at wave-uniform indices (scalar loads).
 __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_models,
-                                          const float* __restrict__ t9, const float* __restrict__ n9, const Ray& r,
-                                          float& t_out, V3& n_out, int& m_out) {
+                                          const float* __restrict__ t9, const float* __restrict__ n9,
+                                          const float4* __restrict__ bvh, const Ray& r, float& t_out, V3& n_out,
+                                          int& m_out) {
     float best = 0.f;
     int bm = -1;
     V3 bn = v3(0, 0, 0);
+    V3 inv = v3(0, 0, 0);
+    bool have_inv = false;
     for (int m = 0; m < n_models; ++m) {
         const bool in_box = box_hit(M[m].box, r);
         if (!__builtin_amdgcn_ballot_w64(in_box)) continue;
@@ -118,15 +228,26 @@ __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_
         bool found = false;
         V3 uvw = v3(0, 0, 0);
         uint32_t ti = 0;
-        const uint32_t t0 = M[m].t0, t1 = M[m].t1;
-        for (uint32_t t = t0; t < t1; ++t) {
-            float pt;
-            V3 pu;
-            if (in_box && tri_hit(r, load_v(t9, t, 0), load_v(t9, t, 1), load_v(t9, t, 2), pt, pu) && pt < mt) {
-                mt = pt;
-                uvw = pu;
-                ti = t;
-                found = true;
+        const int root = M[m].bvh;
+        if (root >= 0) {
+            if (!have_inv) {
+                inv = inv_dir(r.d);
+                have_inv = true;
+            }
+            // a later model matters only below the current best (unless best is 0)
+            const float limit = (bm >= 0 && best != 0.f) ? best : FLT_MAX;
+            if (in_box) found = bvh_hit(bvh, root, t9, r, inv, limit, mt, ti, uvw);
+        } else {
+            const uint32_t t0 = M[m].t0, t1 = M[m].t1;
+            for (uint32_t t = t0; t < t1; ++t) {
+                float pt;
+                V3 pu;
+                if (in_box && tri_hit(r, load_v(t9, t, 0), load_v(t9, t, 1), load_v(t9, t, 2), pt, pu) && pt < mt) {
+                    mt = pt;
+                    uvw = pu;
+                    ti = t;
+                    found = true;
+                }
             }
         }
         if (!found) continue;
@@ -146,18 +267,24 @@ __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_
 }
 
 // Host side: copy the scene's model table to the device as DevModel records
-// with their AABBs (computed on the device over each model's triangles, the
-// live effect of create_kd_tree, KdTree.cu:395-464). d_tmp needs
-// 32 * n_models bytes. On return host_out holds the same records.
+// with their AABBs -- from the scene's acceleration structure when it has
+// one, else computed on the device over each model's triangles (the live
+// effect of create_kd_tree, KdTree.cu:395-464). d_tmp needs 32 * n_models
+// bytes. On return host_out holds the same records.
 inline pm_status upload_models(pm_ctx* ctx, const pm_scene* scene, DevModel* d_models, void* d_tmp,
                                std::vector<DevModel>& host_out) {
     const int nm = scene->n_models;
+    const pm_accel* acc = scene->accel;
     std::vector<DevModel>& hm = host_out;
     hm.assign(nm, DevModel{});
+    if (acc && (acc->n_models != nm || acc->n_tris != scene->n_tris))
+        return pm::fail(ctx, PM_ERR_INVALID, "scene: acceleration structure built for another scene");
     for (int i = 0; i < nm; ++i) {
         const pm_model& s = scene->h_models[i];
         if (s.tri_end > scene->n_tris || s.tri_begin > s.tri_end)
             return pm::fail(ctx, PM_ERR_INVALID, "scene: model %d triangle range", i);
+        if (acc && (acc->ranges[2 * i] != s.tri_begin || acc->ranges[2 * i + 1] != s.tri_end))
+            return pm::fail(ctx, PM_ERR_INVALID, "scene: model %d differs from its acceleration structure", i);
         DevModel& d = hm[i];
         d.t0 = s.tri_begin;
         d.t1 = s.tri_end;
@@ -170,28 +297,34 @@ inline pm_status upload_models(pm_ctx* ctx, const pm_scene* scene, DevModel* d_m
         d.opaque = s.opaque;
         d.shininess = s.shininess;
         d.id = s.id;
-        d.pad = 0;
+        d.bvh = acc ? acc->roots[i] : -1;
+        if (acc)
+            for (int c = 0; c < 6; ++c) d.box[c] = acc->boxes[6 * (size_t)i + c];
     }
-    uint32_t* d_tb = static_cast<uint32_t*>(d_tmp);
-    float* d_box = reinterpret_cast<float*>(static_cast<char*>(d_tmp) + 8 * (size_t)nm);
-    std::vector<uint32_t> tb(2 * (size_t)nm);
-    for (int i = 0; i < nm; ++i) {
-        tb[2 * i] = hm[i].t0;
-        tb[2 * i + 1] = hm[i].t1;
+    if (!acc) {
+        uint32_t* d_tb = static_cast<uint32_t*>(d_tmp);
+        float* d_box = reinterpret_cast<float*>(static_cast<char*>(d_tmp) + 8 * (size_t)nm);
+        std::vector<uint32_t> tb(2 * (size_t)nm);
+        for (int i = 0; i < nm; ++i) {
+            tb[2 * i] = hm[i].t0;
+            tb[2 * i + 1] = hm[i].t1;
+        }
+        PM_HIP(ctx, hipMemcpyAsync(d_tb, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        pm_status st;
+        for (int i = 0; i < nm; ++i)
+            if ((st = pm::model_aabb_launch(ctx, scene->d_tris9, d_tb + 2 * i, 1, d_box + 6 * i)) != PM_OK) return st;
+        std::vector<float> boxes((size_t)nm * 6);
+        PM_HIP(ctx, hipMemcpyAsync(boxes.data(), d_box, boxes.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < nm; ++i)
+            for (int c = 0; c < 6; ++c) hm[i].box[c] = boxes[(size_t)i * 6 + c];
     }
-    PM_HIP(ctx, hipMemcpyAsync(d_tb, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    pm_status st;
-    for (int i = 0; i < nm; ++i)
-        if ((st = pm::model_aabb_launch(ctx, scene->d_tris9, d_tb + 2 * i, 1, d_box + 6 * i)) != PM_OK) return st;
-    std::vector<float> boxes((size_t)nm * 6);
-    PM_HIP(ctx, hipMemcpyAsync(boxes.data(), d_box, boxes.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
-    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < nm; ++i)
-        for (int c = 0; c < 6; ++c) hm[i].box[c] = boxes[(size_t)i * 6 + c];
     PM_HIP(ctx, hipMemcpyAsync(d_models, hm.data(), nm * sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
     PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PM_OK;
 }
+
+inline const float4* bvh_nodes(const pm_scene* scene) { return scene->accel ? scene->accel->nodes : nullptr; }
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11); one stream per
 // (seed, stream id, path id), four uniforms per block, u in (0, 1].

@@ -47,6 +47,7 @@ struct RaysParams {
     const float* nrms;
     const DevModel* models;
     int n_models;
+    const float4* bvh;
     float def_ior;
     Cam cam;
     int max_depth;
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(256) void k_render_rays(RaysParams P) {
         // null ray that hits nothing
         Ray rr = ray;
         if (!alive) rr.d = v3(0, 0, 0);
-        bool hit = scene_hit(P.models, P.n_models, P.tris, P.nrms, rr, th, n, hmod) && alive;
+        bool hit = scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, rr, th, n, hmod) && alive;
         if (!hit) continue;
         // Raytracer::find_intersection: point = dir*t + origin; normal turned
         // toward the ray only inside objects
@@ -323,6 +324,7 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
     P.nrms = scene->d_normals9;
     P.models = d_models;
     P.n_models = nm;
+    P.bvh = bvh_nodes(scene);
     P.def_ior = scene->default_refr_index;
     std::memcpy(&P.cam, camera, sizeof(Cam));
     P.max_depth = R.max_depth;

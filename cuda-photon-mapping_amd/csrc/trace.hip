@@ -35,6 +35,7 @@ struct TraceParams {
     const float* nrms;
     const DevModel* models;
     int n_models;
+    const float4* bvh;
     int light;
     float pp[3];
     float def_ior;
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
         float th;
         V3 n;
         int hm;
-        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, ray, th, n, hm)) break;
+        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, ray, th, n, hm)) break;
         V3 ip = add(mul(ray.d, th), ray.o);
         if (dot(ray.d, n) > 0) n = mul(n, -1.f);
         const DevModel& mo = P.models[hm];
@@ -281,19 +282,9 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         const pm_model& s = scene->h_models[i];
         if (s.tri_end > scene->n_tris || s.tri_begin > s.tri_end)
             return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: model %d triangle range", i);
-        DevModel& d = hm[i];
-        d.t0 = s.tri_begin;
-        d.t1 = s.tri_end;
-        for (int c = 0; c < 3; ++c) {
-            d.kd[c] = s.kd[c];
-            d.ks[c] = s.ks[c];
-            d.ke[c] = s.ke[c];
-        }
-        d.ior = s.refr_index;
-        d.opaque = s.opaque;
-        d.shininess = s.shininess;
-        d.id = s.id;
-        d.pad = 0;
+        hm[i].t0 = s.tri_begin;
+        hm[i].t1 = s.tri_end;
+        hm[i].ior = s.refr_index;
     }
     if (hm[scene->light_model].t1 == hm[scene->light_model].t0)
         return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: light model has no triangles");
@@ -322,7 +313,7 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     if (max_paths < batch) batch = (uint32_t)std::max<uint64_t>(max_paths, 1);
     const int B = max_bounces;
     size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
-                                      (size_t)nm * 8, (size_t)nm * 24, (size_t)batch * B * 36,
+                                      (size_t)nm * 32, (size_t)batch * B * 36,
                                       (size_t)batch * B, (size_t)batch * 4 * 4, 64});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, rs_bytes, &work);
@@ -330,8 +321,7 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     pm::Carve cv{static_cast<char*>(work)};
     DevModel* d_models = cv.take<DevModel>(nm);
     CaEntry* d_ca = cv.take<CaEntry>(ca.size() + 1);
-    uint32_t* d_tb = cv.take<uint32_t>(2 * (size_t)nm);
-    float* d_box = cv.take<float>((size_t)nm * 6);
+    void* d_tmp = cv.take<char>(32 * (size_t)nm);
     float* stage = cv.take<float>((size_t)batch * B * 9);
     uint8_t* kind = cv.take<uint8_t>((size_t)batch * B);
     uint32_t* cnt_g = cv.take<uint32_t>(batch);
@@ -340,22 +330,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     uint32_t* off_c = cv.take<uint32_t>(batch);
     uint64_t* tot = cv.take<uint64_t>(2);
 
-    // model AABBs on the device (the live effect of create_kd_tree): one
-    // reduction per model over its own triangle range
-    std::vector<uint32_t> tb(2 * (size_t)nm);
-    for (int i = 0; i < nm; ++i) {
-        tb[2 * i] = hm[i].t0;
-        tb[2 * i + 1] = hm[i].t1;
-    }
-    PM_HIP(ctx, hipMemcpyAsync(d_tb, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    for (int i = 0; i < nm; ++i)
-        if ((st = pm::model_aabb_launch(ctx, scene->d_tris9, d_tb + 2 * i, 1, d_box + 6 * i)) != PM_OK) return st;
-    std::vector<float> boxes((size_t)nm * 6);
-    PM_HIP(ctx, hipMemcpyAsync(boxes.data(), d_box, boxes.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
-    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < nm; ++i)
-        for (int c = 0; c < 6; ++c) hm[i].box[c] = boxes[(size_t)i * 6 + c];
-    PM_HIP(ctx, hipMemcpyAsync(d_models, hm.data(), nm * sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    // model records with their AABBs (the live effect of create_kd_tree)
+    if ((st = upload_models(ctx, scene, d_models, d_tmp, hm)) != PM_OK) return st;
     if (!ca.empty())
         PM_HIP(ctx, hipMemcpyAsync(d_ca, ca.data(), ca.size() * sizeof(CaEntry), hipMemcpyHostToDevice, ctx->stream));
 
@@ -364,6 +340,7 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     P.nrms = scene->d_normals9;
     P.models = d_models;
     P.n_models = nm;
+    P.bvh = bvh_nodes(scene);
     P.light = scene->light_model;
     for (int c = 0; c < 3; ++c) P.pp[c] = scene->light_intensity[c] / (float)n_global;
     P.def_ior = scene->default_refr_index;

@@ -1,6 +1,6 @@
 // Drives the C++ façade the way the reference's Initializer / tests do:
 // build a PhotonGrid from a photon block, query find_nearests, build a
-// PhotonMap, and write every result to <outdir> as raw little-endian arrays
+// PhotonMap and query it, and write every result to <outdir> as raw little-endian arrays
 // for the parity test to compare with the oracle.
 //   facade_demo <outdir> <n_photons> <n_queries> <seed>
 #include <cstdio>
@@ -62,6 +62,9 @@ int main(int argc, char** argv) {
         std::vector<uint32_t> node_idx;
         std::vector<int8_t> node_plane;
         map.nodes(node_idx, node_plane);
+        pm355::DeviceBuffer<uint32_t> kidx((size_t)nq * k), kcnt(nq);
+        pm355::DeviceBuffer<float> kdist((size_t)nq * k);
+        map.find_nearests(d_q.data(), nq, r, k, kidx.data(), kdist.data(), kcnt.data());
         ctx.sync();
         pm355::HostGrid hg = grid.copy_to_cpu();
         write_file(out + "/photons.f32", ph.data(), ph.size() * 4);
@@ -74,6 +77,12 @@ int main(int argc, char** argv) {
         write_file(out + "/knn_count.u32", hc.data(), hc.size() * 4);
         write_file(out + "/kd_idx.u32", node_idx.data(), node_idx.size() * 4);
         write_file(out + "/kd_plane.i8", node_plane.data(), node_plane.size());
+        auto ki = kidx.to_host();
+        auto kd = kdist.to_host();
+        auto kc = kcnt.to_host();
+        write_file(out + "/kdknn_idx.u32", ki.data(), ki.size() * 4);
+        write_file(out + "/kdknn_dist.f32", kd.data(), kd.size() * 4);
+        write_file(out + "/kdknn_count.u32", kc.data(), kc.size() * 4);
         write_file(out + "/grid_starts.u32", hg.cell_starts.data(), hg.cell_starts.size() * 4);
         write_file(out + "/grid_sizes.u32", hg.cell_sizes.data(), hg.cell_sizes.size() * 4);
         std::printf("facade_demo ok: n=%u nq=%u cells=%u slots=%u\n", n, nq, hg.desc.n_cells, map.desc().n_slots);

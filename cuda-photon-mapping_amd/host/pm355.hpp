@@ -198,6 +198,12 @@ public:
         plane.resize(desc_.n_slots);
         ctx_->check(pm_kdtree_copy_to_host(ctx_->get(), t_, idx.data(), plane.data()));
     }
+    // k nearest photons within radius, the query side of PhotonMap::locate_q
+    // (PhotonMap.cuh:289-371); d_idx holds input photon indices
+    void find_nearests(const float* d_queries3, uint32_t nq, float radius, int k, uint32_t* d_idx, float* d_dist,
+                       uint32_t* d_count) const {
+        ctx_->check(pm_kdtree_knn(ctx_->get(), t_, d_queries3, nq, radius, k, d_idx, d_dist, d_count));
+    }
     const pm_kdtree_desc& desc() const { return desc_; }
     Type type() const { return type_; }
 
@@ -206,6 +212,21 @@ private:
     Type type_;
     pm_kdtree* t_ = nullptr;
     pm_kdtree_desc desc_{};
+};
+
+// Per-model triangle BVHs of a scene (the per-model KdTree of
+// KdTree.cuh:42-157 as designed). Attach with scene.accel = accel.get() while
+// the scene's triangles stay unchanged; results are identical without it.
+class SceneAccel {
+public:
+    SceneAccel(Context& ctx, const pm_scene& scene) { ctx.check(pm_accel_build(ctx.get(), &scene, &a_)); }
+    ~SceneAccel() { pm_accel_destroy(a_); }
+    SceneAccel(const SceneAccel&) = delete;
+    SceneAccel& operator=(const SceneAccel&) = delete;
+    const pm_accel* get() const { return a_; }
+
+private:
+    pm_accel* a_ = nullptr;
 };
 
 // trace_photons (PhotonTracing.cu:379-422): returns the global ("diffuse")

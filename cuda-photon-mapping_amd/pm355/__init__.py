@@ -82,6 +82,8 @@ _SIGS = {
     "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
     "pm_kdtree_destroy": (_I32, [_P]),
     "pm_kdtree_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _P, _P, _P]),
+    "pm_accel_build": (_I32, [_P, _P, C.POINTER(_P)]),
+    "pm_accel_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
@@ -97,7 +99,7 @@ class ModelRec(C.Structure):
 class SceneRec(C.Structure):
     _fields_ = [("d_tris9", C.c_void_p), ("d_normals9", C.c_void_p), ("n_tris", C.c_uint32),
                 ("h_models", C.c_void_p), ("n_models", C.c_int32), ("light_model", C.c_int32),
-                ("light_intensity", C.c_float * 3), ("default_refr_index", C.c_float)]
+                ("light_intensity", C.c_float * 3), ("default_refr_index", C.c_float), ("accel", C.c_void_p)]
 
 
 class TraceStats(C.Structure):
@@ -300,8 +302,12 @@ class Context:
         return order
 
     # -- scene / photon tracing ------------------------------------------------
-    def upload_scene(self, scene) -> "DeviceScene":
-        return DeviceScene(self, scene)
+    def upload_scene(self, scene, accel: bool = False) -> "DeviceScene":
+        """Copy a scene to HBM; accel=True also builds its triangle BVHs."""
+        ds = DeviceScene(self, scene)
+        if accel:
+            ds.build_accel()
+        return ds
 
     def trace_photons(self, dscene: "DeviceScene", seed: int, n_global: int, n_caustic: int,
                       max_bounces: int = 10, max_paths: int = 1 << 40):
@@ -402,7 +408,30 @@ class DeviceScene:
         for i in range(3):
             r.light_intensity[i] = float(scene.light_intensity[i])
         r.default_refr_index = float(scene.default_refr_index)
+        r.accel = None
         self.rec = r
+        self.ctx = ctx
+        self.accel = None
+
+    def build_accel(self):
+        """Build (or rebuild) the per-model BVHs used by tracing and rendering."""
+        self.drop_accel()
+        a = _P()
+        self.ctx.check(self.ctx.lib.pm_accel_build(self.ctx.h, C.byref(self.rec), C.byref(a)))
+        self.accel = a
+        self.rec.accel = a.value
+
+    def drop_accel(self):
+        if self.accel is not None:
+            self.ctx.lib.pm_accel_destroy(self.accel)
+            self.accel = None
+            self.rec.accel = None
+
+    def __del__(self):
+        try:
+            self.drop_accel()
+        except Exception:
+            pass
 
 
 class KdTree:

@@ -57,29 +57,34 @@ def _quad(p0, p1, p2, p3, n):
 
 def uv_sphere(center, radius, slices=24, stacks=21):
     """Tessellated sphere with 2*slices*(stacks-1) triangles and smooth
-    (radial) vertex normals; slices=24, stacks=21 gives 960 triangles."""
+    (radial) vertex normals; slices=24, stacks=21 gives 960 triangles.
+    Triangle order: stack by stack, and within a stack slice by slice the
+    upper triangle (a, b, d) then the lower one (d, b, c)."""
     c = np.asarray(center, np.float64)
+    i = np.arange(stacks + 1, dtype=np.float64)[:, None]
+    j = np.arange(slices + 1, dtype=np.float64)[None, :]
+    th = np.pi * i / stacks
+    ph = 2 * np.pi * j / slices
+    d = np.stack(np.broadcast_arrays(np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)), -1)
+    p = c + radius * d  # [stacks+1, slices+1, 3]
     tris, nrms = [], []
-
-    def vert(i, j):
-        th = np.pi * i / stacks
-        ph = 2 * np.pi * j / slices
-        d = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
-        return c + radius * d, d
-
-    for i in range(stacks):
-        for j in range(slices):
-            a, na = vert(i, j)
-            b, nb = vert(i + 1, j)
-            cc, nc = vert(i + 1, j + 1)
-            d, nd = vert(i, j + 1)
-            if i != 0:
-                tris.append(np.concatenate([a, b, d]))
-                nrms.append(np.concatenate([na, nb, nd]))
-            if i != stacks - 1:
-                tris.append(np.concatenate([d, b, cc]))
-                nrms.append(np.concatenate([nd, nb, nc]))
-    return np.array(tris, np.float32), np.array(nrms, np.float32)
+    for si in range(stacks):
+        a, na = p[si, :-1], d[si, :-1]
+        bb, nb = p[si + 1, :-1], d[si + 1, :-1]
+        cc, nc = p[si + 1, 1:], d[si + 1, 1:]
+        dd, nd = p[si, 1:], d[si, 1:]
+        up = np.concatenate([a, bb, dd], 1), np.concatenate([na, nb, nd], 1)
+        lo = np.concatenate([dd, bb, cc], 1), np.concatenate([nd, nb, nc], 1)
+        if si == 0:
+            t, n = lo
+        elif si == stacks - 1:
+            t, n = up
+        else:
+            t = np.stack([up[0], lo[0]], 1).reshape(-1, 9)
+            n = np.stack([up[1], lo[1]], 1).reshape(-1, 9)
+        tris.append(t)
+        nrms.append(n)
+    return np.concatenate(tris).astype(np.float32), np.concatenate(nrms).astype(np.float32)
 
 
 def _model(tb, te, kd=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ior=1.0, opaque=1.0, shininess=1.0, mid=0):
@@ -161,3 +166,49 @@ def cornell_camera(width=320, height=240) -> np.ndarray:
     """The reference camera (kernel.cu:29-30: (-0.01, 0.795, 2.35) looking
     down -z, 60 degree vertical field of view) mapped to the unit box."""
     return camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, width, height)
+
+
+def random_scene(seed: int, n_spheres=None, tri_range=(200, 20000)) -> Scene:
+    """A scene of the batched-workload distribution: the open unit box of
+    cornell_box (5 walls and the ceiling light) plus 1-8 spheres, each
+    tessellated to 200-20k triangles (log-uniform), radius 0.05-0.2, with a
+    material drawn from diffuse (albedo U[0.2, 0.9]), mirror, or glass
+    (IOR U[1.3, 1.7]). Spheres may overlap each other and the walls."""
+    rng = np.random.default_rng(seed)
+    base = cornell_box()
+    keep = [i for i, nm in enumerate(base.names) if nm not in ("mirrorSphere", "glassSphere")]
+    tris, nrms, mats, names = [], [], [], []
+    for i in keep:
+        m = base.models[i]
+        tris.append(base.tris9[m["tri_begin"]:m["tri_end"]])
+        nrms.append(base.normals9[m["tri_begin"]:m["tri_end"]])
+        mats.append(dict(kd=tuple(m["kd"]), ks=tuple(m["ks"]), ke=tuple(m["ke"]), ior=float(m["refr_index"]),
+                         opaque=float(m["opaque"]), shininess=float(m["shininess"])))
+        names.append(base.names[i])
+    ns = int(rng.integers(1, 9)) if n_spheres is None else int(n_spheres)
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    for s_i in range(ns):
+        target = float(np.exp(rng.uniform(lo, hi)))
+        slices = max(3, int(round(np.sqrt(target / 1.8))))
+        stacks = max(2, int(round(target / (2 * slices))) + 1)
+        r = float(rng.uniform(0.05, 0.2))
+        c = (float(rng.uniform(r, 1 - r)), float(rng.uniform(r, 0.9 - r)), float(rng.uniform(r, 1 - r)))
+        t, n = uv_sphere(c, r, slices, stacks)
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            a = float(rng.uniform(0.2, 0.9))
+            mat = dict(kd=(a, a * float(rng.uniform(0.5, 1)), a * float(rng.uniform(0.5, 1))))
+        elif kind == 1:
+            mat = dict(ks=(1, 1, 1), shininess=0.0)
+        else:
+            mat = dict(ks=(1, 1, 1), opaque=0.0, ior=float(rng.uniform(1.3, 1.7)))
+        tris.append(t)
+        nrms.append(n)
+        mats.append(mat)
+        names.append(f"sphere{s_i}")
+    models, t0 = [], 0
+    for i, (t, mat) in enumerate(zip(tris, mats)):
+        models.append(_model(t0, t0 + len(t), mid=i, **mat))
+        t0 += len(t)
+    return Scene(np.concatenate(tris), np.concatenate(nrms), np.array(models, PM_MODEL_DTYPE),
+                 light_model=names.index("light"), names=names)

@@ -54,6 +54,7 @@ enum {
 typedef struct pm_ctx pm_ctx;
 typedef struct pm_grid pm_grid;
 typedef struct pm_kdtree pm_kdtree;
+typedef struct pm_accel pm_accel;
 
 /* ---- context (replaces the reference's global device state: GlobalParams
  * __constant__ mirrors GlobalParams.cu:1-58, the heap pointers of
@@ -193,7 +194,20 @@ typedef struct {
     int32_t light_model;
     float light_intensity[3]; /* LightSource::intensity */
     float default_refr_index; /* GlobalParams::default_refractive_index */
+    const pm_accel* accel;    /* optional, from pm_accel_build on this geometry; NULL = test every triangle */
 } pm_scene;
+
+/* ---- scene acceleration: per-model triangle BVHs, the structure the
+ * reference's KdTree was designed to be (KdTree.cuh:42-157, KdTree.cu:329-375,
+ * build loop never enabled) for the brute-force Model::intersection loop
+ * (Model.cuh:381-429). Built on the device (Morton order, radix sort, binary
+ * radix tree, bottom-up boxes) for every model with at least 16 triangles.
+ * Traversal returns exactly the brute-force nearest hit (smallest t, lowest
+ * triangle index on ties; boxes are conservatively inflated), so photon maps
+ * and frames are bit-identical with and without it. The scene's triangles and
+ * model ranges must not change while the structure is in use. */
+pm_status pm_accel_build(pm_ctx* ctx, const pm_scene* scene, pm_accel** out);
+pm_status pm_accel_destroy(pm_accel* accel);
 
 typedef struct {
     uint64_t paths;     /* light paths traced */

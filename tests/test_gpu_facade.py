@@ -1,5 +1,5 @@
 """The C++ façade (cuda-photon-mapping_amd/host/pm355.hpp) end to end: the
-demo program builds a PhotonGrid, queries find_nearests and builds a
+demo program builds a PhotonGrid, queries find_nearests, builds and queries a
 PhotonMap through the reference's names; its outputs must equal the oracle's
 on the same photons."""
 import os
@@ -32,3 +32,7 @@ def test_facade_demo_matches_oracle(tmp_path):
     ki, kp = O.kdtree_build(ph[0:3])
     assert np.array_equal(rd("kd_idx.u32", np.uint32), ki)
     assert np.array_equal(rd("kd_plane.i8", np.int8), kp)
+    qi, qd, qc = O.kdtree_knn(ph[0:3], ki, kp, q, 0.05, 32)
+    assert np.array_equal(rd("kdknn_count.u32", np.uint32), qc)
+    assert np.array_equal(rd("kdknn_idx.u32", np.uint32).reshape(nq, 32), qi)
+    assert np.array_equal(rd("kdknn_dist.f32", np.float32).reshape(nq, 32).view(np.uint32), qd.view(np.uint32))

@@ -19,10 +19,10 @@ ABS_MAX = 1e-4       # any channel
 U8_STEP_FRAC = 1e-3  # fraction of uchar channels allowed to differ by 1
 
 
-@pytest.fixture(scope="module")
-def setup(ctx):
+@pytest.fixture(scope="module", params=[False, True], ids=["brute", "bvh"])
+def setup(ctx, request):
     scene = cornell_box()
-    ds = ctx.upload_scene(scene)
+    ds = ctx.upload_scene(scene, accel=request.param)
     g, c, _ = ctx.trace_photons(ds, 4, 100_000, 10_000, 8)
     aabb = scene.aabb()
     gh, ch = g.cpu().numpy(), c.cpu().numpy()

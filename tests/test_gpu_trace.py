@@ -14,9 +14,9 @@ from pm355.scenes import cornell_box
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def dscene(ctx):
-    return ctx.upload_scene(cornell_box())
+@pytest.fixture(scope="module", params=[False, True], ids=["brute", "bvh"])
+def dscene(ctx, request):
+    return ctx.upload_scene(cornell_box(), accel=request.param)
 
 
 def _np(t):
