@@ -119,6 +119,62 @@ def frame_bench(ctx, n_frames):
     }
 
 
+def batch_bench(ctx, n_jobs, reps=2):
+    """Plain per-job form vs the batched persistent form on the many-job
+    gather mix (photon counts log-uniform 10k-4M, one query per pixel of a
+    128x128 / 256x256 / 640x480 / 1280x720 frame, k = 64, r in [0.01, 0.05],
+    cell = r). Both forms gather radiance for every job's queries in cell
+    order with the same kernel code and give identical results; the plain
+    form makes one library call per job, the batched form one call (one
+    processing-order sort and one persistent launch) for all jobs."""
+    import torch
+
+    import pm355
+    from pm355.workloads import batch_mix, gather_inputs_device
+
+    mix = batch_mix(n_jobs=n_jobs)
+    jobs, grids, nq_total, n_ph = [], [], 0, 0
+    for j in mix:
+        ph, q, qn, aabb = gather_inputs_device(j["photons"], j["queries"], j["seed"], ctx.device)
+        g = ctx.grid_build(ph, aabb, cell_size=j["radius"])
+        del ph
+        grids.append(g)
+        jobs.append((g, q, qn, j["radius"], j["k"]))
+        nq_total += j["queries"]
+        n_ph += j["photons"]
+    torch.cuda.synchronize()
+    plain_flags = pm355.KNN_SORT_QUERIES | pm355.KNN_SMALL_BUFFER
+
+    def plain():
+        for g, q, qn, r, k in jobs:
+            ctx.gather_radiance(g, q, qn, r, k, flags=plain_flags)
+
+    def batched():
+        ctx.gather_batch(jobs, flags=pm355.KNN_SORT_QUERIES, want_count=False)
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps
+    for g in grids:
+        g.close()
+    return {
+        "workload": f"{n_jobs}-job gather mix: photons log-uniform 10k-4M per job ({n_ph} total), one query per "
+                    f"pixel of 128^2/256^2/640x480/1280x720 frames ({nq_total} total), k=64, r U[0.01,0.05], "
+                    f"cell=r; fused gather+radiance, cell-ordered",
+        "plain_ms": res["plain"] * 1e3,
+        "batched_ms": res["batched"] * 1e3,
+        "plain_queries_per_s": nq_total / res["plain"],
+        "batched_queries_per_s": nq_total / res["batched"],
+        "speedup": res["plain"] / res["batched"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +188,7 @@ def main():
     ap.add_argument("--sort-queries", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
+    ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -283,6 +340,9 @@ def main():
     if args.frames > 0:
         log(f"per-frame form: {args.frames} Cornell-box frames")
         res["per_frame"] = frame_bench(ctx, args.frames)
+    if args.batch_jobs > 0:
+        log(f"batched form: {args.batch_jobs}-job gather mix")
+        res["batched"] = batch_bench(ctx, args.batch_jobs)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)

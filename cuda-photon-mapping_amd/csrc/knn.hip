@@ -20,8 +20,10 @@
 //   5. optionally the radiance estimate runs on those k in the same wave.
 // A two-pass variant (histogram pass, then collect pass) is kept selectable
 // for tuning.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "pm_select.hpp"
 
@@ -247,12 +249,8 @@ __device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint
         sort_and_emit<4, CAP>(P, L, lane, q, cnt, m);
 }
 
-__device__ __forceinline__ bool query_box(const KnnParams& P, uint32_t& q, Box& B, float& qx, float& qy, float& qz) {
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t ql = blk * kWaves + wave;
-    if (ql >= P.nq) return false;
-    q = P.order ? P.order[ql] : ql;
+// The candidate box of query q: its cell +- rc, clamped (PhotonGrid.cu:262-272).
+__device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, float& qx, float& qy, float& qz) {
     qx = P.qx[q];
     qy = P.qy[q];
     qz = P.qz[q];
@@ -263,6 +261,16 @@ __device__ __forceinline__ bool query_box(const KnnParams& P, uint32_t& q, Box& 
         B.lo[a] = max(0, c - P.rc);
         B.hi[a] = min(P.G.g[a] - 1, c + P.rc);
     }
+}
+
+// Query of this wave in a one-wave-per-query launch (XCD-aware block order,
+// optional processing order); false past the end (wave-uniform).
+__device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t ql = blk * kWaves + wave;
+    if (ql >= P.nq) return false;
+    q = P.order ? P.order[ql] : ql;
     return true;
 }
 
@@ -271,14 +279,10 @@ __device__ __forceinline__ bool query_box(const KnnParams& P, uint32_t& q, Box& 
 // keys and tightens the acceptance bound. Exact: a dropped key has >= k kept
 // keys below it.
 template <int CAP>
-__global__ __launch_bounds__(256) void k_knn_stream(KnnParams P) {
-    __shared__ WaveLds<CAP> lds[kWaves];
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t q;
+__device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
     Box B;
     float qx, qy, qz;
-    if (!query_box(P, q, B, qx, qy, qz)) return;  // wave-uniform; no workgroup barriers below
-    WaveLds<CAP>& L = lds[threadIdx.x >> 6];
+    box_of(P, q, B, qx, qy, qz);
     const float T = P.T;
     const uint32_t k = (uint32_t)P.k;
     uint32_t cnt = 0, found = 0;
@@ -304,6 +308,61 @@ __global__ __launch_bounds__(256) void k_knn_stream(KnnParams P) {
     finish<CAP>(P, L, lane, q, cnt, found, khi);
 }
 
+// 512-key buffer: 17 KB of LDS per workgroup, so 7 waves per SIMD fit when
+// the registers do (68 VGPRs under this occupancy target, no spills).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_knn_stream512(KnnParams P) {
+    __shared__ WaveLds<512> lds[kWaves];
+    uint32_t q;
+    if (!query_of_wave(P, q)) return;  // wave-uniform; no workgroup barriers below
+    stream_one<512>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
+}
+
+// 1024-key buffer: fewer refinements per query, LDS-limited to 4 waves per SIMD.
+__global__ __launch_bounds__(256) void k_knn_stream1024(KnnParams P) {
+    __shared__ WaveLds<1024> lds[kWaves];
+    uint32_t q;
+    if (!query_of_wave(P, q)) return;
+    stream_one<1024>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
+}
+
+// Batched form: one persistent launch over the queries of many jobs (grids,
+// radii, k). Waves take tiles of kBatchTile positions from a device counter;
+// position p maps to a flattened query index (through the batch's processing
+// order, if any), whose job is found by binary search over the jobs' query
+// bases. Every wave exits once the counter passes the total.
+constexpr uint32_t kBatchTile = 4;
+
+template <int CAP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_knn_batch(const KnnParams* __restrict__ jobs, const uint32_t* __restrict__ qbase,
+                                                   uint32_t n_jobs, const uint32_t* __restrict__ order, uint32_t total,
+                                                   uint32_t* __restrict__ counter) {
+    __shared__ WaveLds<CAP> lds[kWaves];
+    const uint32_t lane = threadIdx.x & 63;
+    WaveLds<CAP>& L = lds[threadIdx.x >> 6];
+    for (;;) {
+        uint32_t pos = 0;
+        if (lane == 0) pos = atomicAdd(counter, kBatchTile);
+        pos = __builtin_amdgcn_readfirstlane(pos);
+        if (pos >= total) break;
+        const uint32_t end = min(total, pos + kBatchTile);
+        for (uint32_t p = pos; p < end; ++p) {
+            const uint32_t g = order ? order[p] : p;
+            uint32_t lo = 0, hi = n_jobs;  // last job with qbase <= g
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (qbase[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+            const KnnParams P = jobs[j];
+            stream_one<CAP>(P, L, lane, __builtin_amdgcn_readfirstlane(g - qbase[j]));
+            wave_sync();
+        }
+    }
+}
+
 // Two passes: histogram of in-radius d2 over the float-bit bins, then the
 // keys below the bin edge that keeps >= k are compacted into the buffer.
 __global__ __launch_bounds__(256) void k_knn_two_pass(KnnParams P) {
@@ -313,7 +372,8 @@ __global__ __launch_bounds__(256) void k_knn_two_pass(KnnParams P) {
     uint32_t q;
     Box B;
     float qx, qy, qz;
-    if (!query_box(P, q, B, qx, qy, qz)) return;
+    if (!query_of_wave(P, q)) return;
+    box_of(P, q, B, qx, qy, qz);
     WaveLds<CAP>& L = lds[threadIdx.x >> 6];
     const float T = P.T;
     const uint32_t bT = P.bT, k = (uint32_t)P.k;
@@ -394,6 +454,30 @@ __global__ __launch_bounds__(256) void k_radiance(KnnParams P) {
         P.rgb[P.nq + q] = res1 * coef;
         P.rgb[2 * (size_t)P.nq + q] = res2 * coef;
     }
+}
+
+// Batch processing order keys: flattened query g of job j gets
+// cell_base[j] + its cell, so one sort orders jobs, then cells within a job.
+__global__ __launch_bounds__(256) void k_batch_query_keys(const KnnParams* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ qbase,
+                                                         const uint32_t* __restrict__ cell_base, uint32_t n_jobs,
+                                                         uint32_t total, uint32_t* __restrict__ keys) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (qbase[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const KnnParams& P = jobs[lo];
+    const uint32_t q = g - qbase[lo];
+    int x = pmd::cell_coord(P.qx[q], P.G.mn[0], P.G.cell, P.G.g[0]);
+    int y = pmd::cell_coord(P.qy[q], P.G.mn[1], P.G.cell, P.G.g[1]);
+    int z = pmd::cell_coord(P.qz[q], P.G.mn[2], P.G.cell, P.G.g[2]);
+    keys[g] = cell_base[lo] + pmd::cell_index(P.G, x, y, z);
 }
 
 // Cell key of each query for the optional cell-ordered processing.
@@ -530,9 +614,9 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
     if (flags & PM_KNN_TWO_PASS)
         hipLaunchKernelGGL(k_knn_two_pass, grid_dim, block, 0, ctx->stream, P);
     else if (flags & PM_KNN_SMALL_BUFFER)
-        hipLaunchKernelGGL(k_knn_stream<512>, grid_dim, block, 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_knn_stream512, grid_dim, block, 0, ctx->stream, P);
     else
-        hipLaunchKernelGGL(k_knn_stream<1024>, grid_dim, block, 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_knn_stream1024, grid_dim, block, 0, ctx->stream, P);
     PM_CHECK_LAUNCH(ctx, "k_knn");
     return PM_OK;
 }
@@ -624,4 +708,79 @@ extern "C" pm_status pm_grid_query_order(pm_ctx* ctx, const pm_grid* grid, const
     if (nq == 0) return PM_OK;
     if (!d_order) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_query_order: null output");
     return query_order(ctx, grid, P, d_order);
+}
+
+extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uint32_t n_jobs, uint32_t flags) {
+    if (!ctx) return PM_ERR_INVALID;
+    if (n_jobs == 0) return PM_OK;
+    if (!jobs) return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: null job table");
+    std::vector<KnnParams> hp(n_jobs);
+    std::vector<uint32_t> qbase(n_jobs), cell_base(n_jobs);
+    uint64_t total = 0, cells = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const pm_gather_job& J = jobs[j];
+        KnnParams& P = hp[j];
+        pm_status st = setup(ctx, J.grid, J.d_queries3, J.nq, J.radius, J.k, flags, P, "pm_gather_batch");
+        if (st != PM_OK) return st;
+        if (J.nq > 0) {
+            if ((J.d_idx == nullptr) != (J.d_dist == nullptr))
+                return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: job %u needs both d_idx and d_dist", j);
+            if (J.d_rgb3 && !J.d_normals3)
+                return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: job %u radiance without normals", j);
+            if (!J.d_idx && !J.d_count && !J.d_rgb3)
+                return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: job %u has no output", j);
+        }
+        P.out_idx = J.d_idx;
+        P.out_dist = J.d_dist;
+        P.out_count = J.d_count;
+        P.nrm = J.d_rgb3 ? J.d_normals3 : nullptr;
+        P.rgb = J.d_rgb3;
+        P.order = nullptr;
+        qbase[j] = (uint32_t)total;
+        cell_base[j] = (uint32_t)cells;
+        total += J.nq;
+        cells += J.grid->desc.n_cells;
+        if (total > 0xFFFFFFFFull) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_gather_batch: over 2^32 queries");
+    }
+    if (total == 0) return PM_OK;
+    const bool sorted = (flags & PM_KNN_SORT_QUERIES) && cells <= 0xFFFFFFFFull;
+    const uint32_t nt = (uint32_t)total;
+    const size_t rs = sorted ? pm::radix_sort_scratch_bytes(nt) : 0;
+    const size_t bytes = pm::carve_size({(size_t)n_jobs * sizeof(KnnParams), (size_t)n_jobs * 4, (size_t)n_jobs * 4,
+                                         64, sorted ? (size_t)nt * 4 : 0, sorted ? (size_t)nt * 4 : 0,
+                                         sorted ? (size_t)nt * 4 : 0, rs});
+    void* work = nullptr;
+    pm_status st = pm::scratch_reserve(ctx, bytes, &work);
+    if (st != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    KnnParams* d_jobs = cv.take<KnnParams>(n_jobs);
+    uint32_t* d_qbase = cv.take<uint32_t>(n_jobs);
+    uint32_t* d_cbase = cv.take<uint32_t>(n_jobs);
+    uint32_t* d_counter = cv.take<uint32_t>(16);
+    PM_HIP(ctx, hipMemcpyAsync(d_jobs, hp.data(), hp.size() * sizeof(KnnParams), hipMemcpyHostToDevice, ctx->stream));
+    PM_HIP(ctx, hipMemcpyAsync(d_qbase, qbase.data(), qbase.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PM_HIP(ctx, hipMemcpyAsync(d_cbase, cell_base.data(), cell_base.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    // the tables are copied from pageable host vectors: wait before they go
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t* order = nullptr;
+    if (sorted) {
+        uint32_t* keys = cv.take<uint32_t>(nt);
+        uint32_t* skeys = cv.take<uint32_t>(nt);
+        uint32_t* ord = cv.take<uint32_t>(nt);
+        void* rw = cv.take<char>(rs);
+        hipLaunchKernelGGL(k_batch_query_keys, dim3(pm::div_up(nt, 256)), dim3(256), 0, ctx->stream, d_jobs, d_qbase,
+                           d_cbase, n_jobs, nt, keys);
+        PM_CHECK_LAUNCH(ctx, "k_batch_query_keys");
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < cells) ++bits;
+        if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, ord, nt, bits, rw)) != PM_OK) return st;
+        order = ord;
+    }
+    if ((st = pm::fill_u32(ctx, d_counter, 0u, 1)) != PM_OK) return st;
+    const uint32_t blocks =
+        std::min<uint32_t>(pm::div_up(pm::div_up(nt, kBatchTile), kWaves), (uint32_t)std::max(ctx->cu_count, 1) * 8u);
+    hipLaunchKernelGGL(k_knn_batch<512>, dim3(blocks), dim3(256), 0, ctx->stream, d_jobs, d_qbase, n_jobs, order, nt,
+                       d_counter);
+    PM_CHECK_LAUNCH(ctx, "k_knn_batch");
+    return PM_OK;
 }

@@ -83,6 +83,7 @@ _SIGS = {
     "pm_kdtree_destroy": (_I32, [_P]),
     "pm_kdtree_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _P, _P, _P]),
     "pm_accel_build": (_I32, [_P, _P, C.POINTER(_P)]),
+    "pm_gather_batch": (_I32, [_P, _P, _U32, _U32]),
     "pm_accel_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -100,6 +101,12 @@ class SceneRec(C.Structure):
     _fields_ = [("d_tris9", C.c_void_p), ("d_normals9", C.c_void_p), ("n_tris", C.c_uint32),
                 ("h_models", C.c_void_p), ("n_models", C.c_int32), ("light_model", C.c_int32),
                 ("light_intensity", C.c_float * 3), ("default_refr_index", C.c_float), ("accel", C.c_void_p)]
+
+
+class GatherJob(C.Structure):
+    _fields_ = [("grid", C.c_void_p), ("d_queries3", C.c_void_p), ("nq", C.c_uint32), ("radius", C.c_float),
+                ("k", C.c_int32), ("d_idx", C.c_void_p), ("d_dist", C.c_void_p), ("d_count", C.c_void_p),
+                ("d_normals3", C.c_void_p), ("d_rgb3", C.c_void_p)]
 
 
 class TraceStats(C.Structure):
@@ -262,6 +269,37 @@ class Context:
         self.check(self.lib.pm_grid_gather_radiance(self.h, grid.h, _ptr(q), _ptr(nrm), nq, float(radius), int(k),
                                                     int(flags), _ptr(rgb), _ptr(cnt)))
         return (rgb, cnt) if want_count else rgb
+
+    def gather_batch(self, jobs, flags: int = 0, want_knn: bool = False, want_count: bool = True):
+        """Batched gather over many (grid, queries3, normals3, radius, k) jobs in
+        one persistent launch. Returns per job a dict with "rgb" (when normals
+        are given), "count", and "idx"/"dist" when want_knn."""
+        import torch
+
+        recs = (GatherJob * max(len(jobs), 1))()
+        outs, keep = [], []
+        for i, (grid, q3, n3, radius, k) in enumerate(jobs):
+            q = _f32_dev(q3, self.device)
+            nq = q.shape[1]
+            o = {}
+            r = recs[i]
+            r.grid, r.d_queries3, r.nq, r.radius, r.k = grid.h, _ptr(q), nq, float(radius), int(k)
+            keep.append(q)
+            if n3 is not None:
+                nrm = _f32_dev(n3, self.device)
+                keep.append(nrm)
+                o["rgb"] = torch.empty((3, nq), dtype=torch.float32, device=self.device)
+                r.d_normals3, r.d_rgb3 = _ptr(nrm), _ptr(o["rgb"])
+            if want_count or n3 is None:
+                o["count"] = torch.empty((nq,), dtype=torch.int32, device=self.device)
+                r.d_count = _ptr(o["count"])
+            if want_knn:
+                o["idx"] = torch.empty((nq, k), dtype=torch.int32, device=self.device)
+                o["dist"] = torch.empty((nq, k), dtype=torch.float32, device=self.device)
+                r.d_idx, r.d_dist = _ptr(o["idx"]), _ptr(o["dist"])
+            outs.append(o)
+        self.check(self.lib.pm_gather_batch(self.h, C.cast(recs, C.c_void_p), len(jobs), int(flags)))
+        return outs
 
     # -- kd-tree / model AABB ---------------------------------------------------
     def kdtree_build(self, photons9) -> "KdTree":

@@ -105,3 +105,83 @@ def write_photon_file(path: str, photons9: np.ndarray) -> None:
     with open(path, "wb") as f:
         f.write(np.uint64(n).tobytes())
         f.write(np.ascontiguousarray(photons9.T, dtype="<f4").tobytes())
+
+
+def batch_mix(seed: int = 7, n_jobs: int = 256):
+    """Job sizes of the batched workload: per job a photon count log-uniform
+    in 10k-4M, a frame from {128x128, 256x256, 640x480, 1280x720} whose pixel
+    count is the job's query count, k = 64 and r uniform in [0.01, 0.05]
+    (grid cell = r). Returns a list of dicts."""
+    rng = _rng(seed, 11)
+    res = [(128, 128), (256, 256), (640, 480), (1280, 720)]
+    out = []
+    for j in range(n_jobs):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
+        w, h = res[int(rng.integers(0, 4))]
+        out.append({"photons": n, "queries": w * h, "width": w, "height": h, "k": 64,
+                    "radius": float(rng.uniform(0.01, 0.05)), "seed": seed * 1000 + j})
+    return out
+
+
+def gather_inputs_device(n_photons: int, n_queries: int, seed: int, device, n_clusters: int = 64,
+                         cluster_frac: float = 0.3, sigma: float = 0.005):
+    """The gather microbenchmark's distributions generated on the device with
+    torch's seeded generator (for the many-job batched workload, where host
+    generation would dominate): returns (photons9, queries3, normals3) as
+    float32 device tensors and the unit-cube AABB."""
+    import math
+
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+
+    def faces(n):
+        face = torch.randint(0, 6, (n,), generator=g, device=device)
+        uv = torch.rand((2, n), generator=g, device=device)
+        axis, side = face // 2, (face % 2).float()
+        pos = torch.empty((3, n), device=device)
+        nrm = torch.zeros((3, n), device=device)
+        for a in range(3):
+            sel = axis == a
+            o1, o2 = [b for b in range(3) if b != a]
+            pos[a] = torch.where(sel, side, pos[a])
+            pos[o1] = torch.where(sel, uv[0], pos[o1])
+            pos[o2] = torch.where(sel, uv[1], pos[o2])
+            nrm[a] = torch.where(sel, 1.0 - 2.0 * side, nrm[a])
+        return pos, nrm
+
+    n_cl = int(round(n_photons * cluster_frac))
+    n_cl -= n_cl % max(1, n_clusters)
+    pos_u, nrm_u = faces(n_photons - n_cl)
+    centres = torch.rand((n_clusters, 2), generator=g, device=device)
+    cid = torch.arange(n_cl, device=device) // max(1, n_cl // max(1, n_clusters))
+    cid = cid.clamp_max(n_clusters - 1)
+    dev = torch.randn((2, n_cl), generator=g, device=device) * sigma
+    pos_c = torch.zeros((3, n_cl), device=device)
+    pos_c[0] = (centres[cid, 0] + dev[0]).clamp(0, 1)
+    pos_c[2] = (centres[cid, 1] + dev[1]).clamp(0, 1)
+    nrm_c = torch.zeros((3, n_cl), device=device)
+    nrm_c[1] = 1.0
+    pos = torch.cat([pos_u, pos_c], 1)
+    nrm = torch.cat([nrm_u, nrm_c], 1)
+    ph = torch.empty((9, n_photons), device=device)
+    ph[0:3] = pos
+    ph[3:6] = torch.rand((3, n_photons), generator=g, device=device) * 1e-6
+    # -w, w uniform on the hemisphere about the axis-aligned normal
+    z = torch.rand(n_photons, generator=g, device=device)
+    phi = torch.rand(n_photons, generator=g, device=device) * (2 * math.pi)
+    s = (1 - z * z).clamp_min(0).sqrt()
+    loc = torch.stack([s * phi.cos(), s * phi.sin(), z])
+    axis = nrm.abs().argmax(0)
+    sign = nrm.gather(0, axis[None])[0]
+    w = torch.empty((3, n_photons), device=device)
+    for a in range(3):
+        sel = axis == a
+        o1, o2 = [b for b in range(3) if b != a]
+        w[a] = torch.where(sel, loc[2] * sign, w[a])
+        w[o1] = torch.where(sel, loc[0], w[o1])
+        w[o2] = torch.where(sel, loc[1], w[o2])
+    ph[6:9] = -w
+    q, qn = faces(n_queries)
+    return ph.contiguous(), q.contiguous(), qn.contiguous(), (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)

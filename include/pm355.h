@@ -132,6 +132,30 @@ pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, const float*
 pm_status pm_grid_query_order(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
                               uint32_t* d_order);
 
+/* Batched gather: the queries of many jobs (each with its own grid, radius,
+ * k and outputs) in one persistent launch over a shared work queue, the
+ * batched form of find_nearests + get_color_from_photon_grid for many scenes
+ * and frames at once (Drawer.cu:87-101 launches one frame at a time). Each
+ * job's outputs are exactly those of pm_grid_knn (d_idx + d_dist, d_count)
+ * and/or pm_grid_gather_radiance (d_rgb3 from d_normals3) on its own.
+ * flags: PM_KNN_SORT_QUERIES orders all queries by (job, cell) with one
+ * sort. Calls on one context are ordered on its stream; the job table is
+ * read before the call returns. */
+typedef struct {
+    const pm_grid* grid;
+    const float* d_queries3;  /* [3][nq] */
+    uint32_t nq;
+    float radius;
+    int32_t k;                /* 1..256 */
+    uint32_t* d_idx;          /* optional [nq][k] grid slots, with d_dist */
+    float* d_dist;            /* optional [nq][k] */
+    uint32_t* d_count;        /* optional [nq] */
+    const float* d_normals3;  /* [3][nq], needed with d_rgb3 */
+    float* d_rgb3;            /* optional [3][nq] radiance */
+} pm_gather_job;
+
+pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* h_jobs, uint32_t n_jobs, uint32_t flags);
+
 /* Per-query candidate statistics of the gather box (cells visited C_q and
  * photons in them P_q, summed), used for the algorithmic-byte count. */
 pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
