@@ -152,8 +152,11 @@ def batch_bench(ctx, n_jobs, reps=2):
     def batched():
         ctx.gather_batch(jobs, flags=pm355.KNN_SORT_QUERIES, want_count=False)
 
+    def batched_queue():
+        ctx.gather_batch(jobs, flags=pm355.KNN_SORT_QUERIES | pm355.KNN_BATCH_QUEUE, want_count=False)
+
     res = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
+    for name, fn in (("plain", plain), ("batched", batched), ("batched_queue", batched_queue)):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -169,6 +172,7 @@ def batch_bench(ctx, n_jobs, reps=2):
                     f"cell=r; fused gather+radiance, cell-ordered",
         "plain_ms": res["plain"] * 1e3,
         "batched_ms": res["batched"] * 1e3,
+        "batched_queue_ms": res["batched_queue"] * 1e3,
         "plain_queries_per_s": nq_total / res["plain"],
         "batched_queries_per_s": nq_total / res["batched"],
         "speedup": res["plain"] / res["batched"],

@@ -33,6 +33,7 @@ using pmd::GridGeom;
 using namespace pms;
 
 constexpr int kWaves = 4;  // queries per 256-thread workgroup
+constexpr int kMaxK = 2048;
 
 struct KnnParams {
     GridGeom G;
@@ -325,11 +326,134 @@ __global__ __launch_bounds__(256) void k_knn_stream1024(KnnParams P) {
     stream_one<1024>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
 }
 
-// Batched form: one persistent launch over the queries of many jobs (grids,
-// radii, k). Waves take tiles of kBatchTile positions from a device counter;
-// position p maps to a flattened query index (through the batch's processing
-// order, if any), whose job is found by binary search over the jobs' query
-// bases. Every wave exits once the counter passes the total.
+// Large k (257..2048, e.g. the shipped renderer's k = 2000 / 400): one wave
+// per workgroup with a CAP-key LDS buffer (CAP >= 2k). The stream is the same;
+// the final selection narrows the buffer to exactly the k smallest keys,
+// which a bitonic network sorts in LDS.
+template <int CAP>
+__global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
+    __shared__ WaveLds<CAP> L;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (blk >= P.nq) return;
+    const uint32_t q = P.order ? P.order[blk] : blk;
+    Box B;
+    float qx, qy, qz;
+    box_of(P, q, B, qx, qy, qz);
+    const float T = P.T;
+    const uint32_t k = (uint32_t)P.k;
+    uint32_t cnt = 0, found = 0;
+    uint64_t khi = ~0ull;
+    walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+        bool in = valid && d2 <= T;
+        found += (uint32_t)__popcll(__ballot(in));
+        uint64_t key = make_key(d2, slot);
+        bool acc = in && key < khi;
+        uint64_t bal = __ballot(acc);
+        uint32_t na = (uint32_t)__popcll(bal);
+        if (cnt + na > (uint32_t)CAP) {
+            wave_sync();
+            cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
+            acc = acc && key < khi;
+            bal = __ballot(acc);
+            na = (uint32_t)__popcll(bal);
+        }
+        if (acc) L.buf[cnt + mbcnt(bal)] = key;
+        cnt += na;
+    });
+    wave_sync();
+    const uint32_t m = found < k ? found : k;
+    cnt = buf_select<CAP>(L, lane, cnt, k, k, P.bT, khi);  // exactly min(k, found) keys remain
+    wave_sync();
+    uint32_t n2 = 64;
+    while (n2 < cnt) n2 <<= 1;
+    for (uint32_t e = cnt + lane; e < n2; e += 64) L.buf[e] = ~0ull;
+    wave_sync();
+    lds_bitonic_sort(L.buf, n2, lane);
+    if (P.out_idx) {
+        for (uint32_t e = lane; e < k; e += 64) {
+            const bool keep = e < m;
+            const uint64_t v = keep ? L.buf[e] : 0ull;
+            P.out_idx[(size_t)q * P.k + e] = keep ? (uint32_t)v : 0xFFFFFFFFu;
+            P.out_dist[(size_t)q * P.k + e] = keep ? sqrtf(__uint_as_float((uint32_t)(v >> 32))) : __builtin_inff();
+        }
+    }
+    if (P.out_count && lane == 0) P.out_count[q] = m;
+    if (P.rgb) {
+        // Raytracer.cuh:110-138 over the m kept photons
+        const float alpha = 1.818f, beta = 1.953f;
+        const float pi_f = 3.141592654f;
+        float res0 = 0.f, res1 = 0.f, res2 = 0.f, rmax = 0.f;
+        if (m > 0) {
+            rmax = sqrtf(__uint_as_float((uint32_t)(L.buf[m - 1] >> 32)));
+            const float filter_r = __fmul_rn(rmax, rmax);
+            const float denom = 1.f - expf(-beta);
+            const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
+            for (uint32_t e = lane; e < m; e += 64) {
+                const uint64_t v = L.buf[e];
+                const uint32_t idx = (uint32_t)v;
+                float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+                float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
+                if (cosNL > 0.f) {
+                    float d = sqrtf(__uint_as_float((uint32_t)(v >> 32)));
+                    float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                    float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                    res0 += P.power[idx] * w;
+                    res1 += P.power[P.n + idx] * w;
+                    res2 += P.power[2 * (size_t)P.n + idx] * w;
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            res0 += __shfl_xor(res0, d, 64);
+            res1 += __shfl_xor(res1, d, 64);
+            res2 += __shfl_xor(res2, d, 64);
+        }
+        if (lane == 0) {
+            float coef = 1.f;
+            if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
+            P.rgb[q] = m > 0 ? res0 * coef : 0.f;
+            P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
+            P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
+        }
+    }
+}
+
+// Batched forms: the queries of many jobs (grids, radii, k) in one launch.
+// A position p maps to a flattened query index g (through the batch's
+// processing order, if any), whose job is the last one with qbase <= g.
+__device__ __forceinline__ uint32_t job_of(const uint32_t* __restrict__ qbase, uint32_t n_jobs, uint32_t g) {
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (qbase[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// One wave per position; the 2-D grid keeps every dimension's work-item
+// count below 2^32, and the linear workgroup id is remapped so each XCD
+// takes one contiguous range of positions (neighbouring cells in its L2).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_knn_batch_grid(
+    const KnnParams* __restrict__ jobs, const uint32_t* __restrict__ qbase, uint32_t n_jobs,
+    const uint32_t* __restrict__ order, uint32_t total) {
+    __shared__ WaveLds<512> lds[kWaves];
+    const uint32_t nb = gridDim.x * gridDim.y;
+    const uint32_t blk = pmd::xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+    const uint32_t p = blk * kWaves + (threadIdx.x >> 6);
+    if (p >= total) return;  // wave-uniform
+    const uint32_t g = order ? order[p] : p;
+    const uint32_t j = job_of(qbase, n_jobs, g);
+    const KnnParams P = jobs[j];
+    stream_one<512>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, __builtin_amdgcn_readfirstlane(g - qbase[j]));
+}
+
+// Persistent form: workgroups stay resident and their waves take tiles of
+// kBatchTile positions from a device work counter until it passes the total.
 constexpr uint32_t kBatchTile = 4;
 
 template <int CAP>
@@ -347,15 +471,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const uint32_t end = min(total, pos + kBatchTile);
         for (uint32_t p = pos; p < end; ++p) {
             const uint32_t g = order ? order[p] : p;
-            uint32_t lo = 0, hi = n_jobs;  // last job with qbase <= g
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (qbase[mid] <= g)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+            const uint32_t j = job_of(qbase, n_jobs, g);
             const KnnParams P = jobs[j];
             stream_one<CAP>(P, L, lane, __builtin_amdgcn_readfirstlane(g - qbase[j]));
             wave_sync();
@@ -539,7 +655,7 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     if (!grid) return pm::fail(ctx, PM_ERR_INVALID, "%s: null grid", who);
     if (nq > 0 && !d_q) return pm::fail(ctx, PM_ERR_INVALID, "%s: null queries", who);
     if (!(radius >= 0.f) || !std::isfinite(radius)) return pm::fail(ctx, PM_ERR_INVALID, "%s: bad radius", who);
-    if (k < 1 || k > 256) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: k=%d outside [1, 256]", who, k);
+    if (k < 1 || k > kMaxK) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: k=%d outside [1, %d]", who, k, kMaxK);
     const pm_grid_desc& D = grid->desc;
     P = KnnParams{};
     for (int a = 0; a < 3; ++a) {
@@ -609,6 +725,16 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
         uint32_t* order = cv.take<uint32_t>(P.nq);
         if ((st = query_order(ctx, grid, P, order)) != PM_OK) return st;
         P.order = order;
+    }
+    if (P.k > 1024) {
+        hipLaunchKernelGGL(k_knn_large<4096>, dim3(P.nq), dim3(64), 0, ctx->stream, P);
+        PM_CHECK_LAUNCH(ctx, "k_knn_large");
+        return PM_OK;
+    }
+    if (P.k > 256) {
+        hipLaunchKernelGGL(k_knn_large<2048>, dim3(P.nq), dim3(64), 0, ctx->stream, P);
+        PM_CHECK_LAUNCH(ctx, "k_knn_large");
+        return PM_OK;
     }
     const dim3 grid_dim(pm::div_up(P.nq, kWaves)), block(256);
     if (flags & PM_KNN_TWO_PASS)
@@ -722,6 +848,7 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
         KnnParams& P = hp[j];
         pm_status st = setup(ctx, J.grid, J.d_queries3, J.nq, J.radius, J.k, flags, P, "pm_gather_batch");
         if (st != PM_OK) return st;
+        if (J.k > 256) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_gather_batch: job %u k=%d above 256", j, J.k);
         if (J.nq > 0) {
             if ((J.d_idx == nullptr) != (J.d_dist == nullptr))
                 return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: job %u needs both d_idx and d_dist", j);
@@ -776,11 +903,18 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
         if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, ord, nt, bits, rw)) != PM_OK) return st;
         order = ord;
     }
-    if ((st = pm::fill_u32(ctx, d_counter, 0u, 1)) != PM_OK) return st;
-    const uint32_t blocks =
-        std::min<uint32_t>(pm::div_up(pm::div_up(nt, kBatchTile), kWaves), (uint32_t)std::max(ctx->cu_count, 1) * 8u);
-    hipLaunchKernelGGL(k_knn_batch<512>, dim3(blocks), dim3(256), 0, ctx->stream, d_jobs, d_qbase, n_jobs, order, nt,
-                       d_counter);
+    if (flags & PM_KNN_BATCH_QUEUE) {
+        if ((st = pm::fill_u32(ctx, d_counter, 0u, 1)) != PM_OK) return st;
+        const uint32_t blocks = std::min<uint32_t>(pm::div_up(pm::div_up(nt, kBatchTile), kWaves),
+                                                   (uint32_t)std::max(ctx->cu_count, 1) * 8u);
+        hipLaunchKernelGGL(k_knn_batch<512>, dim3(blocks), dim3(256), 0, ctx->stream, d_jobs, d_qbase, n_jobs, order,
+                           nt, d_counter);
+    } else {
+        const uint32_t blocks = pm::div_up(nt, kWaves);
+        const uint32_t gx = std::min<uint32_t>(blocks, 65536u);
+        hipLaunchKernelGGL(k_knn_batch_grid, dim3(gx, pm::div_up(blocks, gx)), dim3(256), 0, ctx->stream, d_jobs,
+                           d_qbase, n_jobs, order, nt);
+    }
     PM_CHECK_LAUNCH(ctx, "k_knn_batch");
     return PM_OK;
 }

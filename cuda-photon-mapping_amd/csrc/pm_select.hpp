@@ -134,6 +134,24 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
     return out;
 }
 
+// Ascending bitonic sort of buf[0, n) in LDS by one wave; n is a power of two.
+__device__ __forceinline__ void lds_bitonic_sort(uint64_t* buf, uint32_t n, uint32_t lane) {
+    for (uint32_t size = 2; size <= n; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = lane; i < n / 2; i += 64) {
+                const uint32_t a = 2 * stride * (i / stride) + (i % stride), b = a + stride;
+                const bool asc = (a & size) == 0;
+                const uint64_t x = buf[a], y = buf[b];
+                if (asc ? (x > y) : (x < y)) {
+                    buf[a] = y;
+                    buf[b] = x;
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void bitonic_sort(uint64_t (&v)[R], uint32_t lane) {
     constexpr int N = 64 * R;

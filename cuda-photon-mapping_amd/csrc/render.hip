@@ -288,6 +288,8 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
     if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
         return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: empty scene");
     const pm_render_params& R = *params;
+    if (R.k_global < 1 || R.k_global > 2048 || R.k_caustic < 1 || R.k_caustic > 2048)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: k outside [1, 2048]");
     if (R.max_depth < 1 || R.max_depth > kMaxDepth)
         return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: max_depth outside [1, %d]", kMaxDepth);
     if (camera->width <= 0 || camera->height <= 0)
@@ -345,8 +347,9 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
     const uint32_t nq = (uint32_t)nq64;
 
     // query block + gather outputs live in a second carve of the frame arena
+    const size_t kc = (size_t)std::max(1, R.k_caustic);
     size_t qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
-                                    (size_t)nq * 4, (size_t)nq * 4 * 256, (size_t)nq * 4 * 256});
+                                    (size_t)nq * 4, (size_t)nq * 4 * kc, (size_t)nq * 4 * kc});
     // (index/distance buffers are only needed for the reference-caustic mode)
     const bool quirk = (R.flags & PM_RENDER_REFERENCE_CAUSTIC) != 0 && caustic_grid;
     if (!quirk) qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,

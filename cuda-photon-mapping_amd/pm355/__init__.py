@@ -31,6 +31,7 @@ KNN_SORT_QUERIES = 1
 KNN_NO_XCD_REMAP = 2
 KNN_TWO_PASS = 1 << 4
 KNN_SMALL_BUFFER = 1 << 5
+KNN_BATCH_QUEUE = 1 << 6
 KNN_VARIANTS = {"stream1024": 0, "stream512": KNN_SMALL_BUFFER, "two_pass": KNN_TWO_PASS}
 
 _P = C.c_void_p

@@ -48,7 +48,8 @@ enum {
     PM_KNN_SORT_QUERIES = 1u << 0, /* process queries in cell order (outputs stay in caller order) */
     PM_KNN_NO_XCD_REMAP = 1u << 1, /* disable the XCD-aware block mapping */
     PM_KNN_TWO_PASS = 1u << 4,     /* tuning: histogram pass + collect pass instead of one streaming pass */
-    PM_KNN_SMALL_BUFFER = 1u << 5  /* tuning: 512-key LDS buffer per query (higher occupancy) */
+    PM_KNN_SMALL_BUFFER = 1u << 5, /* tuning: 512-key LDS buffer per query (higher occupancy) */
+    PM_KNN_BATCH_QUEUE = 1u << 6   /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
 };
 
 typedef struct pm_ctx pm_ctx;
@@ -109,7 +110,9 @@ pm_status pm_grid_destroy(pm_grid* grid);
  * the k smallest by (d2, grid index) with d2 = (dx*dx + dy*dy) + dz*dz in
  * FP32. Outputs are [nq][k] in ascending order: d_idx = grid slot (as the
  * heap's PhotonMaxHeapItem::idx), d_dist = sqrtf(d2); slots past d_count[q]
- * hold idx 0xFFFFFFFF and dist +inf. 1 <= k <= 256. */
+ * hold idx 0xFFFFFFFF and dist +inf. 1 <= k <= 2048; k > 256 (the shipped
+ * renderer uses k = 2000 and 400) runs a one-wave-per-workgroup kernel that
+ * sorts in LDS. */
 pm_status pm_grid_knn(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq, float radius,
                       int32_t k, uint32_t flags, uint32_t* d_idx, float* d_dist, uint32_t* d_count);
 

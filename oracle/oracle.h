@@ -103,6 +103,8 @@ int orc_render(const float* tris9, const float* normals9, uint32_t n_tris, const
                const void* camera_raw, const int32_t* iparams, const float* fparams, int32_t threads,
                float* out_rgb3, uint8_t* out_rgb8, uint64_t* out_stats);
 
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
 int orc_trace_photons(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
                       int32_t n_models, int32_t light_model, const float intensity[3], float default_ior,
                       uint64_t seed, uint32_t n_global, uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,

@@ -222,6 +222,11 @@ struct Tracer {
 static bool load_models(const void* models_raw, int32_t n_models, uint32_t n_tris, const float* tris9,
                         std::vector<Model>& M);
 
+// One Philox4x32-10 block, for the generator's known-answer tests.
+extern "C" void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    orc::Philox::block(ctr, key[0], key[1], out);
+}
+
 extern "C" int orc_scene_hit(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
                              int32_t n_models, const float* rays6, uint32_t nr, float* out_t, int32_t* out_model,
                              float* out_n3) {

@@ -175,6 +175,19 @@ def radius_threshold(r):
     return float(np.float32(lib().orc_radius_threshold(float(r))))
 
 
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block of the tracer's generator: 4 u32 counter words,
+    2 u32 key words -> 4 u32."""
+    L = lib()
+    L.orc_philox4x32_10.argtypes = [_P, _P, _P]
+    L.orc_philox4x32_10.restype = None
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.empty(4, np.uint32)
+    L.orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return tuple(int(v) for v in out)
+
+
 def trace_photons(scene, seed, n_global, n_caustic, max_bounces=10, max_paths=1 << 40):
     """Oracle photon tracing on a pm355.scenes.Scene; returns (global9,
     caustic9, stats) with stats = (paths, n_global_stored, n_caustic_stored)."""

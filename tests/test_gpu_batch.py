@@ -25,7 +25,7 @@ def _jobs(seed):
     return specs
 
 
-@pytest.mark.parametrize("flags", [0, 1])
+@pytest.mark.parametrize("flags", [0, 1, 1 | (1 << 6), 1 << 6], ids=["grid", "grid-sorted", "queue-sorted", "queue"])
 def test_batch_equals_single_jobs(ctx, flags):
     specs = _jobs(3)
     grids = [ctx.grid_build(ph, aabb, cell_size=cell, layout=layout) for ph, _, _, aabb, _, _, cell, layout in specs]

@@ -155,6 +155,38 @@ def test_knn_duplicates_force_refinement(ctx):
             _compare_knn(gi, gd, gc, oi, od, oc)
 
 
+@pytest.mark.parametrize("k,r", [(257, 0.05), (400, 0.2), (1000, 0.08), (2000, 0.1)])
+def test_large_k_parity(ctx, k, r):
+    """k above the register sort (the shipped renderer's k = 2000 / 400 with
+    r = 0.1 / 0.2): the large-k kernel against the oracle, kNN and radiance."""
+    ph, q, qn, aabb = gather_microbench(seed=k, n_photons=300_000, n_queries=1500, n_clusters=16)
+    q[:, 0] = ph[0:3, -1]
+    ph[0:3, 1000:2500] = ph[0:3, -1:]  # exact duplicates: ties beyond the LDS buffer
+    g = ctx.grid_build(ph, aabb, cell_size=0.05)
+    og = O.OracleGrid(ph, aabb, cell_size=0.05)
+    oi, od, oc = og.knn(q, r, k)
+    for flags in (0, pm355.KNN_SORT_QUERIES):
+        gi, gd, gc = ctx.knn(g, q, r, k, flags=flags)
+        ctx.sync()
+        _compare_knn(gi, gd, gc, oi, od, oc)
+    o32, o64 = O.radiance(oi, od, oc, og.photons, qn, k)
+    rgb, cnt = ctx.gather_radiance(g, q, qn, r, k, want_count=True)
+    ctx.sync()
+    assert np.array_equal(_np(cnt, np.uint32), oc)
+    got = _np(rgb).astype(np.float64)
+    scale = np.maximum(np.abs(o64), 1e-30)
+    rel = np.abs(got - o64) / scale
+    rel[o64 == 0] = np.abs(got[o64 == 0])
+    assert rel.max() <= RADIANCE_RTOL, rel.max()
+
+
+def test_k_above_limit_is_rejected(ctx):
+    ph, q, qn, aabb = gather_microbench(seed=2, n_photons=1000, n_queries=10, n_clusters=1)
+    g = ctx.grid_build(ph, aabb, cell_size=0.1)
+    with pytest.raises(pm355.PMError):
+        ctx.knn(g, q, 0.1, 2049)
+
+
 def test_knn_matches_brute_force_small(ctx):
     rng = np.random.default_rng(4)
     ph = np.zeros((9, 3000), np.float32)

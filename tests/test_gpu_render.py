@@ -30,7 +30,8 @@ def setup(ctx, request):
 
 
 @pytest.mark.parametrize("max_depth,k,r,cell,flags", [(4, 50, 0.05, 0.05, 0), (4, 50, 0.05, 0.05, 1),
-                                                      (1, 20, 0.08, 0.04, 0), (6, 100, 0.02, 0.02, 0)])
+                                                      (1, 20, 0.08, 0.04, 0), (6, 100, 0.02, 0.02, 0),
+                                                      (4, 2000, 0.1, 0.1, 1)])
 def test_frame_parity(ctx, setup, max_depth, k, r, cell, flags):
     scene, ds, aabb, g, c, gh, ch = setup
     gg = ctx.grid_build(g, aabb, cell_size=cell)

@@ -43,6 +43,20 @@ def _mt_brute(scene, o, d):
     return best_t, best_tri
 
 
+def test_philox_known_answers():
+    """The generator is Philox4x32-10 as published (Salmon et al., SC'11):
+    the known-answer vectors of the Random123 distribution (counter, key ->
+    output) for the all-zero, all-ones and pi-digit inputs."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, out in kat:
+        assert O.philox4x32_10(ctr, key) == out
+
+
 def test_scene_hit_matches_float64_brute_force(scene):
     rng = np.random.default_rng(0)
     n = 400
