@@ -43,25 +43,6 @@ pm_status scratch_reserve(pm_ctx* ctx, size_t bytes, void** out) {
     return PM_OK;
 }
 
-pm_status frame_reserve(pm_ctx* ctx, size_t bytes, void** out) {
-    if (bytes > ctx->frame_bytes) {
-        if (ctx->frame) {
-            PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            PM_HIP(ctx, hipFree(ctx->frame));
-            ctx->frame = nullptr;
-            ctx->frame_bytes = 0;
-        }
-        size_t want = bytes + bytes / 8 + (1u << 20);
-        hipError_t e = hipMalloc(&ctx->frame, want);
-        if (e != hipSuccess) {
-            ctx->frame = nullptr;
-            return hip_fail(ctx, e, "hipMalloc(frame)");
-        }
-        ctx->frame_bytes = want;
-    }
-    *out = ctx->frame;
-    return PM_OK;
-}
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,7 +89,7 @@ pm_status pm_ctx_destroy(pm_ctx* ctx) {
     if (!ctx) return PM_OK;
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
-    if (ctx->frame) (void)hipFree(ctx->frame);
+    pm_frame_destroy(ctx->frame_state);
     if (ctx->scan_state) (void)hipFree(ctx->scan_state);
     if (ctx->scan_counter) (void)hipFree(ctx->scan_counter);
     delete ctx;

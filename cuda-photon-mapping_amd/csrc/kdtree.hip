@@ -198,6 +198,158 @@ __global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0
     plane[node] = s.y > s.x ? plane_of(A0, A1, A2, s.x, s.y) : (int8_t)0;
 }
 
+// ---- bottom levels in LDS ---------------------------------------------------
+// Once every segment holds at most kLocal records, one workgroup finishes a
+// whole subtree in LDS: the same median / stable-partition / child-plane
+// steps as the level loop above, on the subtree's slice of the three axis
+// orders, with workgroup scans instead of device scans.
+constexpr int kLocal = 1024;
+constexpr int kLocalThreads = 256;
+
+struct LocalLds {
+    Rec a[3][kLocal];
+    Rec tmp[kLocal];
+    uint32_t pref[kLocal];
+    uint16_t node[kLocal];      // local node (at the current depth) of each position, 0xFFFF = final
+    uint2 seg[2][kLocal];       // node segments, current and next depth
+    int8_t plane[2][kLocal];
+    uint2 med[kLocal];
+    uint32_t mpos[kLocal / 2][3];
+    uint32_t wsum[kLocalThreads / 64];
+};
+
+// Exclusive scan of flags[0, m) (0/1) into L.pref; returns the total.
+__device__ __forceinline__ void local_scan(LocalLds& L, const uint8_t* fl, uint32_t m) {
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr uint32_t per = kLocal / kLocalThreads;  // 4 positions per thread
+    uint32_t v[per], sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) {
+        uint32_t j = t * per + i;
+        v[i] = j < m ? fl[j] : 0u;
+        sum += v[i];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += y;
+    }
+    if (lane == 63) L.wsum[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += L.wsum[w];
+    uint32_t run = before + inc - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) {
+        uint32_t j = t * per + i;
+        if (j < m) L.pref[j] = run;
+        run += v[i];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
+                                                                 const Rec* __restrict__ A2, uint32_t first,
+                                                                 uint32_t depth_left, const uint2* __restrict__ gseg,
+                                                                 const int8_t* __restrict__ gplane,
+                                                                 uint32_t* __restrict__ node_idx,
+                                                                 int8_t* __restrict__ node_plane) {
+    __shared__ LocalLds L;
+    __shared__ uint8_t flag[kLocal];
+    const uint32_t root = first + blockIdx.x, t = threadIdx.x;
+    const uint2 rs = gseg[root];
+    if (rs.y <= rs.x || rs.y - rs.x > (uint32_t)kLocal) return;  // empty subtree; larger ones never reach here
+    const uint32_t m = rs.y - rs.x;
+    const Rec* As[3] = {A0, A1, A2};
+    for (int a = 0; a < 3; ++a)
+        for (uint32_t j = t; j < m; j += kLocalThreads) L.a[a][j] = As[a][rs.x + j];
+    for (uint32_t j = t; j < m; j += kLocalThreads) L.node[j] = 0;
+    if (t == 0) {
+        L.seg[0][0] = make_uint2(0, m);
+        L.plane[0][0] = gplane[root];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t d = 0; d < depth_left; ++d) {
+        const uint32_t nd = 1u << d;
+        const uint32_t gfirst = (root + 1) * nd - 1;  // global index of local node 0 at this depth
+        const bool last = d + 1 == depth_left;
+        // medians (k_level_nodes) and the children's segments
+        for (uint32_t i = t; i < nd; i += kLocalThreads) {
+            uint2 s = L.seg[cur][i];
+            uint2 cl = make_uint2(0, 0), cr = make_uint2(0, 0);
+            if (s.y > s.x) {
+                int8_t p = L.plane[cur][i];
+                uint32_t mid = s.x + (s.y - s.x) / 2;
+                Rec r = L.a[p][mid];
+                node_idx[gfirst + i] = r.id;
+                node_plane[gfirst + i] = p;
+                L.med[i] = make_uint2(__float_as_uint(r.c[p]), r.id);
+                if (s.y - s.x > 1) {
+                    cl = make_uint2(s.x, mid);
+                    cr = make_uint2(mid + 1, s.y);
+                }
+            }
+            if (!last) {
+                L.seg[cur ^ 1][2 * i] = cl;
+                L.seg[cur ^ 1][2 * i + 1] = cr;
+            }
+        }
+        __syncthreads();
+        if (last) break;
+        // stable partition of each axis order around its node's median
+        for (int a = 0; a < 3; ++a) {
+            for (uint32_t j = t; j < m; j += kLocalThreads) {
+                const uint32_t i = L.node[j];
+                uint8_t f = 0;
+                if (i != 0xFFFFu) {
+                    int sd = side_of(L.a[a][j], L.plane[cur][i], L.med[i]);
+                    f = sd == 0;
+                    if (sd == 1) L.mpos[i][a] = j;
+                }
+                flag[j] = f;
+            }
+            __syncthreads();
+            local_scan(L, flag, m);
+            for (uint32_t j = t; j < m; j += kLocalThreads) {
+                const uint32_t i = L.node[j];
+                uint32_t dst = j;
+                if (i != 0xFFFFu) {
+                    const uint2 sg = L.seg[cur][i];
+                    const uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
+                    const uint32_t before_l = L.pref[j] - L.pref[from];
+                    if (flag[j]) {
+                        dst = from + before_l;
+                    } else {
+                        const uint32_t mp = L.mpos[i][a];
+                        dst = mp == j ? mid : mid + 1 + ((j - from) - before_l - (mp < j ? 1u : 0u));
+                    }
+                }
+                L.tmp[dst] = L.a[a][j];
+            }
+            __syncthreads();
+            for (uint32_t j = t; j < m; j += kLocalThreads) L.a[a][j] = L.tmp[j];
+            __syncthreads();
+        }
+        // node of every position at the next depth
+        for (uint32_t j = t; j < m; j += kLocalThreads) {
+            const uint32_t i = L.node[j];
+            if (i == 0xFFFFu) continue;
+            const uint2 sg = L.seg[cur][i];
+            const uint32_t mid = sg.x + (sg.y - sg.x) / 2;
+            L.node[j] = (sg.y - sg.x <= 1 || j == mid) ? (uint16_t)0xFFFFu : (uint16_t)(j < mid ? 2 * i : 2 * i + 1);
+        }
+        // child planes (k_child_planes)
+        for (uint32_t c = t; c < 2 * nd; c += kLocalThreads) {
+            const uint2 s = L.seg[cur ^ 1][c];
+            L.plane[cur ^ 1][c] = s.y > s.x ? plane_of(L.a[0], L.a[1], L.a[2], s.x, s.y) : (int8_t)0;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_tree_photons(const float* __restrict__ in9, uint32_t n,
                                                      const uint32_t* __restrict__ node_idx, uint32_t cap,
                                                      float* __restrict__ out9) {
@@ -319,8 +471,20 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     if ((st = pm::fill_u32(ctx, node_of, 0u, n)) != PM_OK) return bail(st);
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, A[0], A[1], A[2], n, seg, plane);
 
+    // Levels [0, Lg) run on the whole arrays; from level Lg every segment
+    // holds at most kLocal records (a node of size s has children of at most
+    // s / 2) and each subtree is finished by one workgroup in LDS.
+    uint32_t Lg = 0;
+    for (uint32_t smax = n; smax > (uint32_t)kLocal; smax /= 2) ++Lg;
+    // a segment of at most 1024 records spans at most 11 levels (node arrays of kLocal)
+    if (L - Lg > 11) return bail(pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_build: internal level split"));
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
         uint32_t first = (1u << lvl) - 1, count = 1u << lvl;
+        if (lvl == Lg) {
+            hipLaunchKernelGGL(k_local_subtrees, dim3(count), dim3(kLocalThreads), 0, s, A[0], A[1], A[2], first,
+                               L - lvl, seg, plane, T->node_idx, T->node_plane);
+            break;
+        }
         hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(count, 256)), dim3(256), 0, s, A[0], A[1], A[2], first,
                            count, cap, seg, plane, T->node_idx, T->node_plane, med);
         if (lvl + 1 == L) break;  // every segment of the last level has size <= 1

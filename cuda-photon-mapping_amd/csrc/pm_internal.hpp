@@ -26,10 +26,9 @@ struct pm_ctx {
     uint32_t* scan_counter = nullptr;  // ordered tile ids, reset by the last tile
     uint32_t scan_epoch = 0;
     int cu_count = 256;
-    // Second arena for buffers that must survive nested library calls made
-    // by the frame renderer (which itself uses `scratch` through the gathers).
-    void* frame = nullptr;
-    size_t frame_bytes = 0;
+    // Frame state of pm_render_frame (hit slots and query block survive the
+    // gathers, which use `scratch`).
+    pm_frame* frame_state = nullptr;
 };
 
 struct pm_grid {
@@ -65,7 +64,6 @@ pm_status hip_fail(pm_ctx* ctx, hipError_t e, const char* what);
 // Scratch arena: returns at least `bytes` of device memory valid until the
 // next scratch request on this context. Grows (after a stream sync) as needed.
 pm_status scratch_reserve(pm_ctx* ctx, size_t bytes, void** out);
-pm_status frame_reserve(pm_ctx* ctx, size_t bytes, void** out);
 
 // Bump allocator over one reserved scratch block.
 struct Carve {

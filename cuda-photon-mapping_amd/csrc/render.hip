@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "pm_geom.hpp"
@@ -279,143 +280,244 @@ __global__ __launch_bounds__(256) void k_render_combine(const DevModel* __restri
 
 }  // namespace
 
+// Per-frame device state: the ray trees' hit slots, the compacted query
+// block and the two radiance planes, in one allocation that grows as needed.
+struct pm_frame {
+    void* mem = nullptr;
+    size_t cap = 0;
+    uint32_t W = 0, H = 0, npix = 0, nq = 0;
+    int hmax = 0, nm = 0;
+    pm_render_params R{};
+    bool rays_done = false, caustic = false;
+    DevModel* d_models = nullptr;
+    float *hp = nullptr, *hn = nullptr, *hc = nullptr;
+    int* hm = nullptr;
+    uint32_t *count = nullptr, *off = nullptr;
+    uint64_t* total = nullptr;
+    float *q3 = nullptr, *n3 = nullptr, *lg = nullptr, *lc = nullptr;
+    uint32_t* ccount = nullptr;
+    uint32_t* kidx = nullptr;
+    float* kdist = nullptr;
+};
+
+namespace {
+
+// Grow the frame's allocation to `need` bytes, keeping its first `keep`.
+pm_status frame_grow(pm_ctx* ctx, pm_frame* F, size_t need, size_t keep) {
+    if (need <= F->cap) return PM_OK;
+    void* fresh = nullptr;
+    const size_t want = need + need / 8;
+    hipError_t e = hipMalloc(&fresh, want);
+    if (e != hipSuccess) return pm::hip_fail(ctx, e, "hipMalloc(frame)");
+    if (F->mem) {
+        if (keep) PM_HIP(ctx, hipMemcpyAsync(fresh, F->mem, keep, hipMemcpyDeviceToDevice, ctx->stream));
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PM_HIP(ctx, hipFree(F->mem));
+    }
+    F->mem = fresh;
+    F->cap = want;
+    return PM_OK;
+}
+
+size_t rays_bytes(const pm_frame* F) {
+    const size_t slot = (size_t)F->hmax * F->npix;
+    return pm::carve_size({(size_t)F->nm * sizeof(DevModel), (size_t)F->nm * 32, slot * 12, slot * 12, slot * 12,
+                           slot * 4, (size_t)F->npix * 4, (size_t)F->npix * 4, 64});
+}
+
+void carve_rays(pm_frame* F, pm::Carve& cv) {
+    const size_t slot = (size_t)F->hmax * F->npix;
+    F->d_models = cv.take<DevModel>(F->nm);
+    cv.take<char>((size_t)F->nm * 32);
+    F->hp = cv.take<float>(slot * 3);
+    F->hn = cv.take<float>(slot * 3);
+    F->hc = cv.take<float>(slot * 3);
+    F->hm = cv.take<int>(slot);
+    F->count = cv.take<uint32_t>(F->npix);
+    F->off = cv.take<uint32_t>(F->npix);
+    F->total = cv.take<uint64_t>(1);
+}
+
+bool quirk_mode(const pm_render_params& R) { return (R.flags & PM_RENDER_REFERENCE_CAUSTIC) != 0; }
+
+}  // namespace
+
+extern "C" pm_status pm_frame_create(pm_ctx* ctx, pm_frame** out) {
+    if (!ctx || !out) return PM_ERR_INVALID;
+    *out = new (std::nothrow) pm_frame();
+    return *out ? PM_OK : pm::fail(ctx, PM_ERR_NOMEM, "pm_frame_create: host allocation");
+}
+
+extern "C" pm_status pm_frame_destroy(pm_frame* F) {
+    if (!F) return PM_OK;
+    if (F->mem) (void)hipFree(F->mem);
+    delete F;
+    return PM_OK;
+}
+
+// Phase 1: every pixel's ray tree (hits in the reference's visit order) and
+// the compacted query block; returns the number of hits (queries per map).
+extern "C" pm_status pm_frame_rays(pm_ctx* ctx, pm_frame* F, const pm_scene* scene, const pm_camera* camera,
+                                   const pm_render_params* params, uint32_t* n_queries) {
+    if (!ctx || !F || !scene || !camera || !params) return PM_ERR_INVALID;
+    F->rays_done = false;
+    if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_frame_rays: empty scene");
+    const pm_render_params& R = *params;
+    if (R.k_global < 1 || R.k_global > 2048 || R.k_caustic < 1 || R.k_caustic > 2048)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_frame_rays: k outside [1, 2048]");
+    if (R.max_depth < 1 || R.max_depth > kMaxDepth)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_frame_rays: max_depth outside [1, %d]", kMaxDepth);
+    if (camera->width <= 0 || camera->height <= 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_frame_rays: empty canvas");
+    const uint64_t npix64 = (uint64_t)camera->width * (uint64_t)camera->height;
+    if (npix64 > (1ull << 26)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_frame_rays: canvas too large");
+    F->W = (uint32_t)camera->width;
+    F->H = (uint32_t)camera->height;
+    F->npix = (uint32_t)npix64;
+    F->hmax = (1 << R.max_depth) - 1;
+    F->nm = scene->n_models;
+    F->R = R;
+    pm_status st = frame_grow(ctx, F, rays_bytes(F), 0);
+    if (st != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(F->mem)};
+    carve_rays(F, cv);
+    pm::Carve ct{static_cast<char*>(F->mem)};  // the model-AABB scratch right after the model table
+    ct.take<DevModel>(F->nm);
+    void* tmp = ct.take<char>((size_t)F->nm * 32);
+    std::vector<DevModel> host_models;
+    if ((st = upload_models(ctx, scene, F->d_models, tmp, host_models)) != PM_OK) return st;
+
+    RaysParams P{};
+    P.tris = scene->d_tris9;
+    P.nrms = scene->d_normals9;
+    P.models = F->d_models;
+    P.n_models = F->nm;
+    P.bvh = bvh_nodes(scene);
+    P.def_ior = scene->default_refr_index;
+    std::memcpy(&P.cam, camera, sizeof(Cam));
+    P.max_depth = R.max_depth;
+    P.hmax = F->hmax;
+    P.hp = F->hp;
+    P.hn = F->hn;
+    P.hc = F->hc;
+    P.hm = F->hm;
+    P.count = F->count;
+    P.npix = F->npix;
+    const uint32_t tiles = ((F->W + 7) / 8) * ((F->H + 7) / 8);
+    hipLaunchKernelGGL(k_render_rays, dim3(pm::div_up(tiles, 4)), dim3(256), 0, ctx->stream, P);
+    PM_CHECK_LAUNCH(ctx, "k_render_rays");
+    if ((st = pm::scan_u32(ctx, F->count, F->off, F->npix, false, F->total)) != PM_OK) return st;
+    uint64_t nq64 = 0;
+    PM_HIP(ctx, hipMemcpyAsync(&nq64, F->total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t nq = (uint32_t)nq64;
+    F->nq = nq;
+
+    // query block + radiance planes (+ the reference-caustic index/distance rows)
+    const size_t a = rays_bytes(F);
+    const size_t kc = quirk_mode(R) ? (size_t)R.k_caustic : 0;
+    const size_t qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
+                                          (size_t)nq * 4, (size_t)nq * 4 * kc, (size_t)nq * 4 * kc});
+    if ((st = frame_grow(ctx, F, a + qbytes, a)) != PM_OK) return st;
+    pm::Carve c2{static_cast<char*>(F->mem)};
+    carve_rays(F, c2);
+    F->q3 = c2.take<float>((size_t)nq * 3);
+    F->n3 = c2.take<float>((size_t)nq * 3);
+    F->lg = c2.take<float>((size_t)nq * 3);
+    F->lc = c2.take<float>((size_t)nq * 3);
+    F->ccount = c2.take<uint32_t>(nq);
+    F->kidx = c2.take<uint32_t>((size_t)nq * kc);
+    F->kdist = c2.take<float>((size_t)nq * kc);
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_render_compact, dim3(pm::div_up(F->npix, 256)), dim3(256), 0, ctx->stream, F->hp, F->hn,
+                           F->count, F->off, F->npix, F->hmax, nq, F->q3, F->n3);
+        PM_CHECK_LAUNCH(ctx, "k_render_compact");
+    }
+    F->caustic = false;
+    F->rays_done = true;
+    if (n_queries) *n_queries = nq;
+    return PM_OK;
+}
+
+// Phase 2 description: the frame's global and caustic gathers as batch jobs
+// (queries and normals of the frame, radiance into the frame's planes).
+extern "C" pm_status pm_frame_gather_jobs(pm_frame* F, const pm_grid* global_grid, const pm_grid* caustic_grid,
+                                          pm_gather_job* jobs, uint32_t* n_jobs) {
+    if (!F || !global_grid || !jobs || !n_jobs) return PM_ERR_INVALID;
+    if (!F->rays_done || quirk_mode(F->R)) return PM_ERR_INVALID;
+    pm_gather_job g{};
+    g.grid = global_grid;
+    g.d_queries3 = F->q3;
+    g.nq = F->nq;
+    g.radius = F->R.r_global;
+    g.k = F->R.k_global;
+    g.d_normals3 = F->n3;
+    g.d_rgb3 = F->lg;
+    jobs[0] = g;
+    *n_jobs = 1;
+    F->caustic = caustic_grid != nullptr;
+    if (caustic_grid) {
+        pm_gather_job c = g;
+        c.grid = caustic_grid;
+        c.radius = F->R.r_caustic;
+        c.k = F->R.k_caustic;
+        c.d_rgb3 = F->lc;
+        c.d_count = F->ccount;
+        jobs[1] = c;
+        *n_jobs = 2;
+    }
+    return PM_OK;
+}
+
+// Phase 3: fold every pixel's hits with the estimates, HDR, uchar3.
+extern "C" pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* F, float* d_rgb3, uint8_t* d_rgb8,
+                                     pm_frame_stats* stats) {
+    if (!ctx || !F) return PM_ERR_INVALID;
+    if (!F->rays_done) return pm::fail(ctx, PM_ERR_INVALID, "pm_frame_finish: no rays traced");
+    hipLaunchKernelGGL(k_render_combine, dim3(pm::div_up(F->npix, 256)), dim3(256), 0, ctx->stream, F->d_models,
+                       F->hc, F->hm, F->count, F->off, F->npix, F->hmax, F->nq, F->lg, F->lc, F->ccount,
+                       F->R.k_caustic, F->caustic ? 1 : 0, d_rgb3, d_rgb8);
+    PM_CHECK_LAUNCH(ctx, "k_render_combine");
+    if (stats) {
+        stats->hits = F->nq;
+        stats->pixels = F->npix;
+    }
+    return PM_OK;
+}
+
 extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* global_grid,
                                      const pm_grid* caustic_grid, const pm_camera* camera,
                                      const pm_render_params* params, float* d_rgb3, uint8_t* d_rgb8,
                                      pm_frame_stats* stats) {
     if (!ctx || !scene || !camera || !params) return PM_ERR_INVALID;
     if (!global_grid) return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: global grid required");
-    if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: empty scene");
-    const pm_render_params& R = *params;
-    if (R.k_global < 1 || R.k_global > 2048 || R.k_caustic < 1 || R.k_caustic > 2048)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: k outside [1, 2048]");
-    if (R.max_depth < 1 || R.max_depth > kMaxDepth)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: max_depth outside [1, %d]", kMaxDepth);
-    if (camera->width <= 0 || camera->height <= 0)
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_render_frame: empty canvas");
-    const uint32_t W = (uint32_t)camera->width, H = (uint32_t)camera->height;
-    const uint64_t npix64 = (uint64_t)W * H;
-    if (npix64 > (1ull << 26)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_render_frame: canvas too large");
-    const uint32_t npix = (uint32_t)npix64;
-    const int hmax = (1 << R.max_depth) - 1;
-    const int nm = scene->n_models;
-
-    size_t slot = (size_t)hmax * npix;
-    size_t bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), (size_t)nm * 32, slot * 12, slot * 12, slot * 12,
-                                   slot * 4, (size_t)npix * 4, (size_t)npix * 4, 64});
-    void* work = nullptr;
-    pm_status st = pm::frame_reserve(ctx, bytes, &work);
+    if (!ctx->frame_state && pm_frame_create(ctx, &ctx->frame_state) != PM_OK) return PM_ERR_NOMEM;
+    pm_frame* F = ctx->frame_state;
+    pm_status st = pm_frame_rays(ctx, F, scene, camera, params, nullptr);
     if (st != PM_OK) return st;
-    pm::Carve cv{static_cast<char*>(work)};
-    DevModel* d_models = cv.take<DevModel>(nm);
-    void* d_tmp = cv.take<char>((size_t)nm * 32);
-    float* hp = cv.take<float>(slot * 3);
-    float* hn = cv.take<float>(slot * 3);
-    float* hc = cv.take<float>(slot * 3);
-    int* hm = cv.take<int>(slot);
-    uint32_t* count = cv.take<uint32_t>(npix);
-    uint32_t* off = cv.take<uint32_t>(npix);
-    uint64_t* total = cv.take<uint64_t>(1);
-
-    std::vector<DevModel> host_models;
-    if ((st = upload_models(ctx, scene, d_models, d_tmp, host_models)) != PM_OK) return st;
-
-    RaysParams P{};
-    P.tris = scene->d_tris9;
-    P.nrms = scene->d_normals9;
-    P.models = d_models;
-    P.n_models = nm;
-    P.bvh = bvh_nodes(scene);
-    P.def_ior = scene->default_refr_index;
-    std::memcpy(&P.cam, camera, sizeof(Cam));
-    P.max_depth = R.max_depth;
-    P.hmax = hmax;
-    P.hp = hp;
-    P.hn = hn;
-    P.hc = hc;
-    P.hm = hm;
-    P.count = count;
-    P.npix = npix;
-    const uint32_t tiles = ((W + 7) / 8) * ((H + 7) / 8);
-    hipLaunchKernelGGL(k_render_rays, dim3(pm::div_up(tiles, 4)), dim3(256), 0, ctx->stream, P);
-    PM_CHECK_LAUNCH(ctx, "k_render_rays");
-    if ((st = pm::scan_u32(ctx, count, off, npix, false, total)) != PM_OK) return st;
-    uint64_t nq64 = 0;
-    PM_HIP(ctx, hipMemcpyAsync(&nq64, total, 8, hipMemcpyDeviceToHost, ctx->stream));
-    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint32_t nq = (uint32_t)nq64;
-
-    // query block + gather outputs live in a second carve of the frame arena
-    const size_t kc = (size_t)std::max(1, R.k_caustic);
-    size_t qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
-                                    (size_t)nq * 4, (size_t)nq * 4 * kc, (size_t)nq * 4 * kc});
-    // (index/distance buffers are only needed for the reference-caustic mode)
-    const bool quirk = (R.flags & PM_RENDER_REFERENCE_CAUSTIC) != 0 && caustic_grid;
-    if (!quirk) qbytes = pm::carve_size({(size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12, (size_t)nq * 12,
-                                         (size_t)nq * 4});
-    // grow the arena if needed, keeping the hit slots: recompute as one block
-    size_t all = bytes + qbytes;
-    if (all > ctx->frame_bytes) {
-        // the hit slots must survive: allocate a fresh arena and move them
-        void* old = ctx->frame;
-        size_t old_bytes = ctx->frame_bytes;
-        void* fresh = nullptr;
-        PM_HIP(ctx, hipMalloc(&fresh, all + all / 8));
-        PM_HIP(ctx, hipMemcpyAsync(fresh, old, old_bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        PM_HIP(ctx, hipFree(old));
-        ctx->frame = fresh;
-        ctx->frame_bytes = all + all / 8;
-        // re-carve the same layout
-        pm::Carve c2{static_cast<char*>(ctx->frame)};
-        d_models = c2.take<DevModel>(nm);
-        c2.take<char>((size_t)nm * 32);
-        hp = c2.take<float>(slot * 3);
-        hn = c2.take<float>(slot * 3);
-        hc = c2.take<float>(slot * 3);
-        hm = c2.take<int>(slot);
-        count = c2.take<uint32_t>(npix);
-        off = c2.take<uint32_t>(npix);
-        c2.take<uint64_t>(1);
-        cv = c2;
-    }
-    float* q3 = cv.take<float>((size_t)nq * 3);
-    float* n3 = cv.take<float>((size_t)nq * 3);
-    float* lg = cv.take<float>((size_t)nq * 3);
-    float* lc = cv.take<float>((size_t)nq * 3);
-    uint32_t* ccount = cv.take<uint32_t>(nq);
+    const pm_render_params& R = *params;
+    const uint32_t gflags = R.gather_flags | PM_KNN_SORT_QUERIES;
+    const uint32_t nq = F->nq;
+    F->caustic = caustic_grid != nullptr;
     if (nq > 0) {
-        hipLaunchKernelGGL(k_render_compact, dim3(pm::div_up(npix, 256)), dim3(256), 0, ctx->stream, hp, hn, count,
-                           off, npix, hmax, nq, q3, n3);
-        PM_CHECK_LAUNCH(ctx, "k_render_compact");
-        const uint32_t gflags = R.gather_flags | PM_KNN_SORT_QUERIES;
-        if ((st = pm_grid_gather_radiance(ctx, global_grid, q3, n3, nq, R.r_global, R.k_global, gflags, lg, nullptr)) !=
-            PM_OK)
+        if ((st = pm_grid_gather_radiance(ctx, global_grid, F->q3, F->n3, nq, R.r_global, R.k_global, gflags, F->lg,
+                                          nullptr)) != PM_OK)
             return st;
         if (caustic_grid) {
-            if (!quirk) {
-                if ((st = pm_grid_gather_radiance(ctx, caustic_grid, q3, n3, nq, R.r_caustic, R.k_caustic, gflags, lc,
-                                                  ccount)) != PM_OK)
+            if (!quirk_mode(R)) {
+                if ((st = pm_grid_gather_radiance(ctx, caustic_grid, F->q3, F->n3, nq, R.r_caustic, R.k_caustic,
+                                                  gflags, F->lc, F->ccount)) != PM_OK)
                     return st;
             } else {
                 // Raytracer.cuh:203-204: caustic indices, global map arrays
-                uint32_t* kidx = cv.take<uint32_t>((size_t)nq * R.k_caustic);
-                float* kdist = cv.take<float>((size_t)nq * R.k_caustic);
-                if ((st = pm_grid_knn(ctx, caustic_grid, q3, nq, R.r_caustic, R.k_caustic, gflags, kidx, kdist,
-                                      ccount)) != PM_OK)
+                if ((st = pm_grid_knn(ctx, caustic_grid, F->q3, nq, R.r_caustic, R.k_caustic, gflags, F->kidx,
+                                      F->kdist, F->ccount)) != PM_OK)
                     return st;
-                if ((st = pm_radiance(ctx, global_grid, kidx, kdist, ccount, n3, nq, R.k_caustic, lc)) != PM_OK)
+                if ((st = pm_radiance(ctx, global_grid, F->kidx, F->kdist, F->ccount, F->n3, nq, R.k_caustic, F->lc)) !=
+                    PM_OK)
                     return st;
             }
         }
     }
-    hipLaunchKernelGGL(k_render_combine, dim3(pm::div_up(npix, 256)), dim3(256), 0, ctx->stream, d_models, hc, hm,
-                       count, off, npix, hmax, nq, lg, lc, ccount, R.k_caustic, caustic_grid ? 1 : 0, d_rgb3, d_rgb8);
-    PM_CHECK_LAUNCH(ctx, "k_render_combine");
-    if (stats) {
-        stats->hits = nq;
-        stats->pixels = npix;
-    }
-    return PM_OK;
+    return pm_frame_finish(ctx, F, d_rgb3, d_rgb8, stats);
 }

@@ -85,6 +85,11 @@ _SIGS = {
     "pm_kdtree_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _P, _P, _P]),
     "pm_accel_build": (_I32, [_P, _P, C.POINTER(_P)]),
     "pm_gather_batch": (_I32, [_P, _P, _U32, _U32]),
+    "pm_frame_create": (_I32, [_P, C.POINTER(_P)]),
+    "pm_frame_destroy": (_I32, [_P]),
+    "pm_frame_rays": (_I32, [_P, _P, _P, _P, _P, C.POINTER(_U32)]),
+    "pm_frame_gather_jobs": (_I32, [_P, _P, _P, _P, C.POINTER(_U32)]),
+    "pm_frame_finish": (_I32, [_P, _P, _P, _P, _P]),
     "pm_accel_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -374,6 +379,47 @@ class Context:
         self.check(self.lib.pm_render_frame(self.h, C.byref(dscene.rec), grid_g.h, grid_c.h if grid_c else None,
                                             cam.ctypes.data, C.byref(p), _ptr(rgb), _ptr(rgb8), C.byref(st)))
         return rgb, rgb8, int(st.hits)
+
+    def render_frames_batched(self, frames, flags: int = 0, want_rgb8: bool = True):
+        """Render many frames with their gathers in one pm_gather_batch:
+        frames = [(dscene, grid_g, grid_c or None, camera, max_depth, k_global,
+        r_global, k_caustic, r_caustic)]. Returns [(rgb3, rgb8, hits)]."""
+        import torch
+
+        handles, outs = [], []
+        try:
+            recs = (GatherJob * (2 * max(len(frames), 1)))()
+            n_jobs = 0
+            for (ds, gg, gc, camera, md, kg, rg, kc, rc) in frames:
+                f = _P()
+                self.check(self.lib.pm_frame_create(self.h, C.byref(f)))
+                handles.append(f)
+                cam = np.ascontiguousarray(camera)
+                p = RenderParams(int(md), int(kg), float(rg), int(kc), float(rc), 0, 0)
+                nq = _U32()
+                self.check(self.lib.pm_frame_rays(self.h, f, C.byref(ds.rec), cam.ctypes.data, C.byref(p),
+                                                  C.byref(nq)))
+                jb = (GatherJob * 2)()
+                nj = _U32()
+                self.check(self.lib.pm_frame_gather_jobs(f, gg.h, gc.h if gc else None, C.cast(jb, C.c_void_p),
+                                                         C.byref(nj)))
+                for i in range(nj.value):
+                    recs[n_jobs] = jb[i]
+                    n_jobs += 1
+            self.check(self.lib.pm_gather_batch(self.h, C.cast(recs, C.c_void_p), n_jobs, int(flags)))
+            for f, (ds, gg, gc, camera, *_rest) in zip(handles, frames):
+                cam = np.ascontiguousarray(camera)
+                W, H = int(cam["width"].item()), int(cam["height"].item())
+                rgb = torch.empty((3, W * H), dtype=torch.float32, device=self.device)
+                rgb8 = torch.empty((W * H * 3,), dtype=torch.uint8, device=self.device) if want_rgb8 else None
+                st = FrameStats()
+                self.check(self.lib.pm_frame_finish(self.h, f, _ptr(rgb), _ptr(rgb8), C.byref(st)))
+                outs.append((rgb, rgb8, int(st.hits)))
+            torch.cuda.synchronize(self.device)
+        finally:
+            for f in handles:
+                self.lib.pm_frame_destroy(f)
+        return outs
 
     def box_stats(self, grid: "Grid", queries3, radius: float):
         q = _f32_dev(queries3, self.device)

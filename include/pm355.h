@@ -56,6 +56,7 @@ typedef struct pm_ctx pm_ctx;
 typedef struct pm_grid pm_grid;
 typedef struct pm_kdtree pm_kdtree;
 typedef struct pm_accel pm_accel;
+typedef struct pm_frame pm_frame;
 
 /* ---- context (replaces the reference's global device state: GlobalParams
  * __constant__ mirrors GlobalParams.cu:1-58, the heap pointers of
@@ -294,6 +295,25 @@ typedef struct {
 pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* global_grid,
                           const pm_grid* caustic_grid, const pm_camera* camera, const pm_render_params* params,
                           float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
+
+/* ---- phased frame: pm_render_frame split so that the gathers of many
+ * frames can run as one pm_gather_batch (the reference renders one frame per
+ * launch, Drawer.cu:87-101):
+ *   pm_frame_rays        ray trees of every pixel + compacted query block;
+ *                        *n_queries = hits (synchronizes to read it back);
+ *   pm_frame_gather_jobs the frame's global (and caustic, if given) gather as
+ *                        batch jobs writing into the frame (not available
+ *                        with PM_RENDER_REFERENCE_CAUSTIC);
+ *   (run the jobs, e.g. with pm_gather_batch, on the same stream)
+ *   pm_frame_finish      per-pixel fold, HDR and uchar3, as pm_render_frame.
+ * A pm_frame owns its device buffers and may be reused for the next frame. */
+pm_status pm_frame_create(pm_ctx* ctx, pm_frame** out);
+pm_status pm_frame_destroy(pm_frame* frame);
+pm_status pm_frame_rays(pm_ctx* ctx, pm_frame* frame, const pm_scene* scene, const pm_camera* camera,
+                        const pm_render_params* params, uint32_t* n_queries);
+pm_status pm_frame_gather_jobs(pm_frame* frame, const pm_grid* global_grid, const pm_grid* caustic_grid,
+                               pm_gather_job* jobs /* [2] */, uint32_t* n_jobs);
+pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
 
 /* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
  * only live effect is the model's bounding box (min/max over its vertices).

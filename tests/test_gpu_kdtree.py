@@ -34,7 +34,7 @@ def test_kdtree_reference_fixture(ctx):
     assert got == [(30, 5, 0), (-10, 7, 20), (80, 9, 40), (-50, 4, 30), (0, 3, 21), (40, 1, 10)]
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 9, 100, 1023, 1024, 4097])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 9, 100, 1023, 1024, 1025, 2047, 2048, 2049, 4097, 65537])
 def test_kdtree_ties(ctx, n):
     pos = np.random.default_rng(n).integers(0, 5, (3, n)).astype(np.float32)
     pos[0, : n // 3] = -0.0  # signed zeros tie with +0

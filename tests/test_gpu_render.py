@@ -62,3 +62,28 @@ def test_frame_without_caustic_map(ctx, setup):
     orgb, _, ohits = O.render(scene, og, None, cam, 4, 50, 0.05, 50, 0.05)
     assert hits == ohits
     assert np.abs(rgb.cpu().numpy() - orgb).max() <= ABS_MAX
+
+
+def test_batched_frames_equal_single_frames(ctx):
+    """Phased frames whose gathers run as one batch give exactly the frames
+    of pm_render_frame (different scenes, maps, cameras and parameters)."""
+    from pm355.scenes import camera, random_scene
+
+    frames = []
+    for i, sc in enumerate([cornell_box(), random_scene(4), random_scene(9)]):
+        ds = ctx.upload_scene(sc, accel=True)
+        g, c, _ = ctx.trace_photons(ds, 20 + i, 60_000, 6_000, 8, max_paths=3_000_000)
+        aabb = sc.aabb()
+        r = (0.05, 0.03, 0.04)[i]
+        gg = ctx.grid_build(g, aabb, cell_size=r)
+        gc = ctx.grid_build(c, aabb, cell_size=r) if i != 2 else None
+        cam = cornell_camera(160 + 32 * i, 120) if i != 1 else camera((0.5, 0.6, 1.6), (0.1, -0.2, -1.0),
+                                                                       (0, 1, 0), 50.0, 96, 80)
+        frames.append((ds, gg, gc, cam, 3 + i, 40 + 10 * i, r, 30, r))
+    got = ctx.render_frames_batched(frames, flags=1)
+    for (ds, gg, gc, cam, md, kg, rg, kc, rc), (rgb, rgb8, hits) in zip(frames, got):
+        rgb1, rgb81, hits1 = ctx.render_frame(ds, gg, gc, cam, md, kg, rg, kc, rc)
+        ctx.sync()
+        assert hits == hits1
+        assert np.array_equal(rgb.cpu().numpy().view(np.uint32), rgb1.cpu().numpy().view(np.uint32))
+        assert np.array_equal(rgb8.cpu().numpy(), rgb81.cpu().numpy())
