@@ -179,6 +179,59 @@ def batch_bench(ctx, n_jobs, reps=2):
     }
 
 
+def serve_bench(ctx, rates=(10, 50, 200), n_jobs=16, seed=11):
+    """The served form: Poisson arrivals of render jobs (scenes of the batched
+    distribution, resident with their BVHs; 50k-8M photons, 10% caustic;
+    frames of 128x128 up to 1920x1080; 1-30 frames each; k = 64,
+    r in [0.01, 0.05], cell = r) into the continuous-batching server.
+    Reports per arrival rate the p50 / p99 frame latency (eligible ->
+    completed) and gather queries/s over the run."""
+    import numpy as np
+    import torch
+
+    from pm355.scenes import camera, random_scene
+
+    rng = np.random.default_rng(seed)
+    scenes = [random_scene(100 + i) for i in range(6)]
+    dss = [ctx.upload_scene(sc, accel=True) for sc in scenes]
+    res_set = [(128, 128), (256, 256), (640, 480), (1280, 720), (1920, 1080)]
+    out = {}
+    for rate in rates:
+        jobs = []
+        t = 0.0
+        for j in range(n_jobs):
+            t += rng.exponential(1.0 / rate)
+            s = int(rng.integers(0, len(scenes)))
+            w, h = res_set[int(rng.integers(0, len(res_set)))]
+            ng = int(np.exp(rng.uniform(np.log(5e4), np.log(8e6))))
+            r = float(rng.uniform(0.01, 0.05))
+            jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 10, frames=int(rng.integers(1, 31)), r=r,
+                             seed=int(rng.integers(1, 1 << 62))))
+        srv = ctx.server()
+        t0 = time.perf_counter()
+        nxt = 0
+        while nxt < len(jobs) or srv.info()["finished"] < len(jobs):
+            now = time.perf_counter() - t0
+            while nxt < len(jobs) and jobs[nxt]["at"] <= now:
+                jb = jobs[nxt]
+                sc = scenes[jb["scene"]]
+                cam = camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, jb["w"], jb["h"])
+                srv.submit(dss[jb["scene"]], sc.aabb(), jb["r"], jb["ng"], jb["nc"], cam, jb["frames"], jb["seed"],
+                           k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=8 * jb["ng"])
+                nxt += 1
+            if srv.step() == 0 and nxt < len(jobs):
+                time.sleep(max(0.0, min(0.01, jobs[nxt]["at"] - (time.perf_counter() - t0))))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = srv.latencies() * 1e3
+        info = srv.info()
+        out[f"{rate}_jobs_per_s"] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                                     "frames": int(info["frames"]), "queries_per_s": info["queries"] / wall,
+                                     "wall_s": wall}
+        srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +246,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=16, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -347,6 +401,12 @@ def main():
     if args.batch_jobs > 0:
         log(f"batched form: {args.batch_jobs}-job gather mix")
         res["batched"] = batch_bench(ctx, args.batch_jobs)
+    if args.serve_jobs > 0:
+        log(f"served form: {args.serve_jobs} jobs per arrival rate")
+        res["served"] = {"workload": "Poisson arrivals of render jobs: 6 resident scenes of the batched distribution "
+                                     "(unit box + 1-8 spheres of 200-20k triangles, BVHs), 50k-8M global photons + 10% "
+                                     "caustic re-traced per frame, 128^2..1920x1080, 1-30 frames, k=64, r U[0.01,0.05]",
+                         **serve_bench(ctx, n_jobs=args.serve_jobs)}
     if not args.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)

@@ -90,6 +90,12 @@ _SIGS = {
     "pm_frame_rays": (_I32, [_P, _P, _P, _P, _P, C.POINTER(_U32)]),
     "pm_frame_gather_jobs": (_I32, [_P, _P, _P, _P, C.POINTER(_U32)]),
     "pm_frame_finish": (_I32, [_P, _P, _P, _P, _P]),
+    "pm_server_create": (_I32, [_P, C.c_size_t, C.POINTER(_P)]),
+    "pm_server_destroy": (_I32, [_P]),
+    "pm_server_submit": (_I32, [_P, _P, C.POINTER(C.c_uint64)]),
+    "pm_server_step": (_I32, [_P, C.POINTER(_U32)]),
+    "pm_server_stats": (_I32, [_P, _P]),
+    "pm_server_latencies": (_I32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pm_accel_destroy": (_I32, [_P]),
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -113,6 +119,24 @@ class GatherJob(C.Structure):
     _fields_ = [("grid", C.c_void_p), ("d_queries3", C.c_void_p), ("nq", C.c_uint32), ("radius", C.c_float),
                 ("k", C.c_int32), ("d_idx", C.c_void_p), ("d_dist", C.c_void_p), ("d_count", C.c_void_p),
                 ("d_normals3", C.c_void_p), ("d_rgb3", C.c_void_p)]
+
+
+class CameraRec(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("viewport_width", C.c_float), ("viewport_height", C.c_float),
+                ("focal_length", C.c_float), ("width", C.c_int32), ("height", C.c_int32), ("pad", C.c_int32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("aabb", C.c_float * 6), ("cell_size", C.c_float), ("n_global", C.c_uint32),
+                ("n_caustic", C.c_uint32), ("max_bounces", C.c_int32), ("max_paths", C.c_uint64),
+                ("seed", C.c_uint64), ("camera", CameraRec), ("params", C.c_int32 * 7), ("frames", C.c_uint32),
+                ("d_rgb8", C.c_void_p)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", C.c_uint32), ("active", C.c_uint32), ("finished", C.c_uint32), ("frames", C.c_uint64),
+                ("queries", C.c_uint64), ("bytes_in_use", C.c_uint64)]
 
 
 class TraceStats(C.Structure):
@@ -421,6 +445,9 @@ class Context:
                 self.lib.pm_frame_destroy(f)
         return outs
 
+    def server(self, budget_bytes: int = 0) -> "Server":
+        return Server(self, budget_bytes)
+
     def box_stats(self, grid: "Grid", queries3, radius: float):
         q = _f32_dev(queries3, self.device)
         c = C.c_uint64()
@@ -538,6 +565,70 @@ class KdTree:
     def close(self):
         if self.h:
             self.ctx.lib.pm_kdtree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching render server (pm_server_*): submit jobs, step."""
+
+    def __init__(self, ctx: Context, budget_bytes: int = 0):
+        self.ctx = ctx
+        self.h = _P()
+        ctx.check(ctx.lib.pm_server_create(ctx.h, int(budget_bytes), C.byref(self.h)))
+        self._keep = []
+
+    def submit(self, dscene: "DeviceScene", aabb, cell_size, n_global, n_caustic, camera, frames, seed,
+               max_depth=4, k_global=64, r_global=0.05, k_caustic=64, r_caustic=0.05, max_bounces=10,
+               max_paths=1 << 40):
+        """Queue a job; returns (job_id, rgb8 device tensor [frames, H*W*3])."""
+        import torch
+
+        cam = np.ascontiguousarray(camera)
+        W, H = int(cam["width"].item()), int(cam["height"].item())
+        out = torch.zeros((frames, W * H * 3), dtype=torch.uint8, device=self.ctx.device)
+        d = JobDesc()
+        d.scene = C.addressof(dscene.rec)
+        for i in range(6):
+            d.aabb[i] = float(aabb[i])
+        d.cell_size = float(cell_size)
+        d.n_global, d.n_caustic, d.max_bounces = int(n_global), int(n_caustic), int(max_bounces)
+        d.max_paths, d.seed = int(max_paths), int(seed)
+        C.memmove(C.byref(d.camera), cam.ctypes.data, C.sizeof(CameraRec))
+        p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic), 0, 0)
+        C.memmove(C.byref(d.params), C.byref(p), C.sizeof(RenderParams))
+        d.frames = int(frames)
+        d.d_rgb8 = out.data_ptr()
+        jid = C.c_uint64()
+        self.ctx.check(self.ctx.lib.pm_server_submit(self.h, C.byref(d), C.byref(jid)))
+        self._keep.append((dscene, out))
+        return int(jid.value), out
+
+    def step(self) -> int:
+        n = _U32()
+        self.ctx.check(self.ctx.lib.pm_server_step(self.h, C.byref(n)))
+        return int(n.value)
+
+    def info(self) -> dict:
+        s = ServerInfo()
+        self.ctx.check(self.ctx.lib.pm_server_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerInfo._fields_}
+
+    def latencies(self) -> np.ndarray:
+        n = C.c_uint64()
+        self.ctx.check(self.ctx.lib.pm_server_latencies(self.h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.float64)
+        self.ctx.check(self.ctx.lib.pm_server_latencies(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.pm_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -57,6 +57,7 @@ typedef struct pm_grid pm_grid;
 typedef struct pm_kdtree pm_kdtree;
 typedef struct pm_accel pm_accel;
 typedef struct pm_frame pm_frame;
+typedef struct pm_server pm_server;
 
 /* ---- context (replaces the reference's global device state: GlobalParams
  * __constant__ mirrors GlobalParams.cu:1-58, the heap pointers of
@@ -314,6 +315,45 @@ pm_status pm_frame_rays(pm_ctx* ctx, pm_frame* frame, const pm_scene* scene, con
 pm_status pm_frame_gather_jobs(pm_frame* frame, const pm_grid* global_grid, const pm_grid* caustic_grid,
                                pm_gather_job* jobs /* [2] */, uint32_t* n_jobs);
 pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
+
+/* ---- serving: continuous batching of render jobs (the reference renders one
+ * scene, one frame at a time, Drawer.cu:18-101). A job is a resident scene
+ * (with its pm_accel, if any), photon budgets, grid cell size, camera,
+ * render parameters and a number of frames; frame f re-traces its photons
+ * with seed + f * 0x9E3779B97F4A7C15 and writes uchar3 to
+ * d_rgb8 + f * W * H * 3. Each pm_server_step admits queued jobs whose
+ * device footprint fits the budget, then renders the next frame of every
+ * active job with all of their gathers in one pm_gather_batch. A job's frames
+ * are identical alone and in any mix. k <= 256; no reference-caustic mode. */
+typedef struct {
+    const pm_scene* scene;
+    float aabb[6];            /* grid bounds */
+    float cell_size;          /* > 0, or <= 0 for the reference sizing rule */
+    uint32_t n_global, n_caustic;
+    int32_t max_bounces;
+    uint64_t max_paths;
+    uint64_t seed;
+    pm_camera camera;
+    pm_render_params params;
+    uint32_t frames;
+    uint8_t* d_rgb8;          /* [frames][H*W*3], caller-owned */
+} pm_job_desc;
+
+typedef struct {
+    uint32_t queued, active, finished;
+    uint64_t frames;          /* frames completed */
+    uint64_t queries;         /* gather queries run (both maps) */
+    uint64_t bytes_in_use;    /* admitted jobs' footprint */
+} pm_server_info;
+
+/* budget_bytes = 0: 80% of the device memory free at creation */
+pm_status pm_server_create(pm_ctx* ctx, size_t budget_bytes, pm_server** out);
+pm_status pm_server_destroy(pm_server* server);
+pm_status pm_server_submit(pm_server* server, const pm_job_desc* job, uint64_t* job_id);
+pm_status pm_server_step(pm_server* server, uint32_t* frames_done);
+pm_status pm_server_stats(const pm_server* server, pm_server_info* out);
+/* per-frame latency in seconds (eligible -> completed), oldest first */
+pm_status pm_server_latencies(const pm_server* server, double* out, uint64_t cap, uint64_t* n);
 
 /* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
  * only live effect is the model's bounding box (min/max over its vertices).

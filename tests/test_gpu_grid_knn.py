@@ -174,6 +174,11 @@ def test_large_k_parity(ctx, k, r):
     ctx.sync()
     assert np.array_equal(_np(cnt, np.uint32), oc)
     got = _np(rgb).astype(np.float64)
+    # query 0 sits on 1500 duplicates: the k-th distance is 0 and the
+    # reference's filter divides 0 by 0 (Raytracer.cuh:127-133), NaN on both sides
+    nan = np.isnan(o64)
+    assert np.array_equal(np.isnan(got), nan) and nan[:, 0].all()
+    got, o64 = got[:, 1:], o64[:, 1:]
     scale = np.maximum(np.abs(o64), 1e-30)
     rel = np.abs(got - o64) / scale
     rel[o64 == 0] = np.abs(got[o64 == 0])

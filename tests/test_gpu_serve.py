@@ -1,0 +1,76 @@
+"""GPU: the continuous-batching server renders every job's frames exactly as
+the job alone would, and as the one-frame-at-a-time calls do (photon pass
+re-seeded per frame, grids rebuilt, pm_render_frame), whatever else is in
+flight and whatever the memory budget admits at a time."""
+import numpy as np
+import pytest
+
+from pm355.scenes import camera, cornell_box, cornell_camera, random_scene
+
+pytestmark = pytest.mark.gpu
+
+SEED_STEP = 0x9E3779B97F4A7C15
+
+
+def _jobs(ctx):
+    specs = []
+    for i, sc in enumerate([cornell_box(), random_scene(4), random_scene(9)]):
+        ds = ctx.upload_scene(sc, accel=True)
+        cam = [cornell_camera(96, 64), camera((0.5, 0.6, 1.6), (0.1, -0.2, -1.0), (0, 1, 0), 50.0, 64, 48),
+               cornell_camera(80, 60)][i]
+        specs.append(dict(dscene=ds, aabb=sc.aabb(), cell_size=(0.05, 0.04, 0.03)[i],
+                          n_global=(50_000, 30_000, 20_000)[i], n_caustic=(5_000, 3_000, 0)[i], camera=cam,
+                          frames=(3, 2, 4)[i], seed=100 + i, max_depth=(4, 3, 5)[i], k_global=(50, 64, 32)[i],
+                          r_global=(0.05, 0.04, 0.03)[i], k_caustic=(40, 20, 16)[i], r_caustic=(0.05, 0.04, 0.03)[i],
+                          max_paths=20_000_000))
+    return specs
+
+
+def _one_at_a_time(ctx, s):
+    out = []
+    for f in range(s["frames"]):
+        seed = (s["seed"] + SEED_STEP * (f + 1)) & ((1 << 64) - 1)
+        g, c, st = ctx.trace_photons(s["dscene"], seed, s["n_global"], s["n_caustic"], 10, max_paths=s["max_paths"])
+        gg = ctx.grid_build(g, s["aabb"], cell_size=s["cell_size"])
+        gc = ctx.grid_build(c, s["aabb"], cell_size=s["cell_size"]) if st[2] > 0 else None
+        _, rgb8, _ = ctx.render_frame(s["dscene"], gg, gc, s["camera"], s["max_depth"], s["k_global"],
+                                      s["r_global"], s["k_caustic"], s["r_caustic"])
+        ctx.sync()
+        out.append(rgb8.cpu().numpy())
+    return np.stack(out)
+
+
+def _serve(ctx, specs, budget=0):
+    srv = ctx.server(budget)
+    outs = [srv.submit(**s)[1] for s in specs]
+    info = srv.info()
+    steps = 0
+    while srv.info()["finished"] < len(specs):
+        srv.step()
+        steps += 1
+        assert steps < 100
+    ctx.sync()
+    lat = srv.latencies()
+    res = [o.cpu().numpy() for o in outs], info, lat, srv.info()
+    srv.close()
+    return res
+
+
+def test_served_frames_equal_alone_and_one_at_a_time(ctx):
+    specs = _jobs(ctx)
+    mixed, info0, lat, info = _serve(ctx, specs)
+    assert info0["queued"] == 3 and info["finished"] == 3 and info["frames"] == sum(s["frames"] for s in specs)
+    assert len(lat) == info["frames"] and np.all(lat > 0)
+    for s, m in zip(specs, mixed):
+        alone, _, _, _ = _serve(ctx, [s])
+        assert np.array_equal(m, alone[0])
+        assert np.array_equal(m, _one_at_a_time(ctx, s))
+        assert m.any()
+
+
+def test_budget_admits_jobs_as_memory_frees(ctx):
+    specs = _jobs(ctx)
+    small, info0, _, _ = _serve(ctx, specs, budget=100 << 20)  # one job at a time
+    wide, _, _, _ = _serve(ctx, specs)
+    for a, b in zip(small, wide):
+        assert np.array_equal(a, b)
