@@ -63,6 +63,53 @@ def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
     }
 
 
+def cpu_stages(threads):
+    """Per-stage timings of the oracle on the host cores (the reference CPU
+    path restated): photon tracing of the Cornell box (1 thread, as the
+    oracle's tracer is serial), a 2M-photon grid build and kd-tree build
+    (1 thread), kNN + radiance at k = 64 on 1 thread and on all threads, and
+    a 320x240 frame (100k photons, k = 50, r = 0.05) on all threads. Each
+    stage runs once on a bounded input; the units say what was counted."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    from pm355.scenes import cornell_box, cornell_camera
+    from pm355.workloads import gather_microbench
+
+    out = {"threads_all": threads, "nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            out["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    sc = cornell_box()
+    t0 = time.perf_counter()
+    g, c, st = pyoracle.trace_photons(sc, 7, 100_000, 10_000, 8)
+    dt = time.perf_counter() - t0
+    out["trace_photons_per_s_1thread"] = (st[1] + st[2]) / dt
+    ph, q, qn, aabb = gather_microbench(seed=3, n_photons=2_000_000, n_queries=20_000)
+    t0 = time.perf_counter()
+    og = pyoracle.OracleGrid(ph, aabb, cell_size=0.01)
+    out["grid_build_2M_ms_1thread"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    pyoracle.kdtree_build(ph[0:3])
+    out["kdtree_build_2M_ms_1thread"] = (time.perf_counter() - t0) * 1e3
+    for name, th in (("1thread", 1), ("all", threads)):
+        nq = 5_000 if th == 1 else 20_000
+        t0 = time.perf_counter()
+        oi, od, oc = og.knn(q[:, :nq], 0.01, 64, threads=th)
+        pyoracle.radiance(oi, od, oc, og.photons, qn[:, :nq], 64)
+        out[f"knn_radiance_queries_per_s_{name}"] = nq / (time.perf_counter() - t0)
+    gg = pyoracle.OracleGrid(g, sc.aabb(), cell_size=0.05)
+    gc = pyoracle.OracleGrid(c, sc.aabb(), cell_size=0.05)
+    t0 = time.perf_counter()
+    pyoracle.render(sc, gg, gc, cornell_camera(320, 240), 4, 50, 0.05, 50, 0.05, threads=threads)
+    out["frame_320x240_ms_all"] = (time.perf_counter() - t0) * 1e3
+    return out
+
+
 def frame_bench(ctx, n_frames):
     """The per-frame form on the benchmark's Cornell box: 2M global + 500k
     caustic photons traced, both hash grids (cell = r) and both kd-trees
@@ -179,9 +226,9 @@ def batch_bench(ctx, n_jobs, reps=2):
     }
 
 
-def serve_bench(ctx, rates=(10, 50, 200), n_jobs=16, seed=11):
+def serve_bench(ctx, rates=(10, 50, 200), n_jobs=12, seed=11):
     """The served form: Poisson arrivals of render jobs (scenes of the batched
-    distribution, resident with their BVHs; 50k-8M photons, 10% caustic;
+    distribution, resident with their BVHs; 50k-8M photons, 5% caustic;
     frames of 128x128 up to 1920x1080; 1-30 frames each; k = 64,
     r in [0.01, 0.05], cell = r) into the continuous-batching server.
     Reports per arrival rate the p50 / p99 frame latency (eligible ->
@@ -205,7 +252,7 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=16, seed=11):
             w, h = res_set[int(rng.integers(0, len(res_set)))]
             ng = int(np.exp(rng.uniform(np.log(5e4), np.log(8e6))))
             r = float(rng.uniform(0.01, 0.05))
-            jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 10, frames=int(rng.integers(1, 31)), r=r,
+            jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 20, frames=int(rng.integers(1, 31)), r=r,
                              seed=int(rng.integers(1, 1 << 62))))
         srv = ctx.server()
         t0 = time.perf_counter()
@@ -217,7 +264,7 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=16, seed=11):
                 sc = scenes[jb["scene"]]
                 cam = camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, jb["w"], jb["h"])
                 srv.submit(dss[jb["scene"]], sc.aabb(), jb["r"], jb["ng"], jb["nc"], cam, jb["frames"], jb["seed"],
-                           k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=8 * jb["ng"])
+                           k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=3 * jb["ng"])
                 nxt += 1
             if srv.step() == 0 and nxt < len(jobs):
                 time.sleep(max(0.0, min(0.01, jobs[nxt]["at"] - (time.perf_counter() - t0))))
@@ -246,7 +293,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
-    ap.add_argument("--serve-jobs", type=int, default=16, help="render jobs per arrival rate for the served form (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -404,12 +451,13 @@ def main():
     if args.serve_jobs > 0:
         log(f"served form: {args.serve_jobs} jobs per arrival rate")
         res["served"] = {"workload": "Poisson arrivals of render jobs: 6 resident scenes of the batched distribution "
-                                     "(unit box + 1-8 spheres of 200-20k triangles, BVHs), 50k-8M global photons + 10% "
-                                     "caustic re-traced per frame, 128^2..1920x1080, 1-30 frames, k=64, r U[0.01,0.05]",
+                                     "(unit box + 1-8 spheres of 200-20k triangles, BVHs), 50k-8M global photons + 5% "
+                                     "caustic re-traced per frame (at most 3 paths per global photon), 128^2..1920x1080, 1-30 frames, k=64, r U[0.01,0.05]",
                          **serve_bench(ctx, n_jobs=args.serve_jobs)}
     if not args.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)
+        res["cpu_stages"] = cpu_stages(res["cpu_baseline"]["cores"])
     ctx.close()
     print(json.dumps(res), flush=True)
 
