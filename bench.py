@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB on the device 3 times (the byte-counter scale reference) and exit")
     ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
@@ -303,6 +305,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        a = torch.ones(256 * 1024 * 1024, device="cuda")  # 1 GiB of float32
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
 
     import pm355
     from pm355.workloads import gather_microbench

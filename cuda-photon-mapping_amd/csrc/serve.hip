@@ -37,7 +37,6 @@ struct Job {
     pm_frame* frame = nullptr;
     pm_grid* gg = nullptr;
     pm_grid* gc = nullptr;
-    bool failed = false;
 };
 
 size_t job_footprint(const pm_job_desc& d) {

@@ -174,10 +174,10 @@ def test_large_k_parity(ctx, k, r):
     ctx.sync()
     assert np.array_equal(_np(cnt, np.uint32), oc)
     got = _np(rgb).astype(np.float64)
-    # query 0 sits on 1500 duplicates: the k-th distance is 0 and the
-    # reference's filter divides 0 by 0 (Raytracer.cuh:127-133), NaN on both sides
+    # query 0 sits on 1501 duplicates: for k <= 1501 the k-th distance is 0 and
+    # the reference's filter divides 0 by 0 (Raytracer.cuh:127-133): NaN on both sides
     nan = np.isnan(o64)
-    assert np.array_equal(np.isnan(got), nan) and nan[:, 0].all()
+    assert np.array_equal(np.isnan(got), nan) and nan[:, 0].all() == (k <= 1501)
     got, o64 = got[:, 1:], o64[:, 1:]
     scale = np.maximum(np.abs(o64), 1e-30)
     rel = np.abs(got - o64) / scale
