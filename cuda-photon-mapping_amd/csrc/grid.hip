@@ -31,16 +31,23 @@ __global__ __launch_bounds__(256) void k_grid_hash(const float* __restrict__ px,
 }
 
 // Cell index of every photon (the hash of PhotonGrid.cu:152-174 without the
-// counting atomics: counts come from the sorted keys).
-__global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ px, const float* __restrict__ py,
-                                                  const float* __restrict__ pz, uint32_t n, GridGeom G,
-                                                  uint32_t* __restrict__ keys) {
+// counting atomics: counts come from the sorted keys). The same pass copies
+// each photon into a 48-byte record (three float4), so the scatter below
+// gathers one record per photon instead of nine scattered 4-byte values.
+__global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9, uint32_t n, GridGeom G,
+                                                  uint32_t* __restrict__ keys, float4* __restrict__ rec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int x = pmd::cell_coord(px[i], G.mn[0], G.cell, G.g[0]);
-    int y = pmd::cell_coord(py[i], G.mn[1], G.cell, G.g[1]);
-    int z = pmd::cell_coord(pz[i], G.mn[2], G.cell, G.g[2]);
+    float v[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) v[p] = in9[(size_t)p * n + i];
+    int x = pmd::cell_coord(v[0], G.mn[0], G.cell, G.g[0]);
+    int y = pmd::cell_coord(v[1], G.mn[1], G.cell, G.g[1]);
+    int z = pmd::cell_coord(v[2], G.mn[2], G.cell, G.g[2]);
     keys[i] = pmd::cell_index(G, x, y, z);
+    rec[3 * (size_t)i + 0] = make_float4(v[0], v[1], v[2], v[3]);
+    rec[3 * (size_t)i + 1] = make_float4(v[4], v[5], v[6], v[7]);
+    rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
 }
 
 // starts[c] = first sorted position with key >= c (the exclusive scan of the
@@ -81,18 +88,19 @@ __global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ a,
 }
 
 // assign_photons_to_cells (PhotonGrid.cu:152-174) with the slot taken from
-// the stable sort instead of an atomic: in-cell order = input order.
-// blockIdx.y is the plane: blocks are dispatched plane by plane, so the
-// randomly gathered source plane (4 B x n) stays resident in the MALL.
-__global__ __launch_bounds__(256) void k_grid_scatter(const float* __restrict__ in9, uint32_t n,
+// the stable sort instead of an atomic: in-cell order = input order. One
+// 48-byte record gathered per photon, nine coalesced plane writes.
+__global__ __launch_bounds__(256) void k_grid_scatter(const float4* __restrict__ rec, uint32_t n,
                                                      const uint32_t* __restrict__ order, float* __restrict__ out9,
                                                      uint32_t* __restrict__ perm) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const uint32_t p = blockIdx.y;
-    uint32_t src = order[j];
-    if (p == 0) perm[j] = src;
-    out9[(size_t)p * n + j] = in9[(size_t)p * n + src];
+    const uint32_t src = order[j];
+    perm[j] = src;
+    const float4 a = rec[3 * (size_t)src + 0], b = rec[3 * (size_t)src + 1], c = rec[3 * (size_t)src + 2];
+    const float v[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) out9[(size_t)p * n + j] = v[p];
 }
 
 struct HostDims {
@@ -188,7 +196,7 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         float cell2 = reference_cell(n, aabb, std::max(1, desired / 5));
         if (make_dims(aabb, cell2, layout, d2)) need_counts = std::max(need_counts, (size_t)d2.n_cells * 4);
     }
-    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, need_counts});
+    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, need_counts, (size_t)n * 48});
     if ((st = pm::scratch_reserve(ctx, total, &work)) != PM_OK) return st;
     pm::Carve cv{static_cast<char*>(work)};
     uint32_t* keys = cv.take<uint32_t>(n);
@@ -197,6 +205,7 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     void* rwork = cv.take<char>(rs);
     uint32_t* dmax = cv.take<uint32_t>(64);
     uint32_t* tmp_counts = cv.take<uint32_t>(need_counts / 4);
+    float4* rec = cv.take<float4>((size_t)n * 3);
 
     if (!(cell_size > 0.f)) {
         GridGeom G = geom_of(aabb, d, layout);
@@ -236,15 +245,14 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         delete g;
         return s;
     };
-    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys);
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_keys launch"));
     if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(d.n_cells), rwork)) != PM_OK)
         return bail(st);
     hipLaunchKernelGGL(k_cell_bounds, dim3(pm::div_up(d.n_cells, 256)), dim3(256), 0, ctx->stream, skeys, n,
                        d.n_cells, starts, sizes);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_cell_bounds launch"));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks, 9), dim3(256), 0, ctx->stream, d_photons9, n, order, photons,
-                       perm);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, rec, n, order, photons, perm);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
 
     pm_grid_desc& D = g->desc;

@@ -40,16 +40,32 @@ __global__ __launch_bounds__(256) void k_axis_keys(const float* __restrict__ p, 
     if (i < n) keys[i] = orderable(p[i]);
 }
 
-__global__ __launch_bounds__(256) void k_gather_recs(const float* __restrict__ px, const float* __restrict__ py,
-                                                    const float* __restrict__ pz, const uint32_t* __restrict__ order,
-                                                    uint32_t n, Rec* __restrict__ out) {
+// Each photon as one 48-byte record (three float4: pos + power.r, power.gb +
+// dir.xy, dir.z): the gathers by sort order and by node read one record
+// instead of up to nine scattered 4-byte values.
+__global__ __launch_bounds__(256) void k_photon_records(const float* __restrict__ in9, uint32_t n,
+                                                       float4* __restrict__ rec) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) v[p] = in9[(size_t)p * n + i];
+    rec[3 * (size_t)i + 0] = make_float4(v[0], v[1], v[2], v[3]);
+    rec[3 * (size_t)i + 1] = make_float4(v[4], v[5], v[6], v[7]);
+    rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_gather_recs(const float4* __restrict__ prec,
+                                                    const uint32_t* __restrict__ order, uint32_t n,
+                                                    Rec* __restrict__ out) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t id = order[j];
+    const float4 a = prec[3 * (size_t)id];
     Rec r;
-    r.c[0] = px[id];
-    r.c[1] = py[id];
-    r.c[2] = pz[id];
+    r.c[0] = a.x;
+    r.c[1] = a.y;
+    r.c[2] = a.z;
     r.id = id;
     out[j] = r;
 }
@@ -350,15 +366,22 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
     }
 }
 
-__global__ __launch_bounds__(256) void k_tree_photons(const float* __restrict__ in9, uint32_t n,
+__global__ __launch_bounds__(256) void k_tree_photons(const float4* __restrict__ prec,
                                                      const uint32_t* __restrict__ node_idx, uint32_t cap,
                                                      float* __restrict__ out9) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cap) return;
     uint32_t id = node_idx[i];
+    float v[9];
+    if (id != kFinal) {
+        const float4 a = prec[3 * (size_t)id], b = prec[3 * (size_t)id + 1], c = prec[3 * (size_t)id + 2];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w; v[8] = c.x;
+    } else {
 #pragma unroll
-    for (int p = 0; p < 9; ++p)
-        out9[(size_t)p * cap + i] = id != kFinal ? in9[(size_t)p * n + id] : (p < 3 ? __builtin_inff() : 0.f);
+        for (int p = 0; p < 9; ++p) v[p] = p < 3 ? __builtin_inff() : 0.f;
+    }
+#pragma unroll
+    for (int p = 0; p < 9; ++p) out9[(size_t)p * cap + i] = v[p];
 }
 
 // ---- model AABB (the live effect of create_kd_tree, KdTree.cu:395-464) ----
@@ -434,7 +457,7 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     size_t rs = pm::radix_sort_scratch_bytes(n);
     size_t need = pm::carve_size({(size_t)n * 16 * 6, (size_t)n * 4 * 3, (size_t)n * 4 * 3, (size_t)n * 4,
                                   (size_t)cap * 8, (size_t)cap, (size_t)cap * 8, (size_t)cap * 12,
-                                  (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs});
+                                  (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, (size_t)n * 48});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, need, &work);
     if (st != PM_OK) return bail(st);
@@ -454,14 +477,15 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     uint32_t* skeys = cv.take<uint32_t>(n);
     uint32_t* order = cv.take<uint32_t>(n);
     void* rw = cv.take<char>(rs);
+    float4* prec = cv.take<float4>((size_t)n * 3);
 
     const uint32_t nb = pm::div_up(n, 256);
     hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_photon_records, dim3(nb), dim3(256), 0, s, d_photons9, n, prec);
     for (int a = 0; a < 3; ++a) {
         hipLaunchKernelGGL(k_axis_keys, dim3(nb), dim3(256), 0, s, d_photons9 + (size_t)a * n, n, keys);
         if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, 32, rw)) != PM_OK) return bail(st);
-        hipLaunchKernelGGL(k_gather_recs, dim3(nb), dim3(256), 0, s, d_photons9, d_photons9 + n,
-                           d_photons9 + 2 * (size_t)n, order, n, A[a]);
+        hipLaunchKernelGGL(k_gather_recs, dim3(nb), dim3(256), 0, s, prec, order, n, A[a]);
     }
     if ((st = pm::fill_u32(ctx, T->node_idx, kFinal, cap)) != PM_OK) return bail(st);
     // node_plane = -1; the carve pads every block to 256 bytes, so whole
@@ -498,7 +522,7 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
                            plane);
     }
-    hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, d_photons9, n, T->node_idx, cap,
+    hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx, cap,
                        T->photons);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "pm_kdtree_build: launch failed"));
     *out = T;

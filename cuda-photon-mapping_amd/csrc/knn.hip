@@ -264,11 +264,21 @@ __device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, f
     }
 }
 
+// XCD-chunked order: XCD x takes chunks x, x + 8, ... of kXcdChunk
+// consecutive workgroups (the grid is rounded up to a multiple of 8 chunks).
+constexpr uint32_t kXcdChunk = 256;
+__device__ __forceinline__ uint32_t xcd_chunked(uint32_t b) {
+    const uint32_t x = b % 8, i = b / 8;
+    return ((i / kXcdChunk) * 8 + x) * kXcdChunk + (i % kXcdChunk);
+}
+
 // Query of this wave in a one-wave-per-query launch (XCD-aware block order,
 // optional processing order); false past the end (wave-uniform).
 __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t blk = P.remap == 2 ? xcd_chunked(blockIdx.x)
+                         : P.remap   ? pmd::xcd_remap(blockIdx.x, gridDim.x)
+                                     : blockIdx.x;
     const uint32_t ql = blk * kWaves + wave;
     if (ql >= P.nq) return false;
     q = P.order ? P.order[ql] : ql;
@@ -435,16 +445,21 @@ __device__ __forceinline__ uint32_t job_of(const uint32_t* __restrict__ qbase, u
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// One wave per position; the 2-D grid keeps every dimension's work-item
-// count below 2^32, and the linear workgroup id is remapped so each XCD
-// takes one contiguous range of positions (neighbouring cells in its L2).
+// One wave per position. Workgroups go to the 8 XCDs round-robin; the
+// linear workgroup id is remapped so that XCD x takes chunks x, x + 8, ...
+// of kXcdChunk consecutive workgroups: each XCD's L2 still sees runs of
+// neighbouring cells, while at any moment all XCDs work inside the same
+// region of the (job, cell) order and share the photon blocks in the MALL
+// (a contiguous eighth per XCD would put eight different jobs in flight).
+// The 2-D grid keeps every dimension's work-item count below 2^32; the
+// workgroups past the rounded-up count exit at once.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_knn_batch_grid(
     const KnnParams* __restrict__ jobs, const uint32_t* __restrict__ qbase, uint32_t n_jobs,
-    const uint32_t* __restrict__ order, uint32_t total) {
+    const uint32_t* __restrict__ order, uint32_t total, uint32_t n_blocks) {
     __shared__ WaveLds<512> lds[kWaves];
-    const uint32_t nb = gridDim.x * gridDim.y;
-    const uint32_t blk = pmd::xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
-    const uint32_t p = blk * kWaves + (threadIdx.x >> 6);
+    const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (b >= n_blocks) return;  // n_blocks is a multiple of 8 * kXcdChunk
+    const uint32_t p = xcd_chunked(b) * kWaves + (threadIdx.x >> 6);
     if (p >= total) return;  // wave-uniform
     const uint32_t g = order ? order[p] : p;
     const uint32_t j = job_of(qbase, n_jobs, g);
@@ -686,7 +701,7 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     if (!(rcf < 4096.f)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: radius/cell ratio too large", who);
     P.rc = (int)rcf;
     P.k = k;
-    P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : 1;
+    P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : (flags & PM_KNN_XCD_CHUNKS) ? 2 : 1;
     return PM_OK;
 }
 
@@ -736,7 +751,9 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
         PM_CHECK_LAUNCH(ctx, "k_knn_large");
         return PM_OK;
     }
-    const dim3 grid_dim(pm::div_up(P.nq, kWaves)), block(256);
+    uint32_t nblk = pm::div_up(P.nq, kWaves);
+    if (P.remap == 2) nblk = pm::div_up(nblk, 8 * kXcdChunk) * 8 * kXcdChunk;
+    const dim3 grid_dim(nblk), block(256);
     if (flags & PM_KNN_TWO_PASS)
         hipLaunchKernelGGL(k_knn_two_pass, grid_dim, block, 0, ctx->stream, P);
     else if (flags & PM_KNN_SMALL_BUFFER)
@@ -910,10 +927,11 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
         hipLaunchKernelGGL(k_knn_batch<512>, dim3(blocks), dim3(256), 0, ctx->stream, d_jobs, d_qbase, n_jobs, order,
                            nt, d_counter);
     } else {
-        const uint32_t blocks = pm::div_up(nt, kWaves);
+        const uint32_t step = 8 * kXcdChunk;
+        const uint32_t blocks = pm::div_up(pm::div_up(nt, kWaves), step) * step;
         const uint32_t gx = std::min<uint32_t>(blocks, 65536u);
         hipLaunchKernelGGL(k_knn_batch_grid, dim3(gx, pm::div_up(blocks, gx)), dim3(256), 0, ctx->stream, d_jobs,
-                           d_qbase, n_jobs, order, nt);
+                           d_qbase, n_jobs, order, nt, blocks);
     }
     PM_CHECK_LAUNCH(ctx, "k_knn_batch");
     return PM_OK;

@@ -32,7 +32,9 @@ KNN_NO_XCD_REMAP = 2
 KNN_TWO_PASS = 1 << 4
 KNN_SMALL_BUFFER = 1 << 5
 KNN_BATCH_QUEUE = 1 << 6
-KNN_VARIANTS = {"stream1024": 0, "stream512": KNN_SMALL_BUFFER, "two_pass": KNN_TWO_PASS}
+KNN_XCD_CHUNKS = 1 << 7
+KNN_VARIANTS = {"stream1024": 0, "stream512": KNN_SMALL_BUFFER, "two_pass": KNN_TWO_PASS,
+                "stream512_chunked": KNN_SMALL_BUFFER | KNN_XCD_CHUNKS}
 
 _P = C.c_void_p
 _U32 = C.c_uint32

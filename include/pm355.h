@@ -49,7 +49,9 @@ enum {
     PM_KNN_NO_XCD_REMAP = 1u << 1, /* disable the XCD-aware block mapping */
     PM_KNN_TWO_PASS = 1u << 4,     /* tuning: histogram pass + collect pass instead of one streaming pass */
     PM_KNN_SMALL_BUFFER = 1u << 5, /* tuning: 512-key LDS buffer per query (higher occupancy) */
-    PM_KNN_BATCH_QUEUE = 1u << 6   /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
+    PM_KNN_BATCH_QUEUE = 1u << 6,  /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
+    PM_KNN_XCD_CHUNKS = 1u << 7    /* tuning: XCDs take interleaved chunks of 256 workgroups instead of one
+                                      contiguous eighth each */
 };
 
 typedef struct pm_ctx pm_ctx;
