@@ -190,7 +190,7 @@ def batch_bench(ctx, n_jobs, reps=2):
         nq_total += j["queries"]
         n_ph += j["photons"]
     torch.cuda.synchronize()
-    plain_flags = pm355.KNN_SORT_QUERIES | pm355.KNN_SMALL_BUFFER
+    plain_flags = pm355.KNN_SORT_QUERIES
 
     def plain():
         for g, q, qn, r, k in jobs:
@@ -436,7 +436,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": f"k_knn_{best} (fused gather + radiance, cell-ordered queries)",
+            "kernel": {"stream512_chunked": "k_knn_stream512", "stream512": "k_knn_stream512",
+                       "stream1024": "k_knn_stream1024", "two_pass": "k_knn_two_pass"}.get(best, best)
+                      + f" (variant {best}; fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
             "alg_bytes": alg_bytes,
         },

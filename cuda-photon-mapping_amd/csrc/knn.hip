@@ -701,7 +701,7 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     if (!(rcf < 4096.f)) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: radius/cell ratio too large", who);
     P.rc = (int)rcf;
     P.k = k;
-    P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : (flags & PM_KNN_XCD_CHUNKS) ? 2 : 1;
+    P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : (flags & PM_KNN_XCD_CONTIGUOUS) ? 1 : 2;
     return PM_OK;
 }
 
@@ -756,10 +756,10 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
     const dim3 grid_dim(nblk), block(256);
     if (flags & PM_KNN_TWO_PASS)
         hipLaunchKernelGGL(k_knn_two_pass, grid_dim, block, 0, ctx->stream, P);
-    else if (flags & PM_KNN_SMALL_BUFFER)
-        hipLaunchKernelGGL(k_knn_stream512, grid_dim, block, 0, ctx->stream, P);
-    else
+    else if (flags & PM_KNN_LARGE_BUFFER)
         hipLaunchKernelGGL(k_knn_stream1024, grid_dim, block, 0, ctx->stream, P);
+    else
+        hipLaunchKernelGGL(k_knn_stream512, grid_dim, block, 0, ctx->stream, P);
     PM_CHECK_LAUNCH(ctx, "k_knn");
     return PM_OK;
 }

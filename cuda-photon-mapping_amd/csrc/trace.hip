@@ -49,6 +49,7 @@ struct TraceParams {
     uint8_t* kind;    // [max_b][n_paths]: 0 global, 1 caustic
     uint32_t* cnt_g;  // [n_paths]
     uint32_t* cnt_c;  // [n_paths]
+    int caustic_only;  // the global map is already full: stop a path once it cannot store caustics
 };
 
 struct Medium {
@@ -194,15 +195,17 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
             if (dest == DIF || (dest == ABSORB && !ks_one)) {
                 // PathOperator::response (PathOperator.cuh:28-37)
                 const bool caustic = (last == SPEC || last == REFR) && diffuse_surfs == 0;
-                const float v[9] = {ip.x, ip.y, ip.z, pp.x, pp.y, pp.z, ray.d.x, ray.d.y, ray.d.z};
+                if (caustic || !P.caustic_only) {
+                    const float v[9] = {ip.x, ip.y, ip.z, pp.x, pp.y, pp.z, ray.d.x, ray.d.y, ray.d.z};
 #pragma unroll
-                for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + ev) * P.n_paths + i] = v[c];
-                P.kind[(size_t)ev * P.n_paths + i] = caustic ? 1 : 0;
-                ++ev;
-                if (caustic)
-                    ++nc;
-                else
-                    ++ng;
+                    for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + ev) * P.n_paths + i] = v[c];
+                    P.kind[(size_t)ev * P.n_paths + i] = caustic ? 1 : 0;
+                    ++ev;
+                    if (caustic)
+                        ++nc;
+                    else
+                        ++ng;
+                }
             } else if (dest == SPEC) {
                 // Ray::reflect (Ray.cuh:64-71)
                 float dnd = dot(ray.d, n);
@@ -230,6 +233,9 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
         if (dest == DIF) ++diffuse_surfs;
         last = dest;
         if (dest == ABSORB || end) break;
+        // after a diffuse event no later photon of this path is caustic
+        // (PathOperator.cuh:28-37); with the global map full it stores nothing more
+        if (P.caustic_only && diffuse_surfs > 0) break;
         ray = nr;
     }
     P.cnt_g[i] = ng;
@@ -358,6 +364,10 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         uint32_t nb = (uint32_t)std::min<uint64_t>(batch, max_paths - paths);
         P.path_base = paths;
         P.n_paths = nb;
+        // Once the global map is full, its later photons are dropped anyway,
+        // so paths stop at their first diffuse event: the caustic photons
+        // (and the order they are stored in) are unchanged.
+        P.caustic_only = got_g >= n_global ? 1 : 0;
         hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P);
         PM_CHECK_LAUNCH(ctx, "k_trace_paths");
         if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;

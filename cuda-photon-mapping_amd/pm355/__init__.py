@@ -30,11 +30,13 @@ LAYOUT_MORTON = 1
 KNN_SORT_QUERIES = 1
 KNN_NO_XCD_REMAP = 2
 KNN_TWO_PASS = 1 << 4
-KNN_SMALL_BUFFER = 1 << 5
+KNN_LARGE_BUFFER = 1 << 5
 KNN_BATCH_QUEUE = 1 << 6
-KNN_XCD_CHUNKS = 1 << 7
-KNN_VARIANTS = {"stream1024": 0, "stream512": KNN_SMALL_BUFFER, "two_pass": KNN_TWO_PASS,
-                "stream512_chunked": KNN_SMALL_BUFFER | KNN_XCD_CHUNKS}
+KNN_XCD_CONTIGUOUS = 1 << 7
+# gather variants (tuning flags): the default is the 512-key buffer with
+# XCD-interleaved chunks of workgroups
+KNN_VARIANTS = {"stream512_chunked": 0, "stream512": KNN_XCD_CONTIGUOUS, "stream1024": KNN_LARGE_BUFFER,
+                "two_pass": KNN_TWO_PASS}
 
 _P = C.c_void_p
 _U32 = C.c_uint32

@@ -46,12 +46,13 @@ enum { PM_LAYOUT_LINEAR = 0, PM_LAYOUT_MORTON = 1 };
 /* pm_grid_knn / pm_grid_gather_radiance flags */
 enum {
     PM_KNN_SORT_QUERIES = 1u << 0, /* process queries in cell order (outputs stay in caller order) */
-    PM_KNN_NO_XCD_REMAP = 1u << 1, /* disable the XCD-aware block mapping */
+    PM_KNN_NO_XCD_REMAP = 1u << 1, /* disable the XCD-aware block mapping (hardware order) */
     PM_KNN_TWO_PASS = 1u << 4,     /* tuning: histogram pass + collect pass instead of one streaming pass */
-    PM_KNN_SMALL_BUFFER = 1u << 5, /* tuning: 512-key LDS buffer per query (higher occupancy) */
-    PM_KNN_BATCH_QUEUE = 1u << 6,  /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
-    PM_KNN_XCD_CHUNKS = 1u << 7    /* tuning: XCDs take interleaved chunks of 256 workgroups instead of one
-                                      contiguous eighth each */
+    PM_KNN_LARGE_BUFFER = 1u << 5,   /* tuning: 1024-key LDS buffer per query instead of 512 (fewer
+                                        narrowing passes, lower occupancy) */
+    PM_KNN_BATCH_QUEUE = 1u << 6,    /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
+    PM_KNN_XCD_CONTIGUOUS = 1u << 7  /* tuning: each XCD takes one contiguous eighth of the workgroups
+                                        instead of interleaved chunks of 256 */
 };
 
 typedef struct pm_ctx pm_ctx;
