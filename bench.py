@@ -451,6 +451,20 @@ def main():
         "gather_only_queries_per_s": nq / (gather_ms * 1e-3),
         "box_stats": {"cells": cells, "candidates": cands, "kept": kept},
     }
+    # Cell size relative to r, reported beside the reference-compatible cell = r:
+    # the same gather (default variant, cell-ordered queries) on grids of
+    # cell r/2, 2r/3 and r.
+    cell_tune = {}
+    for frac in (0.5, 2.0 / 3.0, 1.0):
+        gt = ctx.grid_build(d_ph, aabb, cell_size=r * frac)
+        ot = ctx.query_order(gt, d_q).long()
+        qs, ns = d_q[:, ot].contiguous(), d_n[:, ot].contiguous()
+        ctx.gather_radiance(gt, qs, ns, r, k)
+        ms = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k), 3)
+        cell_tune[f"{frac:.3f}"] = {"gather_ms": ms, "queries_per_s": nq / (ms * 1e-3)}
+        gt.close()
+    res["autotune_cell_over_r"] = cell_tune
+
     # kd-tree build over the same 16M photons (PhotonMap), HIP events
     kd_ms = timed(lambda: ctx.kdtree_build(d_ph).close(), 3)
     res["components_ms"]["kdtree_build_16M"] = kd_ms

@@ -218,31 +218,43 @@ __global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0
 // Once every segment holds at most kLocal records, one workgroup finishes a
 // whole subtree in LDS: the same median / stable-partition / child-plane
 // steps as the level loop above, on the subtree's slice of the three axis
-// orders, with workgroup scans instead of device scans.
+// orders, with workgroup scans instead of device scans. The subtree's records
+// are loaded once (in axis-0 order); the three orders are permutations of
+// 16-bit local indices, so a partition moves 2 bytes per position and a
+// workgroup needs 46 KB of LDS (three workgroups per CU).
 constexpr int kLocal = 1024;
 constexpr int kLocalThreads = 256;
 
 struct LocalLds {
-    Rec a[3][kLocal];
-    Rec tmp[kLocal];
-    uint32_t pref[kLocal];
-    uint16_t node[kLocal];      // local node (at the current depth) of each position, 0xFFFF = final
-    uint2 seg[2][kLocal];       // node segments, current and next depth
+    Rec rec[kLocal];          // the subtree's records in axis-0 order
+    uint16_t ord[3][kLocal];  // per axis: local record index at each position
+    uint16_t tmp[kLocal];
+    uint16_t pref[kLocal];
+    uint16_t node[kLocal];    // local node (at the current depth) of each position, 0xFFFF = final
+    ushort2 seg[2][kLocal];   // node segments, current and next depth
     int8_t plane[2][kLocal];
-    uint2 med[kLocal];
-    uint32_t mpos[kLocal / 2][3];
+    uint2 med[kLocal / 2];
+    uint16_t mpos[kLocal / 2][3];
+    uint8_t flag[kLocal];
     uint32_t wsum[kLocalThreads / 64];
 };
 
-// Exclusive scan of flags[0, m) (0/1) into L.pref; returns the total.
-__device__ __forceinline__ void local_scan(LocalLds& L, const uint8_t* fl, uint32_t m) {
+// pos0[id] = position of photon id in the axis-0 order (for the subtrees'
+// index maps).
+__global__ __launch_bounds__(256) void k_pos0(const Rec* __restrict__ A0, uint32_t n, uint32_t* __restrict__ pos0) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) pos0[A0[j].id] = j;
+}
+
+// Exclusive scan of L.flag[0, m) (0/1) into L.pref.
+__device__ __forceinline__ void local_scan(LocalLds& L, uint32_t m) {
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     constexpr uint32_t per = kLocal / kLocalThreads;  // 4 positions per thread
     uint32_t v[per], sum = 0;
 #pragma unroll
     for (uint32_t i = 0; i < per; ++i) {
         uint32_t j = t * per + i;
-        v[i] = j < m ? fl[j] : 0u;
+        v[i] = j < m ? L.flag[j] : 0u;
         sum += v[i];
     }
     uint32_t inc = sum;
@@ -259,30 +271,50 @@ __device__ __forceinline__ void local_scan(LocalLds& L, const uint8_t* fl, uint3
 #pragma unroll
     for (uint32_t i = 0; i < per; ++i) {
         uint32_t j = t * per + i;
-        if (j < m) L.pref[j] = run;
+        if (j < m) L.pref[j] = (uint16_t)run;
         run += v[i];
     }
     __syncthreads();
 }
 
+// find_largest_plane over local segment [f, t) (PhotonMap.cuh:161-178)
+__device__ __forceinline__ int8_t local_plane(const LocalLds& L, uint32_t f, uint32_t t) {
+    float d0 = fabsf(L.rec[L.ord[0][t - 1]].c[0] - L.rec[L.ord[0][f]].c[0]);
+    float d1 = fabsf(L.rec[L.ord[1][t - 1]].c[1] - L.rec[L.ord[1][f]].c[1]);
+    float d2 = fabsf(L.rec[L.ord[2][t - 1]].c[2] - L.rec[L.ord[2][f]].c[2]);
+    int8_t p = 0;
+    float mx = d0;
+    if (d1 > mx) {
+        mx = d1;
+        p = 1;
+    }
+    if (d2 > mx) p = 2;
+    return p;
+}
+
 __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                                 const Rec* __restrict__ A2, uint32_t first,
+                                                                 const Rec* __restrict__ A2,
+                                                                 const uint32_t* __restrict__ pos0, uint32_t first,
                                                                  uint32_t depth_left, const uint2* __restrict__ gseg,
                                                                  const int8_t* __restrict__ gplane,
                                                                  uint32_t* __restrict__ node_idx,
                                                                  int8_t* __restrict__ node_plane) {
     __shared__ LocalLds L;
-    __shared__ uint8_t flag[kLocal];
     const uint32_t root = first + blockIdx.x, t = threadIdx.x;
     const uint2 rs = gseg[root];
     if (rs.y <= rs.x || rs.y - rs.x > (uint32_t)kLocal) return;  // empty subtree; larger ones never reach here
     const uint32_t m = rs.y - rs.x;
-    const Rec* As[3] = {A0, A1, A2};
-    for (int a = 0; a < 3; ++a)
-        for (uint32_t j = t; j < m; j += kLocalThreads) L.a[a][j] = As[a][rs.x + j];
-    for (uint32_t j = t; j < m; j += kLocalThreads) L.node[j] = 0;
+    // the segment holds the same photons in each order: axes 1 and 2 map their
+    // records to axis-0 positions through pos0
+    for (uint32_t j = t; j < m; j += kLocalThreads) {
+        L.rec[j] = A0[rs.x + j];
+        L.ord[0][j] = (uint16_t)j;
+        L.ord[1][j] = (uint16_t)(pos0[A1[rs.x + j].id] - rs.x);
+        L.ord[2][j] = (uint16_t)(pos0[A2[rs.x + j].id] - rs.x);
+        L.node[j] = 0;
+    }
     if (t == 0) {
-        L.seg[0][0] = make_uint2(0, m);
+        L.seg[0][0] = make_ushort2(0, (uint16_t)m);
         L.plane[0][0] = gplane[root];
     }
     __syncthreads();
@@ -293,18 +325,18 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
         const bool last = d + 1 == depth_left;
         // medians (k_level_nodes) and the children's segments
         for (uint32_t i = t; i < nd; i += kLocalThreads) {
-            uint2 s = L.seg[cur][i];
-            uint2 cl = make_uint2(0, 0), cr = make_uint2(0, 0);
+            const ushort2 s = L.seg[cur][i];
+            ushort2 cl = make_ushort2(0, 0), cr = make_ushort2(0, 0);
             if (s.y > s.x) {
-                int8_t p = L.plane[cur][i];
-                uint32_t mid = s.x + (s.y - s.x) / 2;
-                Rec r = L.a[p][mid];
+                const int8_t p = L.plane[cur][i];
+                const uint32_t mid = s.x + (s.y - s.x) / 2;
+                const Rec r = L.rec[L.ord[p][mid]];
                 node_idx[gfirst + i] = r.id;
                 node_plane[gfirst + i] = p;
-                L.med[i] = make_uint2(__float_as_uint(r.c[p]), r.id);
+                if (!last) L.med[i] = make_uint2(__float_as_uint(r.c[p]), r.id);
                 if (s.y - s.x > 1) {
-                    cl = make_uint2(s.x, mid);
-                    cr = make_uint2(mid + 1, s.y);
+                    cl = make_ushort2(s.x, (uint16_t)mid);
+                    cr = make_ushort2((uint16_t)(mid + 1), s.y);
                 }
             }
             if (!last) {
@@ -320,46 +352,46 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
                 const uint32_t i = L.node[j];
                 uint8_t f = 0;
                 if (i != 0xFFFFu) {
-                    int sd = side_of(L.a[a][j], L.plane[cur][i], L.med[i]);
+                    const int sd = side_of(L.rec[L.ord[a][j]], L.plane[cur][i], L.med[i]);
                     f = sd == 0;
-                    if (sd == 1) L.mpos[i][a] = j;
+                    if (sd == 1) L.mpos[i][a] = (uint16_t)j;
                 }
-                flag[j] = f;
+                L.flag[j] = f;
             }
             __syncthreads();
-            local_scan(L, flag, m);
+            local_scan(L, m);
             for (uint32_t j = t; j < m; j += kLocalThreads) {
                 const uint32_t i = L.node[j];
                 uint32_t dst = j;
                 if (i != 0xFFFFu) {
-                    const uint2 sg = L.seg[cur][i];
+                    const ushort2 sg = L.seg[cur][i];
                     const uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
-                    const uint32_t before_l = L.pref[j] - L.pref[from];
-                    if (flag[j]) {
+                    const uint32_t before_l = (uint32_t)L.pref[j] - L.pref[from];
+                    if (L.flag[j]) {
                         dst = from + before_l;
                     } else {
                         const uint32_t mp = L.mpos[i][a];
                         dst = mp == j ? mid : mid + 1 + ((j - from) - before_l - (mp < j ? 1u : 0u));
                     }
                 }
-                L.tmp[dst] = L.a[a][j];
+                L.tmp[dst] = L.ord[a][j];
             }
             __syncthreads();
-            for (uint32_t j = t; j < m; j += kLocalThreads) L.a[a][j] = L.tmp[j];
+            for (uint32_t j = t; j < m; j += kLocalThreads) L.ord[a][j] = L.tmp[j];
             __syncthreads();
         }
         // node of every position at the next depth
         for (uint32_t j = t; j < m; j += kLocalThreads) {
             const uint32_t i = L.node[j];
             if (i == 0xFFFFu) continue;
-            const uint2 sg = L.seg[cur][i];
+            const ushort2 sg = L.seg[cur][i];
             const uint32_t mid = sg.x + (sg.y - sg.x) / 2;
             L.node[j] = (sg.y - sg.x <= 1 || j == mid) ? (uint16_t)0xFFFFu : (uint16_t)(j < mid ? 2 * i : 2 * i + 1);
         }
         // child planes (k_child_planes)
         for (uint32_t c = t; c < 2 * nd; c += kLocalThreads) {
-            const uint2 s = L.seg[cur ^ 1][c];
-            L.plane[cur ^ 1][c] = s.y > s.x ? plane_of(L.a[0], L.a[1], L.a[2], s.x, s.y) : (int8_t)0;
+            const ushort2 s = L.seg[cur ^ 1][c];
+            L.plane[cur ^ 1][c] = s.y > s.x ? local_plane(L, s.x, s.y) : (int8_t)0;
         }
         __syncthreads();
         cur ^= 1;
@@ -505,7 +537,8 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
         uint32_t first = (1u << lvl) - 1, count = 1u << lvl;
         if (lvl == Lg) {
-            hipLaunchKernelGGL(k_local_subtrees, dim3(count), dim3(kLocalThreads), 0, s, A[0], A[1], A[2], first,
+            hipLaunchKernelGGL(k_pos0, dim3(nb), dim3(256), 0, s, A[0], n, keys);  // keys: free from here on
+            hipLaunchKernelGGL(k_local_subtrees, dim3(count), dim3(kLocalThreads), 0, s, A[0], A[1], A[2], keys, first,
                                L - lvl, seg, plane, T->node_idx, T->node_plane);
             break;
         }
