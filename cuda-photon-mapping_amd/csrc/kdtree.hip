@@ -8,9 +8,10 @@
 //     tree is defined on), stored as 16-byte records {x, y, z, id};
 //   * at every level each node's segment [from, to) has its median at
 //     mid = from + (to - from)/2 in the order of the node's split axis;
-//     the other two orders are stable-partitioned around that median
-//     (flag -> scan -> scatter), so every order stays sorted inside every
-//     child segment;
+//     the other two orders are stable-partitioned around that median in one
+//     pass (a segmented scan with decoupled look-back), so every order stays
+//     sorted inside every child segment; subtrees of at most 1024 photons
+//     are finished in LDS by one workgroup each;
 //   * a child's split plane is the argmax extent of its points, read from
 //     the first and last record of its segment in each axis order.
 // The implicit heap (children 2i+1, 2i+2) is stored with 2^L - 1 slots so no
@@ -128,79 +129,179 @@ __global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0,
 // side of record e relative to node's median on plane p: 0 left, 1 median, 2 right
 __device__ __forceinline__ int side_of(const Rec& e, int p, uint2 m) {
     if (e.id == m.y) return 1;
-    float a = e.c[p], b = __uint_as_float(m.x);
+    const float a = p == 0 ? e.c[0] : (p == 1 ? e.c[1] : e.c[2]), b = __uint_as_float(m.x);
     bool less = a < b || (a == b && e.id < m.y);
     return less ? 0 : 2;
 }
 
-// flags[a*n + j] = (record j of order a goes left); mpos[node*3 + a] = j of the median.
-__global__ __launch_bounds__(256) void k_level_flags(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                    const Rec* __restrict__ A2, uint32_t n,
-                                                    const uint32_t* __restrict__ node_of,
-                                                    const int8_t* __restrict__ plane, const uint2* __restrict__ med,
-                                                    uint32_t* __restrict__ flags, uint32_t* __restrict__ mpos) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t s = node_of[j];
-    if (s == kFinal) {
-        flags[j] = 0;
-        flags[n + j] = 0;
-        flags[2 * (size_t)n + j] = 0;
-        return;
-    }
-    int p = plane[s];
-    uint2 m = med[s];
-    const Rec* As[3] = {A0, A1, A2};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        Rec e = As[a][j];
-        int sd = side_of(e, p, m);
-        flags[(size_t)a * n + j] = sd == 0 ? 1u : 0u;
-        if (sd == 1) mpos[(size_t)s * 3 + a] = j;
-    }
+// ---- one level's stable partition in one pass --------------------------------
+// The three axis orders are one stream of 3n positions (axis-major). Each
+// position carries a segmented-scan value: bit 31 = a segment starts here
+// (the node's `from`, or a finished position), bit 30 = the node's median,
+// bits 0-29 = goes left. A tile scans its positions in registers / LDS and
+// takes its prefix from earlier tiles by decoupled look-back (epoch-tagged
+// tile words, as in the u32 scan); within a node's segment the inclusive
+// value then counts the left records (Lb) and the median (Mb) up to each
+// position, which gives the destination directly:
+//   left: from + Lb,   median: mid,   right: mid + 1 + (j - from - Lb - Mb).
+// This replaces flags -> device scan -> scatter (three passes over 3n
+// positions) by one.
+constexpr int kPThreads = 256;
+constexpr int kPItems = 8;                      // rows of 256 positions per tile
+constexpr uint32_t kPTile = kPThreads * kPItems;  // 2048
+constexpr uint32_t kSegH = 1u << 31, kSegM = 1u << 30, kSegL = (1u << 30) - 1;
+constexpr uint64_t kStAgg = 1, kStIncl = 2;
+
+// x then y
+__device__ __forceinline__ uint32_t seg_op(uint32_t x, uint32_t y) {
+    if (y & kSegH) return y;
+    return ((x & kSegL) + (y & kSegL)) | ((x | y) & kSegM) | (x & kSegH);
 }
 
-__global__ __launch_bounds__(256) void k_level_scatter(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                      const Rec* __restrict__ A2, Rec* __restrict__ B0,
-                                                      Rec* __restrict__ B1, Rec* __restrict__ B2, uint32_t n,
-                                                      uint32_t* __restrict__ node_of, const uint2* __restrict__ seg,
-                                                      const uint32_t* __restrict__ flags,
-                                                      const uint32_t* __restrict__ pref,
-                                                      const uint32_t* __restrict__ mpos) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t s = node_of[j];
-    const Rec* As[3] = {A0, A1, A2};
-    Rec* Bs[3] = {B0, B1, B2};
-    if (s == kFinal) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) Bs[a][j] = As[a][j];
-        return;
+__device__ __forceinline__ uint64_t seg_state(uint32_t epoch, uint64_t status, uint32_t value) {
+    return ((uint64_t)epoch << 34) | (status << 32) | value;
+}
+
+__global__ __launch_bounds__(kPThreads) void k_level_partition(
+    const Rec* __restrict__ A0, const Rec* __restrict__ A1, const Rec* __restrict__ A2, Rec* __restrict__ B0,
+    Rec* __restrict__ B1, Rec* __restrict__ B2, uint32_t n, const uint32_t* __restrict__ node_in,
+    uint32_t* __restrict__ node_out, const uint2* __restrict__ seg, const int8_t* __restrict__ plane,
+    const uint2* __restrict__ med, uint64_t* __restrict__ state, uint32_t* __restrict__ counter, uint32_t epoch,
+    uint32_t n_tiles) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wagg[kPItems * (kPThreads / 64)];
+    __shared__ uint32_t s_wpre[kPItems * (kPThreads / 64)];
+    __shared__ uint32_t s_prefix;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) {
+        uint32_t id = atomicAdd(counter, 1u);
+        if (id == n_tiles - 1) atomicExch(counter, 0u);  // every tile holds its id: reset for the next launch
+        s_tile = id;
     }
-    uint2 sg = seg[s];
-    uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t total = 3u * n;  // n <= 2^30
+
+    // position of item k: tile row k, wave, lane (rows of 256 consecutive positions)
+    uint32_t val[kPItems], incl[kPItems];
+    Rec rec[kPItems];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        size_t o = (size_t)a * n;
-        uint32_t before_l = pref[o + j] - pref[o + from];
-        uint32_t dst;
-        if (flags[o + j]) {
-            dst = from + before_l;
+    for (int k = 0; k < kPItems; ++k) {
+        const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
+        uint32_t v = kSegH;
+        if (g < total) {
+            const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
+            const uint32_t s = node_in[j];
+            rec[k] = (a == 0 ? A0 : a == 1 ? A1 : A2)[j];
+            if (s != kFinal) {
+                const uint2 sg = seg[s];
+                const int sd = side_of(rec[k], plane[s], med[s]);
+                v = (j == sg.x ? kSegH : 0u) | (sd == 0 ? 1u : 0u) | (sd == 1 ? kSegM : 0u);
+            }
+        }
+        val[k] = v;
+        // wave-level inclusive segmented scan over the row
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= (uint32_t)d) x = seg_op(y, x);
+        }
+        incl[k] = x;
+        if (lane == 63) s_wagg[k * (kPThreads / 64) + wave] = x;
+    }
+    __syncthreads();
+    // exclusive prefixes of the 32 wave rows, in tile order (row-major)
+    if (wave == 0) {
+        constexpr uint32_t nw = kPItems * (kPThreads / 64);
+        uint32_t x = lane < nw ? s_wagg[lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= (uint32_t)d) x = seg_op(y, x);
+        }
+        // x = inclusive over rows 0..lane; exclusive = inclusive of lane - 1
+        uint32_t ex = __shfl_up(x, 1, 64);
+        if (lane < nw) s_wpre[lane] = lane == 0 ? 0u : ex;
+        const uint32_t agg = __shfl(x, nw - 1, 64);
+        // decoupled look-back for the tile prefix
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&state[0], seg_state(epoch, kStIncl, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            uint32_t mp = mpos[(size_t)s * 3 + a];
-            if (mp == j)
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], seg_state(epoch, kStAgg, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int64_t jt = (int64_t)tile - 1;
+            bool have = false;  // prefix holds the combination of the tiles after the current window
+            for (;;) {
+                const int64_t idx = jt - (int64_t)lane;
+                uint64_t w;
+                uint32_t status;
+                uint64_t stop_mask, inval_mask, upto;
+                for (;;) {
+                    if (idx >= 0) {
+                        w = __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        status = ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3) : 0u;
+                    } else {
+                        w = 0;
+                        status = (uint32_t)kStIncl;
+                    }
+                    // a tile whose value holds a segment start ends the walk too
+                    stop_mask = __ballot(status == kStIncl || (status == kStAgg && ((uint32_t)w & kSegH)));
+                    inval_mask = __ballot(status == 0);
+                    const int first = stop_mask ? __builtin_ctzll(stop_mask) : 63;
+                    upto = (first == 63) ? ~0ull : ((2ull << first) - 1);
+                    if ((inval_mask & upto) == 0) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                // combine the window's values from the farthest used lane down to lane 0
+                const int last = (stop_mask & upto) ? __builtin_ctzll(stop_mask) : 63;
+                uint32_t win = 0;
+                for (int l = last; l >= 0; --l) {
+                    const uint32_t wl = (uint32_t)__shfl((uint32_t)w, l, 64);
+                    win = (l == last) ? wl : seg_op(win, wl);
+                }
+                prefix = have ? seg_op(win, prefix) : win;
+                have = true;
+                if (stop_mask & upto) break;
+                jt -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], seg_state(epoch, kStIncl, seg_op(prefix, agg)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    const uint32_t tprefix = s_prefix;
+#pragma unroll
+    for (int k = 0; k < kPItems; ++k) {
+        const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
+        if (g >= total) continue;
+        const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
+        const uint32_t s = node_in[j];
+        uint32_t dst = j, nn = kFinal;
+        if (s != kFinal) {
+            const uint32_t before = seg_op(seg_op(tprefix, s_wpre[k * (kPThreads / 64) + wave]), incl[k]);
+            const uint2 sg = seg[s];
+            const uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
+            const uint32_t own = val[k];
+            const uint32_t lb = (before & kSegL) - (own & kSegL), mb = ((before & kSegM) ? 1u : 0u) -
+                                                                       ((own & kSegM) ? 1u : 0u);
+            if (own & 1u)
+                dst = from + lb;
+            else if (own & kSegM)
                 dst = mid;
             else
-                dst = mid + 1 + ((j - from) - before_l - (mp < j ? 1u : 0u));
+                dst = mid + 1 + ((j - from) - lb - mb);
+            if (!(sg.y - sg.x <= 1 || j == mid)) nn = j < mid ? 2 * s + 1 : 2 * s + 2;
         }
-        Bs[a][dst] = As[a][j];
+        if (dst < n) (a == 0 ? B0 : a == 1 ? B1 : B2)[dst] = rec[k];  // always true for a consistent tree
+        if (a == 0) node_out[j] = nn;
     }
-    uint32_t nn;
-    if (sg.y - sg.x <= 1 || j == mid)
-        nn = kFinal;
-    else
-        nn = j < mid ? 2 * s + 1 : 2 * s + 2;
-    node_of[j] = nn;
 }
 
 __global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
@@ -487,9 +588,9 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     };
 
     size_t rs = pm::radix_sort_scratch_bytes(n);
-    size_t need = pm::carve_size({(size_t)n * 16 * 6, (size_t)n * 4 * 3, (size_t)n * 4 * 3, (size_t)n * 4,
-                                  (size_t)cap * 8, (size_t)cap, (size_t)cap * 8, (size_t)cap * 12,
-                                  (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, (size_t)n * 48});
+    const size_t rb = (size_t)n * sizeof(Rec);
+    size_t need = pm::carve_size({rb, rb, rb, rb, rb, rb, (size_t)n * 4, (size_t)n * 4, (size_t)cap * 8, (size_t)cap,
+                                  (size_t)cap * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, (size_t)n * 48});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, need, &work);
     if (st != PM_OK) return bail(st);
@@ -498,13 +599,11 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     Rec* B[3];
     for (int a = 0; a < 3; ++a) A[a] = cv.take<Rec>(n);
     for (int a = 0; a < 3; ++a) B[a] = cv.take<Rec>(n);
-    uint32_t* flags = cv.take<uint32_t>((size_t)n * 3);
-    uint32_t* pref = cv.take<uint32_t>((size_t)n * 3);
     uint32_t* node_of = cv.take<uint32_t>(n);
+    uint32_t* node_of2 = cv.take<uint32_t>(n);
     uint2* seg = cv.take<uint2>(cap);
     int8_t* plane = cv.take<int8_t>(cap);
     uint2* med = cv.take<uint2>(cap);
-    uint32_t* mpos = cv.take<uint32_t>((size_t)cap * 3);
     uint32_t* keys = cv.take<uint32_t>(n);
     uint32_t* skeys = cv.take<uint32_t>(n);
     uint32_t* order = cv.take<uint32_t>(n);
@@ -545,12 +644,17 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(count, 256)), dim3(256), 0, s, A[0], A[1], A[2], first,
                            count, cap, seg, plane, T->node_idx, T->node_plane, med);
         if (lvl + 1 == L) break;  // every segment of the last level has size <= 1
-        hipLaunchKernelGGL(k_level_flags, dim3(nb), dim3(256), 0, s, A[0], A[1], A[2], n, node_of, plane, med, flags,
-                           mpos);
-        if ((st = pm::scan_u32(ctx, flags, pref, 3 * n, false, nullptr)) != PM_OK) return bail(st);
-        hipLaunchKernelGGL(k_level_scatter, dim3(nb), dim3(256), 0, s, A[0], A[1], A[2], B[0], B[1], B[2], n, node_of,
-                           seg, flags, pref, mpos);
+        {
+            const uint32_t tiles = pm::div_up(3ull * n, kPTile);
+            uint64_t* lstate;
+            uint32_t* lcounter;
+            uint32_t lepoch;
+            if ((st = pm::lookback_begin(ctx, tiles, &lstate, &lcounter, &lepoch)) != PM_OK) return bail(st);
+            hipLaunchKernelGGL(k_level_partition, dim3(tiles), dim3(kPThreads), 0, s, A[0], A[1], A[2], B[0], B[1],
+                               B[2], n, node_of, node_of2, seg, plane, med, lstate, lcounter, lepoch, tiles);
+        }
         for (int a = 0; a < 3; ++a) std::swap(A[a], B[a]);
+        std::swap(node_of, node_of2);
         uint32_t cf = (1u << (lvl + 1)) - 1, cc = 1u << (lvl + 1);
         hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
                            plane);

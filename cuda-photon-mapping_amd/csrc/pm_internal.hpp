@@ -83,6 +83,12 @@ static inline size_t carve_size(std::initializer_list<size_t> bytes) {
     return t + 256;
 }
 
+// Decoupled look-back state for one single-pass launch of `tiles` tiles (the
+// context's epoch-tagged 64-bit tile words {epoch:30, status:2, value:32}
+// and its ordered tile counter, which the last tile resets). One launch at a
+// time per context: launches on the context's stream never overlap.
+pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter, uint32_t* epoch);
+
 // Scan over u32 (exclusive or inclusive), optional u64 device total.
 pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total);
 

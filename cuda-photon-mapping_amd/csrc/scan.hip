@@ -158,15 +158,7 @@ __global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
 namespace pm {
 
-pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total) {
-    if (n == 0) {
-        if (total) {
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, total);
-            PM_CHECK_LAUNCH(ctx, "k_zero_u64");
-        }
-        return PM_OK;
-    }
-    uint32_t tiles = div_up(n, kTile);
+pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter, uint32_t* epoch) {
     if (tiles > ctx->scan_state_tiles) {
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->scan_state) PM_HIP(ctx, hipFree(ctx->scan_state));
@@ -183,12 +175,32 @@ pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, b
         PM_HIP(ctx, hipMemset(ctx->scan_state, 0, ctx->scan_state_tiles * sizeof(uint64_t)));
         ctx->scan_epoch = 1;
     }
+    *state = ctx->scan_state;
+    *counter = ctx->scan_counter;
+    *epoch = ctx->scan_epoch;
+    return PM_OK;
+}
+
+pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total) {
+    if (n == 0) {
+        if (total) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, total);
+            PM_CHECK_LAUNCH(ctx, "k_zero_u64");
+        }
+        return PM_OK;
+    }
+    uint32_t tiles = div_up(n, kTile);
+    uint64_t* state;
+    uint32_t* counter;
+    uint32_t epoch;
+    pm_status st = lookback_begin(ctx, tiles, &state, &counter, &epoch);
+    if (st != PM_OK) return st;
     if (inclusive)
-        hipLaunchKernelGGL(k_scan_u32<true>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n,
-                           ctx->scan_state, ctx->scan_counter, ctx->scan_epoch, tiles, total);
+        hipLaunchKernelGGL(k_scan_u32<true>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, state, counter,
+                           epoch, tiles, total);
     else
-        hipLaunchKernelGGL(k_scan_u32<false>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n,
-                           ctx->scan_state, ctx->scan_counter, ctx->scan_epoch, tiles, total);
+        hipLaunchKernelGGL(k_scan_u32<false>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, state, counter,
+                           epoch, tiles, total);
     PM_CHECK_LAUNCH(ctx, "k_scan_u32");
     return PM_OK;
 }
