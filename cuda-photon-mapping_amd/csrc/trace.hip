@@ -50,6 +50,10 @@ struct TraceParams {
     uint32_t* cnt_g;  // [n_paths]
     uint32_t* cnt_c;  // [n_paths]
     int caustic_only;  // the global map is already full: stop a path once it cannot store caustics
+    // models whose first event can be specular or refraction (Ks != 0 or
+    // opaque < 1), as a bit mask; valid when n_models <= 64
+    uint64_t spec_mask;
+    int spec_mask_valid;
 };
 
 struct Medium {
@@ -67,19 +71,12 @@ __device__ __forceinline__ bool ca_lookup(const TraceParams& P, float n1, float 
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n_paths) return;
-    const uint32_t path = (uint32_t)(P.path_base + i);
-    Stream rng;
-    rng.init(P.seed, 0u, path);
+// Emission (PhotonTracing.cu:345-370) with gpu_get_random_point_with_normal
+// (Model.cuh:202-243): roundf triangle pick, barycentrics x = u,
+// y = u' (1 - x), origin offset 1e-4 along the (unnormalised) normal.
+__device__ __forceinline__ Ray emit(const TraceParams& P, Stream& rng) {
     const DevModel& L = P.models[P.light];
     const uint32_t lt = L.t1 - L.t0;
-    const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
-
-    // emit (PhotonTracing.cu:345-370), gpu_get_random_point_with_normal
-    // (Model.cuh:202-243): roundf triangle pick, barycentrics x = u,
-    // y = u' (1 - x), origin offset 1e-4 along the (unnormalised) normal.
     int tsel = (int)roundf(0 + rng.next() * (float)((3 * (int)lt - 3) / 3));
     uint32_t t = L.t0 + (uint32_t)tsel;
     float ux = rng.next();
@@ -102,7 +99,29 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
             ray.d = mul(ray.d, -1.f);
         }
     } while (prob > dt);
+    return ray;
+}
 
+// With the global map full, a path whose first ray reaches no specular or
+// refractive model's AABB (the test scene_hit gates every model with) first
+// hits a diffuse surface or nothing: it can store no caustic photon.
+__device__ __forceinline__ bool may_store_caustic(const TraceParams& P, const Ray& ray) {
+    bool any = false;
+    for (uint64_t msk = P.spec_mask; msk; msk &= msk - 1) any |= box_hit(P.models[__builtin_ctzll(msk)].box, ray);
+    return any;
+}
+
+// A caustic-phase path that survived emission: its ray and generator state.
+struct PathState {
+    float o[3], d[3];
+    uint32_t i, blk, pos;
+    uint32_t buf[4];
+};
+
+// The bounce loop of one path (trace / destiny, PhotonTracing.cu:127-326),
+// staging its photons at path slot i.
+__device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream& rng, Ray ray) {
+    const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
     Medium ms;  // photon_trace_stack: (refractive index, model id) entries
     ms.size = 0;
     int diffuse_surfs = 0;
@@ -242,6 +261,49 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
     P.cnt_c[i] = nc;
 }
 
+__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P, PathState* __restrict__ queue,
+                                                    uint32_t* __restrict__ queue_len) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n_paths) return;
+    Stream rng;
+    rng.init(P.seed, 0u, (uint32_t)(P.path_base + i));
+    const Ray ray = emit(P, rng);
+    if (queue) {
+        // caustic phase: most paths end here; the rest continue in dense
+        // waves (k_trace_continue) instead of holding their wave
+        P.cnt_g[i] = 0;
+        P.cnt_c[i] = 0;
+        if (!may_store_caustic(P, ray)) return;
+        const uint32_t slot = atomicAdd(queue_len, 1u);
+        PathState st;
+        st.o[0] = ray.o.x, st.o[1] = ray.o.y, st.o[2] = ray.o.z;
+        st.d[0] = ray.d.x, st.d[1] = ray.d.y, st.d[2] = ray.d.z;
+        st.i = i;
+        st.blk = rng.blk;
+        st.pos = rng.pos;
+        for (int c = 0; c < 4; ++c) st.buf[c] = rng.buf[c];
+        queue[slot] = st;
+        return;
+    }
+    bounce(P, i, rng, ray);
+}
+
+__global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const PathState* __restrict__ queue,
+                                                       const uint32_t* __restrict__ queue_len) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *queue_len) return;
+    const PathState st = queue[q];
+    Stream rng;
+    rng.init(P.seed, 0u, (uint32_t)(P.path_base + st.i));
+    rng.blk = st.blk;
+    rng.pos = st.pos;
+    for (int c = 0; c < 4; ++c) rng.buf[c] = st.buf[c];
+    Ray ray;
+    ray.o = v3(st.o[0], st.o[1], st.o[2]);
+    ray.d = v3(st.d[0], st.d[1], st.d[2]);
+    bounce(P, st.i, rng, ray);
+}
+
 // Write each staged photon to its map slot: base + exclusive-scan offset of
 // its path + its rank among the path's photons of the same kind.
 __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__ stage, const uint8_t* __restrict__ kind,
@@ -320,7 +382,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     const int B = max_bounces;
     size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
                                       (size_t)nm * 32, (size_t)batch * B * 36,
-                                      (size_t)batch * B, (size_t)batch * 4 * 4, 64});
+                                      (size_t)batch * B, (size_t)batch * 4 * 4, 64,
+                                      (size_t)batch * sizeof(PathState), 16});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, rs_bytes, &work);
     if (st != PM_OK) return st;
@@ -335,6 +398,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     uint32_t* off_g = cv.take<uint32_t>(batch);
     uint32_t* off_c = cv.take<uint32_t>(batch);
     uint64_t* tot = cv.take<uint64_t>(2);
+    PathState* queue = cv.take<PathState>(batch);
+    uint32_t* queue_len = cv.take<uint32_t>(4);
 
     // model records with their AABBs (the live effect of create_kd_tree)
     if ((st = upload_models(ctx, scene, d_models, d_tmp, hm)) != PM_OK) return st;
@@ -347,6 +412,12 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     P.models = d_models;
     P.n_models = nm;
     P.bvh = bvh_nodes(scene);
+    P.spec_mask = 0;
+    P.spec_mask_valid = nm <= 64 ? 1 : 0;
+    for (int i = 0; i < nm && i < 64; ++i) {
+        const pm_model& m = scene->h_models[i];
+        if (m.opaque != 1.f || m.ks[0] != 0.f || m.ks[1] != 0.f || m.ks[2] != 0.f) P.spec_mask |= 1ull << i;
+    }
     P.light = scene->light_model;
     for (int c = 0; c < 3; ++c) P.pp[c] = scene->light_intensity[c] / (float)n_global;
     P.def_ior = scene->default_refr_index;
@@ -368,8 +439,18 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         // so paths stop at their first diffuse event: the caustic photons
         // (and the order they are stored in) are unchanged.
         P.caustic_only = got_g >= n_global ? 1 : 0;
-        hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P);
+        // In that phase most paths end at emission; the survivors go through
+        // a queue so that the bounce loop runs in dense waves.
+        const bool queued = P.caustic_only && P.spec_mask_valid;
+        if (queued && (st = pm::fill_u32(ctx, queue_len, 0u, 1)) != PM_OK) return st;
+        hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P,
+                           queued ? queue : nullptr, queue_len);
         PM_CHECK_LAUNCH(ctx, "k_trace_paths");
+        if (queued) {
+            hipLaunchKernelGGL(k_trace_continue, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P, queue,
+                               queue_len);
+            PM_CHECK_LAUNCH(ctx, "k_trace_continue");
+        }
         if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
         hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind, cnt_g,
