@@ -122,6 +122,15 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, V3 v0, V3 v1, V3 v2, float
 // are inflated by a margin far above the rounding error of tri_hit, so a
 // triangle that tri_hit accepts always lies inside every box on its path.
 constexpr int kBvhStack = 64;  // a node's key prefix grows down the tree: depth <= 64
+// The top kLdsStack entries of each lane's traversal stack live in LDS
+// (lane-interleaved, one 8-byte entry per lane per level, so a wave's
+// accesses are conflict-free whatever its lanes' depths); deeper entries,
+// rare in practice, fall back to a per-lane array.
+constexpr int kLdsStack = 16;
+constexpr int kStackLanes = 256;  // block size of every kernel that traces rays
+struct BvhStack {
+    uint2* lds;  // this lane's column of a [kLdsStack][kStackLanes] block array
+};
 
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz, V3 o, V3 inv,
                                      float tb, float& tn) {
@@ -147,9 +156,10 @@ __device__ __forceinline__ V3 inv_dir(V3 d) {
 // skipped (the caller discards them). A box is pruned only when its entry t
 // exceeds the bound by a relative 2^-16, far above the slab test's rounding.
 __device__ __forceinline__ bool bvh_hit(const float4* __restrict__ nodes, int root, const float* __restrict__ t9,
-                                        const Ray& r, V3 inv, float limit, float& mt, uint32_t& ti, V3& uvw) {
-    int stack[kBvhStack];
-    float stn[kBvhStack];
+                                        const Ray& r, V3 inv, float limit, float& mt, uint32_t& ti, V3& uvw,
+                                        BvhStack S) {
+    int stack[kBvhStack - kLdsStack];
+    float stn[kBvhStack - kLdsStack];
     int sp = 0;
     int node = root;
     bool found = false;
@@ -166,8 +176,12 @@ __device__ __forceinline__ bool bvh_hit(const float4* __restrict__ nodes, int ro
             const int c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
             if (h0 && h1) {
                 const bool first0 = t0 <= t1;
-                stack[sp] = first0 ? c1 : c0;
-                stn[sp] = first0 ? t1 : t0;
+                const int cn = first0 ? c1 : c0;
+                const float ct = first0 ? t1 : t0;
+                if (sp < kLdsStack)
+                    S.lds[sp * kStackLanes] = make_uint2((uint32_t)cn, __float_as_uint(ct));
+                else
+                    stack[sp - kLdsStack] = cn, stn[sp - kLdsStack] = ct;
                 ++sp;
                 node = first0 ? c0 : c1;
                 continue;
@@ -194,8 +208,18 @@ __device__ __forceinline__ bool bvh_hit(const float4* __restrict__ nodes, int ro
         node = 0x7FFFFFFF;
         while (sp > 0) {
             --sp;
-            if (stn[sp] <= tcut) {
-                node = stack[sp];
+            int cn;
+            float ct;
+            if (sp < kLdsStack) {
+                const uint2 e = S.lds[sp * kStackLanes];
+                cn = (int)e.x;
+                ct = __uint_as_float(e.y);
+            } else {
+                cn = stack[sp - kLdsStack];
+                ct = stn[sp - kLdsStack];
+            }
+            if (ct <= tcut) {
+                node = cn;
                 break;
             }
         }
@@ -215,7 +239,7 @@ __device__ __forceinline__ bool bvh_hit(const float4* __restrict__ nodes, int ro
 __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_models,
                                           const float* __restrict__ t9, const float* __restrict__ n9,
                                           const float4* __restrict__ bvh, const Ray& r, float& t_out, V3& n_out,
-                                          int& m_out) {
+                                          int& m_out, BvhStack S) {
     float best = 0.f;
     int bm = -1;
     V3 bn = v3(0, 0, 0);
@@ -236,7 +260,7 @@ __device__ __forceinline__ bool scene_hit(const DevModel* __restrict__ M, int n_
             }
             // a later model matters only below the current best (unless best is 0)
             const float limit = (bm >= 0 && best != 0.f) ? best : FLT_MAX;
-            if (in_box) found = bvh_hit(bvh, root, t9, r, inv, limit, mt, ti, uvw);
+            if (in_box) found = bvh_hit(bvh, root, t9, r, inv, limit, mt, ti, uvw, S);
         } else {
             const uint32_t t0 = M[m].t0, t1 = M[m].t1;
             for (uint32_t t = t0; t < t1; ++t) {

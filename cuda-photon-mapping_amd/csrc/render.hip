@@ -72,6 +72,8 @@ struct Plan {
 };
 
 __global__ __launch_bounds__(256) void k_render_rays(RaysParams P) {
+    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
+    const BvhStack S{s_stack + threadIdx.x};
     // 8x8 pixel tile per wave: coherent rays share model AABB outcomes
     const uint32_t lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int W = P.cam.w, H = P.cam.h;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_render_rays(RaysParams P) {
         // null ray that hits nothing
         Ray rr = ray;
         if (!alive) rr.d = v3(0, 0, 0);
-        bool hit = scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, rr, th, n, hmod) && alive;
+        bool hit = scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, rr, th, n, hmod, S) && alive;
         if (!hit) continue;
         // Raytracer::find_intersection: point = dir*t + origin; normal turned
         // toward the ray only inside objects

@@ -56,10 +56,33 @@ struct TraceParams {
     int spec_mask_valid;
 };
 
+// photon_trace_stack (PhotonTraceStack.cuh:37-116) as a register shift
+// stack of 16-bit model indices, the top entry in the low bits: an entry's
+// refractive index is its model's ior and its id the model's id, and only
+// the top two entries are ever read, so no per-lane array is indexed.
 struct Medium {
-    float n[kMaxBounces];
-    int id[kMaxBounces];
+    uint64_t w[4];
     int size;
+    __device__ __forceinline__ void clear() {
+        w[0] = w[1] = w[2] = w[3] = 0;
+        size = 0;
+    }
+    __device__ __forceinline__ uint32_t top() const { return (uint32_t)w[0] & 0xFFFFu; }
+    __device__ __forceinline__ uint32_t second() const { return (uint32_t)(w[0] >> 16) & 0xFFFFu; }
+    __device__ __forceinline__ void push(uint32_t m) {
+        w[3] = (w[3] << 16) | (w[2] >> 48);
+        w[2] = (w[2] << 16) | (w[1] >> 48);
+        w[1] = (w[1] << 16) | (w[0] >> 48);
+        w[0] = (w[0] << 16) | m;
+        ++size;
+    }
+    __device__ __forceinline__ void pop() {
+        w[0] = (w[0] >> 16) | (w[1] << 48);
+        w[1] = (w[1] >> 16) | (w[2] << 48);
+        w[2] = (w[2] >> 16) | (w[3] << 48);
+        w[3] >>= 16;
+        --size;
+    }
 };
 
 __device__ __forceinline__ bool ca_lookup(const TraceParams& P, float n1, float n2, float& ca) {
@@ -120,10 +143,10 @@ struct PathState {
 
 // The bounce loop of one path (trace / destiny, PhotonTracing.cu:127-326),
 // staging its photons at path slot i.
-__device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream& rng, Ray ray) {
+__device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream& rng, Ray ray, BvhStack S) {
     const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
     Medium ms;  // photon_trace_stack: (refractive index, model id) entries
-    ms.size = 0;
+    ms.clear();
     int diffuse_surfs = 0;
     uint8_t last = NONE;
     uint32_t ng = 0, nc = 0, ev = 0;
@@ -133,7 +156,7 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
         float th;
         V3 n;
         int hm;
-        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, ray, th, n, hm)) break;
+        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, ray, th, n, hm, S)) break;
         V3 ip = add(mul(ray.d, th), ray.o);
         if (dot(ray.d, n) > 0) n = mul(n, -1.f);
         const DevModel& mo = P.models[hm];
@@ -147,11 +170,11 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
             if (ms.size == 0) {
                 n1 = P.def_ior;
                 n2 = ior;
-            } else if (ms.id[ms.size - 1] == mid) {
-                n1 = ms.n[ms.size - 1];
-                n2 = ms.size == 1 ? P.def_ior : ms.n[ms.size - 2];
+            } else if (P.models[ms.top()].id == mid) {
+                n1 = P.models[ms.top()].ior;
+                n2 = ms.size == 1 ? P.def_ior : P.models[ms.second()].ior;
             } else {
-                n1 = ms.n[ms.size - 1];
+                n1 = P.models[ms.top()].ior;
                 n2 = ior;
             }
             float ca;
@@ -185,18 +208,14 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
         if (dest == REFR) {
             // get_update_refraction_pair (PhotonTracing.cu:127-151)
             float n1, n2;
-            if (ms.size != 0 && ms.id[ms.size - 1] == mid) {
-                n1 = ms.n[ms.size - 1];
-                n2 = ms.size == 1 ? P.def_ior : ms.n[ms.size - 2];
-                --ms.size;
+            if (ms.size != 0 && P.models[ms.top()].id == mid) {
+                n1 = P.models[ms.top()].ior;
+                n2 = ms.size == 1 ? P.def_ior : P.models[ms.second()].ior;
+                ms.pop();
             } else {
-                n1 = ms.size == 0 ? P.def_ior : ms.n[ms.size - 1];
+                n1 = ms.size == 0 ? P.def_ior : P.models[ms.top()].ior;
                 n2 = ior;
-                if (ms.size < P.max_b) {
-                    ms.n[ms.size] = ior;
-                    ms.id[ms.size] = mid;
-                    ++ms.size;
-                }
+                if (ms.size < P.max_b) ms.push((uint32_t)hm);
             }
             // Ray::refract (Ray.cuh:85-96)
             float eta = n1 / n2;
@@ -273,8 +292,16 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P, PathState* _
         // waves (k_trace_continue) instead of holding their wave
         P.cnt_g[i] = 0;
         P.cnt_c[i] = 0;
-        if (!may_store_caustic(P, ray)) return;
-        const uint32_t slot = atomicAdd(queue_len, 1u);
+        // one queue reservation per wave (lanes past n_paths have returned)
+        const bool keep = may_store_caustic(P, ray);
+        const uint64_t m = __ballot(keep);
+        if (!keep) return;
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(queue_len, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader, 64);
+        const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         PathState st;
         st.o[0] = ray.o.x, st.o[1] = ray.o.y, st.o[2] = ray.o.z;
         st.d[0] = ray.d.x, st.d[1] = ray.d.y, st.d[2] = ray.d.z;
@@ -285,7 +312,8 @@ __global__ __launch_bounds__(256) void k_trace_paths(TraceParams P, PathState* _
         queue[slot] = st;
         return;
     }
-    bounce(P, i, rng, ray);
+    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
+    bounce(P, i, rng, ray, BvhStack{s_stack + threadIdx.x});
 }
 
 __global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const PathState* __restrict__ queue,
@@ -301,7 +329,8 @@ __global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const Pat
     Ray ray;
     ray.o = v3(st.o[0], st.o[1], st.o[2]);
     ray.d = v3(st.d[0], st.d[1], st.d[2]);
-    bounce(P, st.i, rng, ray);
+    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
+    bounce(P, st.i, rng, ray, BvhStack{s_stack + threadIdx.x});
 }
 
 // Write each staged photon to its map slot: base + exclusive-scan offset of
@@ -341,6 +370,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: bad light model");
     if (max_bounces < 1 || max_bounces > kMaxBounces)
         return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_trace_photons: max_bounces outside [1, %d]", kMaxBounces);
+    if (scene->n_models > 65536)
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_trace_photons: more than 65536 models");
     if (n_global == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: n_global must be >= 1");
     if ((n_global && !d_global9) || (n_caustic && !d_caustic9))
         return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: null output map");
