@@ -54,6 +54,7 @@ struct TraceParams {
     // opaque < 1), as a bit mask; valid when n_models <= 64
     uint64_t spec_mask;
     int spec_mask_valid;
+    uint32_t n_segs;  // caustic phase: emission blocks (queue segments) of this batch
 };
 
 // photon_trace_stack (PhotonTraceStack.cuh:37-116) as a register shift
@@ -280,46 +281,76 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
     P.cnt_c[i] = nc;
 }
 
-__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P, PathState* __restrict__ queue,
-                                                    uint32_t* __restrict__ queue_len) {
+// Caustic phase, step 1: emission only. Most paths end here; each block of
+// kEmitBlock paths packs its survivors, in path order, at the front of its
+// own queue segment and records how many there are, so the bounce loop
+// (k_trace_continue) runs in dense waves without any global atomic.
+constexpr int kEmitBlock = 1024;
+
+__global__ __launch_bounds__(kEmitBlock) void k_trace_emit(TraceParams P, PathState* __restrict__ queue,
+                                                          uint32_t* __restrict__ seg_len) {
+    __shared__ uint32_t s_wave[kEmitBlock / 64];
+    const uint32_t i = blockIdx.x * kEmitBlock + threadIdx.x;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    bool keep = false;
+    Ray ray;
+    Stream rng;
+    if (i < P.n_paths) {
+        rng.init(P.seed, 0u, (uint32_t)(P.path_base + i));
+        ray = emit(P, rng);
+        P.cnt_g[i] = 0;
+        P.cnt_c[i] = 0;
+        keep = may_store_caustic(P, ray);
+    }
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kEmitBlock / 64; ++w) {
+        const uint32_t c = s_wave[w];
+        before += (uint32_t)w < wave ? c : 0u;
+        total += c;
+    }
+    if (threadIdx.x == 0) seg_len[blockIdx.x] = total;
+    if (!keep) return;
+    const uint32_t slot = blockIdx.x * kEmitBlock + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    PathState st;
+    st.o[0] = ray.o.x, st.o[1] = ray.o.y, st.o[2] = ray.o.z;
+    st.d[0] = ray.d.x, st.d[1] = ray.d.y, st.d[2] = ray.d.z;
+    st.i = i;
+    st.blk = rng.blk;
+    st.pos = rng.pos;
+    for (int c = 0; c < 4; ++c) st.buf[c] = rng.buf[c];
+    queue[slot] = st;
+}
+
+__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
+    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n_paths) return;
     Stream rng;
     rng.init(P.seed, 0u, (uint32_t)(P.path_base + i));
     const Ray ray = emit(P, rng);
-    if (queue) {
-        // caustic phase: most paths end here; the rest continue in dense
-        // waves (k_trace_continue) instead of holding their wave
-        P.cnt_g[i] = 0;
-        P.cnt_c[i] = 0;
-        // one queue reservation per wave (lanes past n_paths have returned)
-        const bool keep = may_store_caustic(P, ray);
-        const uint64_t m = __ballot(keep);
-        if (!keep) return;
-        const uint32_t lane = __lane_id();
-        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(queue_len, (uint32_t)__popcll(m));
-        base = __shfl(base, (int)leader, 64);
-        const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        PathState st;
-        st.o[0] = ray.o.x, st.o[1] = ray.o.y, st.o[2] = ray.o.z;
-        st.d[0] = ray.d.x, st.d[1] = ray.d.y, st.d[2] = ray.d.z;
-        st.i = i;
-        st.blk = rng.blk;
-        st.pos = rng.pos;
-        for (int c = 0; c < 4; ++c) st.buf[c] = rng.buf[c];
-        queue[slot] = st;
-        return;
-    }
-    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
     bounce(P, i, rng, ray, BvhStack{s_stack + threadIdx.x});
 }
 
 __global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const PathState* __restrict__ queue,
-                                                       const uint32_t* __restrict__ queue_len) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= *queue_len) return;
+                                                       const uint32_t* __restrict__ seg_off,
+                                                       const uint64_t* __restrict__ n_queued) {
+    // survivor r (in path order) is entry r - seg_off[s] of segment s, the
+    // last segment whose exclusive offset is <= r
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (uint32_t)*n_queued) return;
+    uint32_t lo = 0, hi = P.n_segs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (seg_off[mid] <= r)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const uint32_t q = lo * kEmitBlock + (r - seg_off[lo]);
     const PathState st = queue[q];
     Stream rng;
     rng.init(P.seed, 0u, (uint32_t)(P.path_base + st.i));
@@ -414,7 +445,9 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
                                       (size_t)nm * 32, (size_t)batch * B * 36,
                                       (size_t)batch * B, (size_t)batch * 4 * 4, 64,
-                                      (size_t)batch * sizeof(PathState), 16});
+                                      (size_t)pm::div_up(batch, kEmitBlock) * kEmitBlock * sizeof(PathState),
+                                      (size_t)pm::div_up(batch, kEmitBlock) * 4,
+                                      (size_t)pm::div_up(batch, kEmitBlock) * 4});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, rs_bytes, &work);
     if (st != PM_OK) return st;
@@ -428,9 +461,10 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     uint32_t* cnt_c = cv.take<uint32_t>(batch);
     uint32_t* off_g = cv.take<uint32_t>(batch);
     uint32_t* off_c = cv.take<uint32_t>(batch);
-    uint64_t* tot = cv.take<uint64_t>(2);
-    PathState* queue = cv.take<PathState>(batch);
-    uint32_t* queue_len = cv.take<uint32_t>(4);
+    uint64_t* tot = cv.take<uint64_t>(3);
+    PathState* queue = cv.take<PathState>((size_t)pm::div_up(batch, kEmitBlock) * kEmitBlock);
+    uint32_t* seg_len = cv.take<uint32_t>(pm::div_up(batch, kEmitBlock));
+    uint32_t* seg_off = cv.take<uint32_t>(pm::div_up(batch, kEmitBlock));
 
     // model records with their AABBs (the live effect of create_kd_tree)
     if ((st = upload_models(ctx, scene, d_models, d_tmp, hm)) != PM_OK) return st;
@@ -473,14 +507,17 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         // In that phase most paths end at emission; the survivors go through
         // a queue so that the bounce loop runs in dense waves.
         const bool queued = P.caustic_only && P.spec_mask_valid;
-        if (queued && (st = pm::fill_u32(ctx, queue_len, 0u, 1)) != PM_OK) return st;
-        hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P,
-                           queued ? queue : nullptr, queue_len);
-        PM_CHECK_LAUNCH(ctx, "k_trace_paths");
         if (queued) {
+            P.n_segs = pm::div_up(nb, kEmitBlock);
+            hipLaunchKernelGGL(k_trace_emit, dim3(P.n_segs), dim3(kEmitBlock), 0, ctx->stream, P, queue, seg_len);
+            PM_CHECK_LAUNCH(ctx, "k_trace_emit");
+            if ((st = pm::scan_u32(ctx, seg_len, seg_off, P.n_segs, false, tot + 2)) != PM_OK) return st;
             hipLaunchKernelGGL(k_trace_continue, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P, queue,
-                               queue_len);
+                               seg_off, tot + 2);
             PM_CHECK_LAUNCH(ctx, "k_trace_continue");
+        } else {
+            hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P);
+            PM_CHECK_LAUNCH(ctx, "k_trace_paths");
         }
         if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
