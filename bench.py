@@ -1,13 +1,17 @@
 """Benchmark of the photon gather hot path on one MI355X.
 
-Workload: the gather microbenchmark — 16M FP32 photons (70% uniform on the
+Workload: the gather microbenchmark -- 16M FP32 photons (70% uniform on the
 unit-cube faces, 30% in 64 Gaussian clusters on the floor), 4M queries on the
-faces, k = 64 nearest photons within r = 0.01, hash-grid cell = r. One step =
-build the photon hash grid from the photon block (count -> scan -> stable
-scatter) + the fused k-nearest gather and radiance estimate for all 4M
-queries. Inputs are resident in HBM before the timed region.
+faces, k = 64 nearest photons within r = 0.01. The hash-grid cell size
+(r/2, 2r/3 or r) and the gather variant are autotuned jointly; the result is
+the same for every choice. One step = build the photon hash grid from the
+photon block (keys -> radix sort -> cell bounds -> record gather) + the fused
+k-nearest gather and radiance estimate for all 4M queries. Inputs are
+resident in HBM before the timed region.
 
-Prints one JSON line (see the README of the harness contract in DESIGN.md).
+Prints one JSON line: metric, value (queries/s), step time, the roofline of
+the gather kernel and the CPU baseline, plus the per-frame, batched and
+served forms.
 """
 import argparse
 import json
@@ -295,6 +299,10 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the byte-counter scale reference) and exit")
+    ap.add_argument("--cell-over-r", type=float, default=0.0, help="grid cell size / r (0 = autotune)")
+    ap.add_argument("--variant", default="", help="gather variant (empty = autotune)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="after tuning, launch the tuned gather once on the cell-ordered block and exit")
     ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
@@ -338,25 +346,47 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    # Pick the gather variant (autotune): each variant's kernel alone on the
-    # cell-ordered query block, timed with HIP events on the launch stream.
-    g = ctx.grid_build(d_ph, aabb, cell_size=r)
+    # Autotune: the grid's cell size relative to r and the gather variant,
+    # jointly. Each (cell/r, variant) pair times the gather kernel alone on
+    # the cell-ordered query block with HIP events on the launch stream.
+    fracs = [args.cell_over_r] if args.cell_over_r else [0.5, 2.0 / 3.0, 1.0]
+    variants = [args.variant] if args.variant else list(pm355.KNN_VARIANTS)
+    tune, best, best_ms = {}, None, None
+    for frac in fracs:
+        gt = ctx.grid_build(d_ph, aabb, cell_size=r * frac)
+        ot = ctx.query_order(gt, d_q).long()
+        qs, ns = d_q[:, ot].contiguous(), d_n[:, ot].contiguous()
+        torch.cuda.synchronize()
+        row = {}
+        for name in variants:
+            vf = pm355.KNN_VARIANTS[name]
+            ctx.gather_radiance(gt, qs, ns, r, k, flags=vf)
+            row[name] = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k, flags=vf), 2)
+            if best_ms is None or row[name] < best_ms:
+                best, best_ms = (frac, name), row[name]
+        tune[f"{frac:.3f}"] = row
+        gt.close()
+        del qs, ns
+    frac, vname = best
+    cell = r * frac
+    vflags = pm355.KNN_VARIANTS[vname]
+    flags = (pm355.KNN_SORT_QUERIES if args.sort_queries else 0) | vflags
+    log(f"cell/r {frac:.3f}, variant {vname}: {tune}")
+    g = ctx.grid_build(d_ph, aabb, cell_size=cell)
     order = ctx.query_order(g, d_q).long()
     d_qs = d_q[:, order].contiguous()
     d_ns = d_n[:, order].contiguous()
     torch.cuda.synchronize()
-    log("autotune gather variant")
-    tune = {}
-    for name, vf in pm355.KNN_VARIANTS.items():
-        ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vf)
-        tune[name] = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vf), 2)
-    best = min(tune, key=tune.get)
-    vflags = pm355.KNN_VARIANTS[best]
-    flags = (pm355.KNN_SORT_QUERIES if args.sort_queries else 0) | vflags
-    log(f"variant {best}: {tune}")
+    if args.traffic_probe:
+        # one launch of the tuned gather on the cell-ordered block, for
+        # rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE per dispatch)
+        ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_probe": {"cell_over_r": frac, "variant": vname}}), flush=True)
+        return
 
     def step():
-        gg = ctx.grid_build(d_ph, aabb, cell_size=r)
+        gg = ctx.grid_build(d_ph, aabb, cell_size=cell)
         rgb, cnt = ctx.gather_radiance(gg, d_q, d_n, r, k, flags=flags, want_count=True)
         return gg, rgb, cnt
 
@@ -382,7 +412,7 @@ def main():
 
     # Component timing with HIP events on the stream the kernels run on.
     reps = max(3, args.steps)
-    build_ms = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r).close(), reps)
+    build_ms = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=cell).close(), reps)
     order_ms = timed(lambda: ctx.query_order(g, d_q), reps)
     gather_ms = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags), reps)
 
@@ -402,7 +432,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_hbm_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        if t.get("variant") == best:
+        if t.get("variant") == vname and abs(t.get("cell_over_r", 1.0) - frac) < 1e-3:
             traffic = t["hbm_read_bytes_per_dispatch"] + t["hbm_write_bytes_per_dispatch"]
 
     value = nq / (ms_per_step * 1e-3)
@@ -421,13 +451,15 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "gather microbenchmark: 16M photons (70% cube faces, 30% 64 floor clusters sigma=0.005), "
-                        "4M face queries, k=64, r=0.01, grid cell=r; step = grid build + fused kNN gather/radiance",
+                        f"4M face queries, k=64, r=0.01, grid cell={frac:.3f}r (autotuned); step = grid build + "
+                        "fused kNN gather/radiance",
             "photons": args.photons,
             "queries": nq,
             "k": k,
             "radius": r,
+            "cell_over_r": frac,
             "sort_queries": bool(args.sort_queries),
-            "gather_variant": best,
+            "gather_variant": vname,
         },
         "roofline": {
             "bound": "hbm",
@@ -437,8 +469,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "kernel": {"stream512_chunked": "k_knn_stream512", "stream512": "k_knn_stream512",
-                       "stream1024": "k_knn_stream1024", "two_pass": "k_knn_two_pass"}.get(best, best)
-                      + f" (variant {best}; fused gather + radiance, cell-ordered queries)",
+                       "stream1024": "k_knn_stream1024", "two_pass": "k_knn_two_pass"}.get(vname, vname)
+                      + f" (variant {vname}, cell {frac:.3f}r; fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
             "alg_bytes": alg_bytes,
         },
@@ -451,19 +483,18 @@ def main():
         "gather_only_queries_per_s": nq / (gather_ms * 1e-3),
         "box_stats": {"cells": cells, "candidates": cands, "kept": kept},
     }
-    # Cell size relative to r, reported beside the reference-compatible cell = r:
-    # the same gather (default variant, cell-ordered queries) on grids of
-    # cell r/2, 2r/3 and r.
-    cell_tune = {}
-    for frac in (0.5, 2.0 / 3.0, 1.0):
-        gt = ctx.grid_build(d_ph, aabb, cell_size=r * frac)
-        ot = ctx.query_order(gt, d_q).long()
-        qs, ns = d_q[:, ot].contiguous(), d_n[:, ot].contiguous()
-        ctx.gather_radiance(gt, qs, ns, r, k)
-        ms = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k), 3)
-        cell_tune[f"{frac:.3f}"] = {"gather_ms": ms, "queries_per_s": nq / (ms * 1e-3)}
-        gt.close()
-    res["autotune_cell_over_r"] = cell_tune
+    if frac != 1.0:
+        # the reference-compatible cell = r beside the tuned cell size
+        gr = ctx.grid_build(d_ph, aabb, cell_size=r)
+        orr = ctx.query_order(gr, d_q).long()
+        qs, ns = d_q[:, orr].contiguous(), d_n[:, orr].contiguous()
+        ctx.gather_radiance(gr, qs, ns, r, k, flags=vflags)
+        ms_r = timed(lambda: ctx.gather_radiance(gr, qs, ns, r, k, flags=vflags), 3)
+        b_r = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r).close(), 3)
+        res["cell_equals_r"] = {"gather_ms": ms_r, "grid_build_ms": b_r, "step_queries_per_s_estimate":
+                                nq / ((ms_r + b_r + order_ms) * 1e-3)}
+        gr.close()
+        del qs, ns
 
     # kd-tree build over the same 16M photons (PhotonMap), HIP events
     kd_ms = timed(lambda: ctx.kdtree_build(d_ph).close(), 3)

@@ -385,9 +385,14 @@ struct Stream {
         ++blk;
         pos = 0;
     }
+    // buf is a shift register: the next word is always buf[0], so no lane
+    // state is ever indexed dynamically
     __device__ __forceinline__ float next() {
         if (pos == 4) refill();
-        uint32_t x = pos == 0 ? buf[0] : pos == 1 ? buf[1] : pos == 2 ? buf[2] : buf[3];
+        const uint32_t x = buf[0];
+        buf[0] = buf[1];
+        buf[1] = buf[2];
+        buf[2] = buf[3];
         ++pos;
         return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
     }

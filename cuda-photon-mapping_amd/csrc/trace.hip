@@ -142,22 +142,43 @@ struct PathState {
     uint32_t buf[4];
 };
 
-// The bounce loop of one path (trace / destiny, PhotonTracing.cu:127-326),
-// staging its photons at path slot i.
-__device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream& rng, Ray ray, BvhStack S) {
-    const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
+// The state of one light path between bounces.
+struct Live {
+    Ray ray;
+    Stream rng;
     Medium ms;  // photon_trace_stack: (refractive index, model id) entries
-    ms.clear();
-    int diffuse_surfs = 0;
-    uint8_t last = NONE;
-    uint32_t ng = 0, nc = 0, ev = 0;
+    int diffuse_surfs;
+    int b;  // bounces done
+    uint32_t i;  // path slot in the batch (staging column)
+    uint32_t ng, nc, ev;
+    uint8_t last;
+    __device__ __forceinline__ void start(uint32_t slot, const Stream& r, const Ray& ry) {
+        ray = ry;
+        rng = r;
+        ms.clear();
+        diffuse_surfs = 0;
+        b = 0;
+        i = slot;
+        ng = nc = ev = 0;
+        last = NONE;
+    }
+};
 
-    for (int b = 0; b < P.max_b; ++b) {
+// One bounce of a path (trace / destiny, PhotonTracing.cu:127-326), staging
+// its photon at path slot L.i. Returns false once the path has ended; the
+// caller then stores the path's photon counts.
+__device__ __forceinline__ bool bounce_step(const TraceParams& P, Live& L, BvhStack S) {
+    const V3 pp = v3(P.pp[0], P.pp[1], P.pp[2]);
+    Ray& ray = L.ray;
+    Stream& rng = L.rng;
+    Medium& ms = L.ms;
+    const uint32_t i = L.i;
+    {
         // trace (PhotonTracing.cu:278-326)
         float th;
         V3 n;
         int hm;
-        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, ray, th, n, hm, S)) break;
+        if (!scene_hit(P.models, P.n_models, P.tris, P.nrms, P.bvh, ray, th, n, hm, S)) return false;
         V3 ip = add(mul(ray.d, th), ray.o);
         if (dot(ray.d, n) > 0) n = mul(n, -1.f);
         const DevModel& mo = P.models[hm];
@@ -233,17 +254,17 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
             bool ks_one = fabsf(mo.ks[0] - 1.f) < 1e-3f && fabsf(mo.ks[1] - 1.f) < 1e-3f && fabsf(mo.ks[2] - 1.f) < 1e-3f;
             if (dest == DIF || (dest == ABSORB && !ks_one)) {
                 // PathOperator::response (PathOperator.cuh:28-37)
-                const bool caustic = (last == SPEC || last == REFR) && diffuse_surfs == 0;
+                const bool caustic = (L.last == SPEC || L.last == REFR) && L.diffuse_surfs == 0;
                 if (caustic || !P.caustic_only) {
                     const float v[9] = {ip.x, ip.y, ip.z, pp.x, pp.y, pp.z, ray.d.x, ray.d.y, ray.d.z};
 #pragma unroll
-                    for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + ev) * P.n_paths + i] = v[c];
-                    P.kind[(size_t)ev * P.n_paths + i] = caustic ? 1 : 0;
-                    ++ev;
+                    for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + L.ev) * P.n_paths + i] = v[c];
+                    P.kind[(size_t)L.ev * P.n_paths + i] = caustic ? 1 : 0;
+                    ++L.ev;
                     if (caustic)
-                        ++nc;
+                        ++L.nc;
                     else
-                        ++ng;
+                        ++L.ng;
                 }
             } else if (dest == SPEC) {
                 // Ray::reflect (Ray.cuh:64-71)
@@ -269,16 +290,15 @@ __device__ __forceinline__ void bounce(const TraceParams& P, uint32_t i, Stream&
                 nr.o = add(ip, mul(nr.d, 0.0001f));
             }
         }
-        if (dest == DIF) ++diffuse_surfs;
-        last = dest;
-        if (dest == ABSORB || end) break;
+        if (dest == DIF) ++L.diffuse_surfs;
+        L.last = dest;
+        if (dest == ABSORB || end) return false;
         // after a diffuse event no later photon of this path is caustic
         // (PathOperator.cuh:28-37); with the global map full it stores nothing more
-        if (P.caustic_only && diffuse_surfs > 0) break;
+        if (P.caustic_only && L.diffuse_surfs > 0) return false;
         ray = nr;
     }
-    P.cnt_g[i] = ng;
-    P.cnt_c[i] = nc;
+    return ++L.b < P.max_b;
 }
 
 // Caustic phase, step 1: emission only. Most paths end here; each block of
@@ -325,23 +345,11 @@ __global__ __launch_bounds__(kEmitBlock) void k_trace_emit(TraceParams P, PathSt
     queue[slot] = st;
 }
 
-__global__ __launch_bounds__(256) void k_trace_paths(TraceParams P) {
-    __shared__ uint2 s_stack[kLdsStack * kStackLanes];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n_paths) return;
-    Stream rng;
-    rng.init(P.seed, 0u, (uint32_t)(P.path_base + i));
-    const Ray ray = emit(P, rng);
-    bounce(P, i, rng, ray, BvhStack{s_stack + threadIdx.x});
-}
-
-__global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const PathState* __restrict__ queue,
-                                                       const uint32_t* __restrict__ seg_off,
-                                                       const uint64_t* __restrict__ n_queued) {
-    // survivor r (in path order) is entry r - seg_off[s] of segment s, the
-    // last segment whose exclusive offset is <= r
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= (uint32_t)*n_queued) return;
+// Path r of the caustic-phase survivor list (in path order): entry
+// r - seg_off[s] of queue segment s, the last segment whose exclusive offset
+// is <= r.
+__device__ __forceinline__ void queued_path(const TraceParams& P, const PathState* __restrict__ queue,
+                                            const uint32_t* __restrict__ seg_off, uint32_t r, Live& L) {
     uint32_t lo = 0, hi = P.n_segs - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
@@ -350,8 +358,7 @@ __global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const Pat
         else
             hi = mid - 1;
     }
-    const uint32_t q = lo * kEmitBlock + (r - seg_off[lo]);
-    const PathState st = queue[q];
+    const PathState st = queue[lo * kEmitBlock + (r - seg_off[lo])];
     Stream rng;
     rng.init(P.seed, 0u, (uint32_t)(P.path_base + st.i));
     rng.blk = st.blk;
@@ -360,8 +367,63 @@ __global__ __launch_bounds__(256) void k_trace_continue(TraceParams P, const Pat
     Ray ray;
     ray.o = v3(st.o[0], st.o[1], st.o[2]);
     ray.d = v3(st.d[0], st.d[1], st.d[2]);
+    L.start(st.i, rng, ray);
+}
+
+// Persistent path tracing. Block b owns a contiguous range of the batch's
+// paths (global phase: path slots; caustic phase: survivor list entries).
+// Each lane traces one path a bounce at a time and, when it ends, takes the
+// block's next path from an LDS counter, so a wave keeps its lanes busy
+// instead of waiting on its longest path. Which lane traces a path does not
+// change the photons it stores.
+template <bool kQueued>
+__global__ __launch_bounds__(256) void k_trace_pool(TraceParams P, const PathState* __restrict__ queue,
+                                                   const uint32_t* __restrict__ seg_off,
+                                                   const uint64_t* __restrict__ n_queued) {
     __shared__ uint2 s_stack[kLdsStack * kStackLanes];
-    bounce(P, st.i, rng, ray, BvhStack{s_stack + threadIdx.x});
+    __shared__ uint32_t s_next;
+    const BvhStack S{s_stack + threadIdx.x};
+    const uint32_t total = kQueued ? (uint32_t)*n_queued : P.n_paths;
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = min(blockIdx.x * per, total), hi = min(lo + per, total);
+    if (threadIdx.x == 0) s_next = lo;
+    __syncthreads();
+    const uint32_t lane = __lane_id();
+    Live L;
+    bool alive = false, done = false;
+    for (;;) {
+        const bool need = !alive && !done;
+        const uint64_t m = __ballot(need);
+        if (m) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&s_next, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (need) {
+                const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < hi) {
+                    if (kQueued) {
+                        queued_path(P, queue, seg_off, idx, L);
+                    } else {
+                        Stream rng;
+                        rng.init(P.seed, 0u, (uint32_t)(P.path_base + idx));
+                        const Ray ray = emit(P, rng);
+                        L.start(idx, rng, ray);
+                    }
+                    alive = true;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        // every lane is now tracing a path or out of work for good
+        if (!__ballot(alive)) break;
+        if (alive && !bounce_step(P, L, S)) {
+            P.cnt_g[L.i] = L.ng;
+            P.cnt_c[L.i] = L.nc;
+            alive = false;
+        }
+    }
 }
 
 // Write each staged photon to its map slot: base + exclusive-scan offset of
@@ -495,6 +557,11 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     P.cnt_g = cnt_g;
     P.cnt_c = cnt_c;
 
+    // resident blocks of the persistent tracer: one full round over the CUs
+    int per_cu = 0;
+    PM_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_pool<false>, 256, 0));
+    const uint32_t pool_blocks = (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->cu_count;
+
     uint64_t paths = 0, got_g = 0, got_c = 0;
     while ((got_g < n_global || got_c < n_caustic) && paths < max_paths) {
         uint32_t nb = (uint32_t)std::min<uint64_t>(batch, max_paths - paths);
@@ -512,12 +579,13 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
             hipLaunchKernelGGL(k_trace_emit, dim3(P.n_segs), dim3(kEmitBlock), 0, ctx->stream, P, queue, seg_len);
             PM_CHECK_LAUNCH(ctx, "k_trace_emit");
             if ((st = pm::scan_u32(ctx, seg_len, seg_off, P.n_segs, false, tot + 2)) != PM_OK) return st;
-            hipLaunchKernelGGL(k_trace_continue, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P, queue,
-                               seg_off, tot + 2);
-            PM_CHECK_LAUNCH(ctx, "k_trace_continue");
+            hipLaunchKernelGGL(k_trace_pool<true>, dim3(std::min(pm::div_up(nb, 256), pool_blocks)), dim3(256), 0,
+                               ctx->stream, P, queue, seg_off, tot + 2);
+            PM_CHECK_LAUNCH(ctx, "k_trace_pool<queued>");
         } else {
-            hipLaunchKernelGGL(k_trace_paths, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, P);
-            PM_CHECK_LAUNCH(ctx, "k_trace_paths");
+            hipLaunchKernelGGL(k_trace_pool<false>, dim3(std::min(pm::div_up(nb, 256), pool_blocks)), dim3(256), 0,
+                               ctx->stream, P, nullptr, nullptr, nullptr);
+            PM_CHECK_LAUNCH(ctx, "k_trace_pool");
         }
         if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
