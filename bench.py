@@ -157,6 +157,15 @@ def frame_bench(ctx, n_frames):
         stages.append(sm)
         stored = st[1] + st[2]
     stages = np.median(np.array(stages), 0)
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "cornell_c2_trace_valu.json")
+    if os.path.exists(vpath):
+        # FP32 VALU operations of one C2 photon trace, counted by rocprofv3
+        # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32 x 64 lanes, FMA x 2)
+        v = json.load(open(vpath))
+        tflops = v["fp32_flop_per_trace"] / (stages[0] * 1e-3) / 1e12
+        valu = {"fp32_tflops": tflops, "peak_tflops": 157.3, "frac": tflops / 157.3,
+                "source": "profiles/cornell_c2_trace_valu.json"}
     return {
         "workload": "Cornell box (10 wall + 2x960 sphere + 2 light triangles, per-model BVHs), 2M global + 500k "
                     "caustic photons (<=10 bounces), grids cell=r and kd-trees built per frame, 1280x720, "
@@ -165,8 +174,11 @@ def frame_bench(ctx, n_frames):
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "photons_traced_per_s": stored / (stages[0] * 1e-3),
+        # tracing writes 36 B per stored photon (position, power, direction)
+        "trace_hbm_frac": 36 * stored / (stages[0] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "stage_ms": {"trace": stages[0], "grid_builds": stages[1], "kdtree_builds": stages[2],
                      "render": stages[3]},
+        "trace_valu": valu,
     }
 
 
@@ -206,8 +218,21 @@ def batch_bench(ctx, n_jobs, reps=2):
     def batched_queue():
         ctx.gather_batch(jobs, flags=pm355.KNN_SORT_QUERIES | pm355.KNN_BATCH_QUEUE, want_count=False)
 
+    # the small jobs alone (128x128 and 256x256 frames): the launch-bound
+    # regime, where one launch for many jobs pays most
+    small = [j for j in jobs if j[1].shape[1] <= 256 * 256]
+    nq_small = sum(j[1].shape[1] for j in small)
+
+    def plain_small():
+        for g, q, qn, r, k in small:
+            ctx.gather_radiance(g, q, qn, r, k, flags=plain_flags)
+
+    def batched_small():
+        ctx.gather_batch(small, flags=pm355.KNN_SORT_QUERIES, want_count=False)
+
     res = {}
-    for name, fn in (("plain", plain), ("batched", batched), ("batched_queue", batched_queue)):
+    for name, fn in (("plain", plain), ("batched", batched), ("batched_queue", batched_queue),
+                     ("plain_small", plain_small), ("batched_small", batched_small)):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -227,6 +252,9 @@ def batch_bench(ctx, n_jobs, reps=2):
         "plain_queries_per_s": nq_total / res["plain"],
         "batched_queries_per_s": nq_total / res["batched"],
         "speedup": res["plain"] / res["batched"],
+        "small_jobs": {"jobs": len(small), "queries": nq_small, "plain_ms": res["plain_small"] * 1e3,
+                       "batched_ms": res["batched_small"] * 1e3,
+                       "speedup": res["plain_small"] / res["batched_small"]},
     }
 
 
@@ -301,6 +329,8 @@ def main():
                     help="only copy 1 GiB on the device 3 times (the byte-counter scale reference) and exit")
     ap.add_argument("--cell-over-r", type=float, default=0.0, help="grid cell size / r (0 = autotune)")
     ap.add_argument("--variant", default="", help="gather variant (empty = autotune)")
+    ap.add_argument("--trace-probe", action="store_true",
+                    help="only trace the Cornell-box photon maps twice (for rocprofv3 counter runs) and exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after tuning, launch the tuned gather once on the cell-ordered block and exit")
     ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
@@ -325,6 +355,21 @@ def main():
 
     import pm355
     from pm355.workloads import gather_microbench
+
+    if args.trace_probe:
+        # two C2 photon traces (2M global + 500k caustic, Cornell box with
+        # BVHs) and nothing else, for rocprofv3 counter runs
+        from pm355.scenes import cornell_box
+
+        ctx = pm355.Context(0)
+        ds = ctx.upload_scene(cornell_box(), accel=True)
+        for seed in (1, 2):
+            g, c, st = ctx.trace_photons(ds, seed, 2_000_000, 500_000, 10, max_paths=200_000_000)
+        torch.cuda.synchronize()
+        print(json.dumps({"trace_probe": {"paths": int(st[0]), "global": int(st[1]), "caustic": int(st[2])}}),
+              flush=True)
+        ctx.close()
+        return
 
     log("generating workload")
     ph, q, qn, aabb = gather_microbench(seed=args.seed, n_photons=args.photons, n_queries=args.queries)
@@ -499,6 +544,34 @@ def main():
     # kd-tree build over the same 16M photons (PhotonMap), HIP events
     kd_ms = timed(lambda: ctx.kdtree_build(d_ph).close(), 3)
     res["components_ms"]["kdtree_build_16M"] = kd_ms
+
+    # Measured copy peak (a STREAM-like device copy of 1 GiB: bytes read +
+    # written per second) beside the published 8 TB/s, and the build
+    # rooflines. Algorithmic bytes: grid build N*(12 + 36 + 36) + 12*G
+    # (position read for the key; record read + SoA write; starts, sizes and
+    # the cell-bounds pass per cell); kd-tree build N*L*12 + N*72 with
+    # L = ceil(log2(N + 1)) levels (split key, index read and write per
+    # level; the final record permutation).
+    a = torch.empty(256 * 1024 * 1024, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    copy_ms = timed(lambda: b.copy_(a), 5)
+    copy_gbs = 2 * a.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del a, b
+    n_ph = d_ph.shape[1]
+    n_cells = g.info.n_cells
+    grid_bytes = n_ph * (12 + 36 + 36) + 12 * n_cells
+    levels = int(np.ceil(np.log2(n_ph + 1)))
+    kd_bytes = n_ph * levels * 12 + n_ph * 72
+    res["copy_peak_GBs"] = copy_gbs
+    res["build_roofline"] = {
+        "grid_build": {"ms": build_ms, "alg_bytes": grid_bytes, "cells": n_cells,
+                       "achieved_GBs": grid_bytes / (build_ms * 1e-3) / 1e9,
+                       "frac": grid_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kdtree_build_16M": {"ms": kd_ms, "alg_bytes": kd_bytes, "levels": levels,
+                             "achieved_GBs": kd_bytes / (kd_ms * 1e-3) / 1e9,
+                             "frac": kd_bytes / (kd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
     g.close()
     if args.frames > 0:
         log(f"per-frame form: {args.frames} Cornell-box frames")
