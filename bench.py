@@ -3,7 +3,7 @@
 Workload: the gather microbenchmark -- 16M FP32 photons (70% uniform on the
 unit-cube faces, 30% in 64 Gaussian clusters on the floor), 4M queries on the
 faces, k = 64 nearest photons within r = 0.01. The hash-grid cell size
-(r/2, 2r/3 or r) and the gather variant are autotuned jointly; the result is
+(r/4, r/3, r/2, 2r/3 or r) and the gather variant are autotuned jointly; the result is
 the same for every choice. One step = build the photon hash grid from the
 photon block (keys -> radix sort -> cell bounds -> record gather) + the fused
 k-nearest gather and radiance estimate for all 4M queries. Inputs are
@@ -394,7 +394,7 @@ def main():
     # Autotune: the grid's cell size relative to r and the gather variant,
     # jointly. Each (cell/r, variant) pair times the gather kernel alone on
     # the cell-ordered query block with HIP events on the launch stream.
-    fracs = [args.cell_over_r] if args.cell_over_r else [0.5, 2.0 / 3.0, 1.0]
+    fracs = [args.cell_over_r] if args.cell_over_r else [0.25, 1.0 / 3.0, 0.5, 2.0 / 3.0, 1.0]
     variants = [args.variant] if args.variant else list(pm355.KNN_VARIANTS)
     tune, best, best_ms = {}, None, None
     for frac in fracs:
