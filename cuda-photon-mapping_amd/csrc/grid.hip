@@ -52,30 +52,40 @@ __global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9
 
 // starts[c] = first sorted position with key >= c (the exclusive scan of the
 // counts, PhotonGrid.cu:195-196), sizes[c] = starts[c+1] - starts[c].
-__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ skeys, uint32_t n,
-                                                    uint32_t n_cells, uint32_t* __restrict__ starts,
-                                                    uint32_t* __restrict__ sizes) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cells) return;
-    uint32_t lo = 0, hi = n;
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                    uint32_t v) {
     while (lo < hi) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (skeys[mid] < c)
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < v)
             lo = mid + 1;
         else
             hi = mid;
     }
-    uint32_t end = lo;
-    hi = n;
-    while (end < hi) {
-        uint32_t mid = end + ((hi - end) >> 1);
-        if (skeys[mid] <= c)
-            end = mid + 1;
-        else
-            hi = mid;
+    return lo;
+}
+
+// starts[c] = first sorted position with key >= c (lower bound), sizes[c] =
+// starts[c + 1] - starts[c]. A block of 256 cells first brackets its keys
+// with two full searches, then each cell searches only inside the bracket.
+__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ skeys, uint32_t n,
+                                                    uint32_t n_cells, uint32_t* __restrict__ starts,
+                                                    uint32_t* __restrict__ sizes) {
+    __shared__ uint32_t s_lb[257];
+    const uint32_t t = threadIdx.x, c0 = blockIdx.x * 256;
+    if (t < 2) {
+        const uint32_t v = c0 + 256 * t;
+        s_lb[256 * t] = v >= n_cells ? n : lower_bound_u32(skeys, 0, n, v);
     }
-    starts[c] = lo;
-    sizes[c] = end - lo;
+    __syncthreads();
+    const uint32_t lo = s_lb[0], hi = s_lb[256];
+    const uint32_t c = c0 + t;
+    const uint32_t lb = c < n_cells ? lower_bound_u32(skeys, lo, hi, c) : n;
+    if (t > 0) s_lb[t] = lb;
+    __syncthreads();
+    if (c < n_cells) {
+        starts[c] = lb;
+        sizes[c] = s_lb[t + 1] - lb;
+    }
 }
 
 // analyze_density_kernel's max (PhotonGrid.cu:70-90), as a reduction.
