@@ -126,7 +126,7 @@ constexpr int kBvhStack = 64;  // a node's key prefix grows down the tree: depth
 // (lane-interleaved, one 8-byte entry per lane per level, so a wave's
 // accesses are conflict-free whatever its lanes' depths); deeper entries,
 // rare in practice, fall back to a per-lane array.
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 12;
 constexpr int kStackLanes = 256;  // block size of every kernel that traces rays
 struct BvhStack {
     uint2* lds;  // this lane's column of a [kLdsStack][kStackLanes] block array

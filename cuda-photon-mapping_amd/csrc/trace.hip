@@ -377,7 +377,7 @@ __device__ __forceinline__ void queued_path(const TraceParams& P, const PathStat
 // instead of waiting on its longest path. Which lane traces a path does not
 // change the photons it stores.
 template <bool kQueued>
-__global__ __launch_bounds__(256) void k_trace_pool(TraceParams P, const PathState* __restrict__ queue,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_pool(TraceParams P, const PathState* __restrict__ queue,
                                                    const uint32_t* __restrict__ seg_off,
                                                    const uint64_t* __restrict__ n_queued) {
     __shared__ uint2 s_stack[kLdsStack * kStackLanes];
