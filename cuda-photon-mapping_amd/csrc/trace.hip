@@ -45,8 +45,9 @@ struct TraceParams {
     int max_b;
     uint64_t path_base;
     uint32_t n_paths;
-    float* stage;     // [9][max_b][n_paths]
-    uint8_t* kind;    // [max_b][n_paths]: 0 global, 1 caustic
+    float* stage;     // [9][stage_depth][n_paths]
+    uint8_t* kind;    // [stage_depth][n_paths]: 0 global, 1 caustic
+    int stage_depth;  // photon slots per path: max_b, or 1 once the global map is full
     uint32_t* cnt_g;  // [n_paths]
     uint32_t* cnt_c;  // [n_paths]
     int caustic_only;  // the global map is already full: stop a path once it cannot store caustics
@@ -255,10 +256,10 @@ __device__ __forceinline__ bool bounce_step(const TraceParams& P, Live& L, BvhSt
             if (dest == DIF || (dest == ABSORB && !ks_one)) {
                 // PathOperator::response (PathOperator.cuh:28-37)
                 const bool caustic = (L.last == SPEC || L.last == REFR) && L.diffuse_surfs == 0;
-                if (caustic || !P.caustic_only) {
+                if ((caustic || !P.caustic_only) && L.ev < (uint32_t)P.stage_depth) {
                     const float v[9] = {ip.x, ip.y, ip.z, pp.x, pp.y, pp.z, ray.d.x, ray.d.y, ray.d.z};
 #pragma unroll
-                    for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.max_b + L.ev) * P.n_paths + i] = v[c];
+                    for (int c = 0; c < 9; ++c) P.stage[((size_t)c * P.stage_depth + L.ev) * P.n_paths + i] = v[c];
                     P.kind[(size_t)L.ev * P.n_paths + i] = caustic ? 1 : 0;
                     ++L.ev;
                     if (caustic)
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__
                                                       const uint32_t* __restrict__ cnt_g,
                                                       const uint32_t* __restrict__ cnt_c,
                                                       const uint32_t* __restrict__ off_g,
-                                                      const uint32_t* __restrict__ off_c, uint32_t n_paths, int max_b,
+                                                      const uint32_t* __restrict__ off_c, uint32_t n_paths, int depth,
                                                       uint64_t base_g, uint64_t base_c, float* __restrict__ out_g,
                                                       uint32_t cap_g, float* __restrict__ out_c, uint32_t cap_c) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__
         float* out = c ? out_c : out_g;
         if (s < cap) {
 #pragma unroll
-            for (int p = 0; p < 9; ++p) out[(size_t)p * cap + s] = stage[((size_t)p * max_b + e) * n_paths + i];
+            for (int p = 0; p < 9; ++p) out[(size_t)p * cap + s] = stage[((size_t)p * depth + e) * n_paths + i];
         }
     }
 }
@@ -500,16 +501,23 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         add_pair(hm[i].ior, scene->default_refr_index);
     }
 
-    uint64_t want = (uint64_t)n_global + n_caustic;
-    uint32_t batch = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 65536), 4u << 20);
-    if (max_paths < batch) batch = (uint32_t)std::max<uint64_t>(max_paths, 1);
+    // Global phase: batches of up to 4M paths, each staging up to
+    // max_bounces photons. Caustic phase (global map full): a path stores at
+    // most one photon (its first diffuse event, or an absorption, ends it),
+    // so batches of up to 16M paths stage one slot each.
+    constexpr uint32_t kGlobalBatch = 4u << 20, kCausticBatch = 16u << 20;
     const int B = max_bounces;
+    const uint64_t want = (uint64_t)n_global + n_caustic;
+    const uint32_t batch_g = (uint32_t)std::max<uint64_t>(
+        std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(want, 65536), kGlobalBatch), max_paths), 1);
+    const uint32_t batch_c = (uint32_t)std::max<uint64_t>(std::min<uint64_t>(kCausticBatch, max_paths), 1);
+    const uint32_t npath = std::max(batch_g, batch_c);
+    const size_t slots = std::max((size_t)batch_g * B, (size_t)batch_c);
+    const uint32_t nseg = pm::div_up(batch_c, kEmitBlock);
     size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
-                                      (size_t)nm * 32, (size_t)batch * B * 36,
-                                      (size_t)batch * B, (size_t)batch * 4 * 4, 64,
-                                      (size_t)pm::div_up(batch, kEmitBlock) * kEmitBlock * sizeof(PathState),
-                                      (size_t)pm::div_up(batch, kEmitBlock) * 4,
-                                      (size_t)pm::div_up(batch, kEmitBlock) * 4});
+                                      (size_t)nm * 32, slots * 36, slots, (size_t)npath * 4 * 4, 64,
+                                      (size_t)nseg * kEmitBlock * sizeof(PathState), (size_t)nseg * 4,
+                                      (size_t)nseg * 4});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, rs_bytes, &work);
     if (st != PM_OK) return st;
@@ -517,16 +525,16 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     DevModel* d_models = cv.take<DevModel>(nm);
     CaEntry* d_ca = cv.take<CaEntry>(ca.size() + 1);
     void* d_tmp = cv.take<char>(32 * (size_t)nm);
-    float* stage = cv.take<float>((size_t)batch * B * 9);
-    uint8_t* kind = cv.take<uint8_t>((size_t)batch * B);
-    uint32_t* cnt_g = cv.take<uint32_t>(batch);
-    uint32_t* cnt_c = cv.take<uint32_t>(batch);
-    uint32_t* off_g = cv.take<uint32_t>(batch);
-    uint32_t* off_c = cv.take<uint32_t>(batch);
+    float* stage = cv.take<float>(slots * 9);
+    uint8_t* kind = cv.take<uint8_t>(slots);
+    uint32_t* cnt_g = cv.take<uint32_t>(npath);
+    uint32_t* cnt_c = cv.take<uint32_t>(npath);
+    uint32_t* off_g = cv.take<uint32_t>(npath);
+    uint32_t* off_c = cv.take<uint32_t>(npath);
     uint64_t* tot = cv.take<uint64_t>(3);
-    PathState* queue = cv.take<PathState>((size_t)pm::div_up(batch, kEmitBlock) * kEmitBlock);
-    uint32_t* seg_len = cv.take<uint32_t>(pm::div_up(batch, kEmitBlock));
-    uint32_t* seg_off = cv.take<uint32_t>(pm::div_up(batch, kEmitBlock));
+    PathState* queue = cv.take<PathState>((size_t)nseg * kEmitBlock);
+    uint32_t* seg_len = cv.take<uint32_t>(nseg);
+    uint32_t* seg_off = cv.take<uint32_t>(nseg);
 
     // model records with their AABBs (the live effect of create_kd_tree)
     if ((st = upload_models(ctx, scene, d_models, d_tmp, hm)) != PM_OK) return st;
@@ -563,14 +571,17 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     const uint32_t pool_blocks = (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->cu_count;
 
     uint64_t paths = 0, got_g = 0, got_c = 0;
+    uint32_t batch = batch_g;
     while ((got_g < n_global || got_c < n_caustic) && paths < max_paths) {
-        uint32_t nb = (uint32_t)std::min<uint64_t>(batch, max_paths - paths);
-        P.path_base = paths;
-        P.n_paths = nb;
         // Once the global map is full, its later photons are dropped anyway,
         // so paths stop at their first diffuse event: the caustic photons
         // (and the order they are stored in) are unchanged.
         P.caustic_only = got_g >= n_global ? 1 : 0;
+        P.stage_depth = P.caustic_only ? 1 : B;
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(std::min(batch, P.caustic_only ? batch_c : batch_g),
+                                                         max_paths - paths);
+        P.path_base = paths;
+        P.n_paths = nb;
         // In that phase most paths end at emission; the survivors go through
         // a queue so that the bounce loop runs in dense waves.
         const bool queued = P.caustic_only && P.spec_mask_valid;
@@ -590,7 +601,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
         hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind, cnt_g,
-                           cnt_c, off_g, off_c, nb, B, got_g, got_c, d_global9, n_global, d_caustic9, n_caustic);
+                           cnt_c, off_g, off_c, nb, P.stage_depth, got_g, got_c, d_global9, n_global, d_caustic9,
+                           n_caustic);
         PM_CHECK_LAUNCH(ctx, "k_trace_compact");
         uint64_t h[2];
         PM_HIP(ctx, hipMemcpyAsync(h, tot, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -603,7 +615,10 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         double need = 0;
         if (got_g < n_global) need = std::max(need, (n_global - got_g) / std::max(rate_g, 1e-9));
         if (got_c < n_caustic) need = std::max(need, (n_caustic - got_c) / std::max(rate_c, 1e-9));
-        batch = (uint32_t)std::min<double>(std::max<double>(need * 1.05 + 1024, 65536), (double)batch);
+        // (the yield estimate's relative error is ~1/sqrt(photons so far),
+        // well under the 2% margin; a short batch only costs one more)
+        const uint32_t cap = got_g >= n_global ? batch_c : batch_g;
+        batch = (uint32_t)std::min<double>(std::max<double>(need * 1.02 + 1024, 65536), (double)cap);
     }
     if (stats) {
         stats->paths = paths;
