@@ -501,7 +501,13 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
     const uint32_t gflags = R.gather_flags | PM_KNN_SORT_QUERIES;
     const uint32_t nq = F->nq;
     F->caustic = caustic_grid != nullptr;
-    if (nq > 0) {
+    if (nq > 0 && !quirk_mode(R) && R.k_global <= 256 && (!caustic_grid || R.k_caustic <= 256)) {
+        // both gathers in one cell-ordered launch (one tail instead of two)
+        pm_gather_job jobs[2];
+        uint32_t nj = 0;
+        if ((st = pm_frame_gather_jobs(F, global_grid, caustic_grid, jobs, &nj)) != PM_OK) return st;
+        if ((st = pm_gather_batch(ctx, jobs, nj, gflags)) != PM_OK) return st;
+    } else if (nq > 0) {
         if ((st = pm_grid_gather_radiance(ctx, global_grid, F->q3, F->n3, nq, R.r_global, R.k_global, gflags, F->lg,
                                           nullptr)) != PM_OK)
             return st;
