@@ -92,16 +92,16 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
 
     auto feed = [&](bool valid, uint32_t pid, float d2) {
         bool in = valid && d2 <= T;
-        found += (uint32_t)__popcll(__ballot(in));
+        found += (uint32_t)__popcll(wballot(in));
         uint64_t key = make_key(d2, pid);
         bool acc = in && key < khi;
-        uint64_t bal = __ballot(acc);
+        uint64_t bal = wballot(acc);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)kCap) {
             wave_sync();
             cnt = buf_select<kCap>(L.sel, lane, cnt, k, kCap / 2, P.bT, khi);
             acc = acc && key < khi;
-            bal = __ballot(acc);
+            bal = wballot(acc);
             na = (uint32_t)__popcll(bal);
         }
         if (acc) L.sel.buf[cnt + mbcnt(bal)] = key;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
         float d2 = 0.f;
         visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
         feed(valid, pid, d2);
-        uint64_t bl = __ballot(gl), br = __ballot(gr);
+        uint64_t bl = wballot(gl), br = wballot(gr);
         uint32_t nl = (uint32_t)__popcll(bl);
         wave_sync();
         if (gl) L.frontier[cur ^ 1][mbcnt(bl)] = 2 * node + 1;

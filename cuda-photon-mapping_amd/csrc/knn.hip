@@ -67,16 +67,29 @@ struct Box {
     int lo[3], hi[3];
 };
 
+// One FP32 plane as a raw buffer: loads take a 32-bit byte offset (one VALU
+// address for the three position planes) and return 0 past the plane's end.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n * 4u), 0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+}
+
 // Visit every candidate of the box, 64 per step, calling
-// f(valid, slot, d2) on all lanes (valid = lane holds a candidate). Runs of
-// photons (rows) are held one per lane; a scalar cursor walks them, and the
-// next step's positions are loaded before the current step is processed.
+// f(valid, slot, d2) on all lanes (valid = lane holds a candidate; d2 is
+// computed on every lane). Runs of photons (rows) are held one per lane; a
+// scalar cursor walks them (all control is wave-uniform, in SGPRs), and the
+// next step's positions are loaded, unconditionally, before the current step
+// is processed: a lane past its run reads a neighbouring photon, or 0 past
+// the end of the block, and is masked by `valid`.
 template <class F>
 __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
                                      F&& f) {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
     const int nseg = linear ? ny * nz : nx * ny * nz;
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(P.px, P.n), ry = plane_rsrc(P.py, P.n), rz = plane_rsrc(P.pz, P.n);
     for (int sbase = 0; sbase < nseg; sbase += 64) {
         const int s = sbase + (int)lane;
         uint32_t b = 0, len = 0;
@@ -97,19 +110,14 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
         }
         const int ns = min(64, nseg - sbase);
         // Non-empty rows as a lane mask; the cursor visits them in order.
-        uint64_t rows = __ballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
+        uint64_t rows = wballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
         if (!rows) continue;
         int cs = __builtin_ctzll(rows);
         rows &= rows - 1;
         uint32_t rb = __builtin_amdgcn_readlane(b, cs), rl = __builtin_amdgcn_readlane(len, cs), cc = 0;
         bool v0 = lane < rl;
-        uint32_t slot0 = rb + lane;
-        float x0 = 0.f, y0 = 0.f, z0 = 0.f;
-        if (v0) {
-            x0 = P.px[slot0];
-            y0 = P.py[slot0];
-            z0 = P.pz[slot0];
-        }
+        uint32_t o0 = (rb + lane) * 4u;
+        float x0 = bload(rx, o0), y0 = bload(ry, o0), z0 = bload(rz, o0);
         for (;;) {
             // next step: same row if it has more, else the next non-empty row
             bool has_next = true;
@@ -125,25 +133,17 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
                     ncc = 0;
                 }
             }
-            bool v1 = false;
-            uint32_t slot1 = 0;
-            float x1 = 0.f, y1 = 0.f, z1 = 0.f;
-            if (has_next) {
-                v1 = ncc + lane < nrl;
-                slot1 = nrb + ncc + lane;
-                if (v1) {
-                    x1 = P.px[slot1];
-                    y1 = P.py[slot1];
-                    z1 = P.pz[slot1];
-                }
-            }
-            f(v0, slot0, v0 ? dist2(x0, y0, z0, qx, qy, qz) : 0.f);
+            // (after the last step this reads past the row once, harmlessly)
+            const bool v1 = ncc + lane < nrl;
+            const uint32_t o1 = (nrb + ncc + lane) * 4u;
+            const float x1 = bload(rx, o1), y1 = bload(ry, o1), z1 = bload(rz, o1);
+            f(v0, o0 >> 2, dist2(x0, y0, z0, qx, qy, qz));
             if (!has_next) break;
-            cc = ncc;
-            rb = nrb;
-            rl = nrl;
+            cc = __builtin_amdgcn_readfirstlane(ncc);
+            rb = __builtin_amdgcn_readfirstlane(nrb);
+            rl = __builtin_amdgcn_readfirstlane(nrl);
             v0 = v1;
-            slot0 = slot1;
+            o0 = o1;
             x0 = x1;
             y0 = y1;
             z0 = z1;
@@ -300,16 +300,16 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
     uint64_t khi = ~0ull;
     walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
         bool in = valid && d2 <= T;
-        found += (uint32_t)__popcll(__ballot(in));
+        found += (uint32_t)__popcll(wballot(in));
         uint64_t key = make_key(d2, slot);
         bool acc = in && key < khi;
-        uint64_t bal = __ballot(acc);
+        uint64_t bal = wballot(acc);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)CAP) {
             wave_sync();
             cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
             acc = acc && key < khi;
-            bal = __ballot(acc);
+            bal = wballot(acc);
             na = (uint32_t)__popcll(bal);
         }
         if (acc) L.buf[cnt + mbcnt(bal)] = key;
@@ -356,16 +356,16 @@ __global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
     uint64_t khi = ~0ull;
     walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
         bool in = valid && d2 <= T;
-        found += (uint32_t)__popcll(__ballot(in));
+        found += (uint32_t)__popcll(wballot(in));
         uint64_t key = make_key(d2, slot);
         bool acc = in && key < khi;
-        uint64_t bal = __ballot(acc);
+        uint64_t bal = wballot(acc);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)CAP) {
             wave_sync();
             cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
             acc = acc && key < khi;
-            bal = __ballot(acc);
+            bal = wballot(acc);
             na = (uint32_t)__popcll(bal);
         }
         if (acc) L.buf[cnt + mbcnt(bal)] = key;
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(256) void k_knn_two_pass(KnnParams P) {
     const uint32_t found = __shfl(cum, 63, 64);
     uint64_t khi = ~0ull;
     if (found > k) {
-        int ls = __builtin_ctzll(__ballot(cum >= k));
+        int ls = __builtin_ctzll(wballot(cum >= k));
         khi = (uint64_t)(bT - (63u - (uint32_t)ls) + 1u) << 54;
     }
     // collect; if the bin edge keeps more than CAP keys, tighten on the fly
@@ -529,13 +529,13 @@ __global__ __launch_bounds__(256) void k_knn_two_pass(KnnParams P) {
     walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
         uint64_t key = make_key(d2, slot);
         bool acc = valid && d2 <= T && key < khi;
-        uint64_t bal = __ballot(acc);
+        uint64_t bal = wballot(acc);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)CAP) {
             wave_sync();
             cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, bT, khi);
             acc = acc && key < khi;
-            bal = __ballot(acc);
+            bal = wballot(acc);
             na = (uint32_t)__popcll(bal);
         }
         if (acc) L.buf[cnt + mbcnt(bal)] = key;
@@ -672,6 +672,9 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     if (!(radius >= 0.f) || !std::isfinite(radius)) return pm::fail(ctx, PM_ERR_INVALID, "%s: bad radius", who);
     if (k < 1 || k > kMaxK) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: k=%d outside [1, %d]", who, k, kMaxK);
     const pm_grid_desc& D = grid->desc;
+    // candidate positions are read through raw buffers with 32-bit byte offsets
+    if (D.n_photons >= (1u << 30))
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: grid of %u photons (the gather takes < 2^30)", who, D.n_photons);
     P = KnnParams{};
     for (int a = 0; a < 3; ++a) {
         P.G.mn[a] = D.aabb[a];

@@ -52,6 +52,10 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane mask of a predicate (the ballot builtin on the i1 directly, without
+// the int round trip of __ballot).
+__device__ __forceinline__ uint64_t wballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t bal) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
 }
@@ -84,7 +88,7 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
     }
     wave_sync();
     uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
-    int ls = __builtin_ctzll(__ballot(cum >= k));
+    int ls = __builtin_ctzll(wballot(cum >= k));
     uint32_t hstar = 63u - (uint32_t)ls;
     uint32_t A = __shfl(cum, ls, 64);
     uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
@@ -107,7 +111,7 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
         wave_sync();
         uint32_t cu = wave_incl_scan(L.hist[lane], lane);
         uint32_t need = k - below;
-        int js = __builtin_ctzll(__ballot(cu >= need));
+        int js = __builtin_ctzll(wballot(cu >= need));
         uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
         uint32_t at = __shfl(cu, js, 64);
         uint64_t nlo = klo + ((uint64_t)js << s);
@@ -125,7 +129,7 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
         uint64_t key = e < cnt ? L.buf[e] : ~0ull;
         wave_sync();
         bool keep = e < cnt && key < khi;
-        uint64_t bal = __ballot(keep);
+        uint64_t bal = wballot(keep);
         if (keep) L.buf[out + mbcnt(bal)] = key;
         out += (uint32_t)__popcll(bal);
         wave_sync();
