@@ -252,9 +252,11 @@ __device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint
 
 // The candidate box of query q: its cell +- rc, clamped (PhotonGrid.cu:262-272).
 __device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, float& qx, float& qy, float& qz) {
-    qx = P.qx[q];
-    qy = P.qy[q];
-    qz = P.qz[q];
+    // the query is the wave's: its coordinates (and so the box, the row
+    // count and the cursor of walk) are wave-uniform, held in SGPRs
+    qx = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qx[q])));
+    qy = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qy[q])));
+    qz = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qz[q])));
     const float qv[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -275,13 +277,13 @@ __device__ __forceinline__ uint32_t xcd_chunked(uint32_t b) {
 // Query of this wave in a one-wave-per-query launch (XCD-aware block order,
 // optional processing order); false past the end (wave-uniform).
 __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t blk = P.remap == 2 ? xcd_chunked(blockIdx.x)
                          : P.remap   ? pmd::xcd_remap(blockIdx.x, gridDim.x)
                                      : blockIdx.x;
     const uint32_t ql = blk * kWaves + wave;
     if (ql >= P.nq) return false;
-    q = P.order ? P.order[ql] : ql;
+    q = __builtin_amdgcn_readfirstlane(P.order ? P.order[ql] : ql);
     return true;
 }
 
@@ -299,11 +301,15 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
     uint32_t cnt = 0, found = 0;
     uint64_t khi = ~0ull;
     walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
-        bool in = valid && d2 <= T;
-        found += (uint32_t)__popcll(wballot(in));
+        // each ballot is taken of a plain compare (no bool round trip): an
+        // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
+        const bool in = d2 <= (valid ? T : -1.0f);
+        const uint64_t bin = wballot(in);
+        found += (uint32_t)__popcll(bin);
         uint64_t key = make_key(d2, slot);
-        bool acc = in && key < khi;
-        uint64_t bal = wballot(acc);
+        const bool below = key < khi;
+        bool acc = in && below;
+        uint64_t bal = bin & wballot(below);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)CAP) {
             wave_sync();
@@ -355,11 +361,15 @@ __global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
     uint32_t cnt = 0, found = 0;
     uint64_t khi = ~0ull;
     walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
-        bool in = valid && d2 <= T;
-        found += (uint32_t)__popcll(wballot(in));
+        // each ballot is taken of a plain compare (no bool round trip): an
+        // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
+        const bool in = d2 <= (valid ? T : -1.0f);
+        const uint64_t bin = wballot(in);
+        found += (uint32_t)__popcll(bin);
         uint64_t key = make_key(d2, slot);
-        bool acc = in && key < khi;
-        uint64_t bal = wballot(acc);
+        const bool below = key < khi;
+        bool acc = in && below;
+        uint64_t bal = bin & wballot(below);
         uint32_t na = (uint32_t)__popcll(bal);
         if (cnt + na > (uint32_t)CAP) {
             wave_sync();
