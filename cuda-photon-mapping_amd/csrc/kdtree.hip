@@ -182,22 +182,40 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
     const uint32_t tile = s_tile;
     const uint32_t total = 3u * n;  // n <= 2^30
 
-    // position of item k: tile row k, wave, lane (rows of 256 consecutive positions)
-    uint32_t val[kPItems], incl[kPItems];
+    // position of item k: tile row k, wave, lane (rows of 256 consecutive
+    // positions). All loads of the tile's items are issued before any item
+    // is scanned: positions, then their nodes' segments, planes and medians.
+    uint32_t val[kPItems], incl[kPItems], node[kPItems];
+    uint2 sgk[kPItems];
     Rec rec[kPItems];
 #pragma unroll
     for (int k = 0; k < kPItems; ++k) {
         const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
-        uint32_t v = kSegH;
+        node[k] = kFinal;
         if (g < total) {
             const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
-            const uint32_t s = node_in[j];
+            node[k] = node_in[j];
             rec[k] = (a == 0 ? A0 : a == 1 ? A1 : A2)[j];
-            if (s != kFinal) {
-                const uint2 sg = seg[s];
-                const int sd = side_of(rec[k], plane[s], med[s]);
-                v = (j == sg.x ? kSegH : 0u) | (sd == 0 ? 1u : 0u) | (sd == 1 ? kSegM : 0u);
-            }
+        }
+    }
+    int8_t pl[kPItems];
+    uint2 md[kPItems];
+#pragma unroll
+    for (int k = 0; k < kPItems; ++k) {
+        if (node[k] != kFinal) {
+            sgk[k] = seg[node[k]];
+            pl[k] = plane[node[k]];
+            md[k] = med[node[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPItems; ++k) {
+        const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
+        uint32_t v = kSegH;
+        if (node[k] != kFinal) {
+            const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
+            const int sd = side_of(rec[k], pl[k], md[k]);
+            v = (j == sgk[k].x ? kSegH : 0u) | (sd == 0 ? 1u : 0u) | (sd == 1 ? kSegM : 0u);
         }
         val[k] = v;
         // wave-level inclusive segmented scan over the row
@@ -282,11 +300,11 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
         const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
         if (g >= total) continue;
         const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
-        const uint32_t s = node_in[j];
+        const uint32_t s = node[k];
         uint32_t dst = j, nn = kFinal;
         if (s != kFinal) {
             const uint32_t before = seg_op(seg_op(tprefix, s_wpre[k * (kPThreads / 64) + wave]), incl[k]);
-            const uint2 sg = seg[s];
+            const uint2 sg = sgk[k];
             const uint32_t from = sg.x, mid = sg.x + (sg.y - sg.x) / 2;
             const uint32_t own = val[k];
             const uint32_t lb = (before & kSegL) - (own & kSegL), mb = ((before & kSegM) ? 1u : 0u) -
