@@ -319,8 +319,7 @@ __global__ __launch_bounds__(kEmitBlock) void k_trace_emit(TraceParams P, PathSt
     if (i < P.n_paths) {
         rng.init(P.seed, 0u, (uint32_t)(P.path_base + i));
         ray = emit(P, rng);
-        P.cnt_g[i] = 0;
-        P.cnt_c[i] = 0;
+        P.cnt_c[i] = 0;  // (the global counts are not read in this phase)
         keep = may_store_caustic(P, ray);
     }
     const uint64_t m = __ballot(keep);
@@ -438,8 +437,10 @@ __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__
                                                       uint32_t cap_g, float* __restrict__ out_c, uint32_t cap_c) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_paths) return;
-    uint32_t ne = cnt_g[i] + cnt_c[i];
-    uint64_t sg = base_g + off_g[i], sc = base_c + off_c[i];
+    // cnt_g == nullptr: caustic phase, every staged photon is caustic
+    const uint32_t ne = (cnt_g ? cnt_g[i] : 0u) + cnt_c[i];
+    if (ne == 0) return;
+    uint64_t sg = cnt_g ? base_g + off_g[i] : 0, sc = base_c + off_c[i];
     for (uint32_t e = 0; e < ne; ++e) {
         bool c = kind[(size_t)e * n_paths + i] != 0;
         uint64_t s = c ? sc++ : sg++;
@@ -598,16 +599,17 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
                                ctx->stream, P, nullptr, nullptr, nullptr);
             PM_CHECK_LAUNCH(ctx, "k_trace_pool");
         }
-        if ((st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
+        // (in the caustic phase no path stores a global photon)
+        if (!P.caustic_only && (st = pm::scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
-        hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind, cnt_g,
-                           cnt_c, off_g, off_c, nb, P.stage_depth, got_g, got_c, d_global9, n_global, d_caustic9,
-                           n_caustic);
+        hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind,
+                           P.caustic_only ? nullptr : cnt_g, cnt_c, off_g, off_c, nb, P.stage_depth, got_g, got_c,
+                           d_global9, n_global, d_caustic9, n_caustic);
         PM_CHECK_LAUNCH(ctx, "k_trace_compact");
         uint64_t h[2];
         PM_HIP(ctx, hipMemcpyAsync(h, tot, 16, hipMemcpyDeviceToHost, ctx->stream));
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        got_g += h[0];
+        got_g += P.caustic_only ? 0 : h[0];
         got_c += h[1];
         paths += nb;
         // size the next batch for the map that is still short
