@@ -66,3 +66,43 @@ def test_accel_of_another_scene_is_rejected(ctx):
     with pytest.raises(pm355.PMError):
         ctx.trace_photons(b, 1, 1000, 100, 4, max_paths=100_000)
     b.rec.accel = None
+
+
+def _stacked_scene(n_copies=32768):
+    """The Cornell box plus a model of n identical triangles (a horizontal
+    slab at y = 0.5 under the light). All its BVH boxes coincide, so every
+    visited node has both children hit and a traversal pushes at each of the
+    tree's ~log2(n) levels: deeper than the 12 stack levels held in LDS, so
+    the per-lane overflow entries are used. Identical triangles tie on t, so
+    the lowest index must win, as in the brute-force loop."""
+    from pm355.scenes import PM_MODEL_DTYPE, Scene, _model
+
+    base = cornell_box()
+    tri = np.array([0.2, 0.5, 0.2, 0.8, 0.5, 0.2, 0.5, 0.5, 0.8], np.float32)
+    tris = np.tile(tri, (n_copies, 1))
+    nrms = np.tile(np.array([0, 1, 0] * 3, np.float32), (n_copies, 1))
+    t0 = base.tris9.shape[0]
+    models = list(base.models) + [_model(t0, t0 + n_copies, kd=(0.5, 0.5, 0.5), mid=len(base.models))]
+    return Scene(np.concatenate([base.tris9, tris]), np.concatenate([base.normals9, nrms]),
+                 np.array(models, PM_MODEL_DTYPE), light_model=base.light_model, names=base.names + ["stack"])
+
+
+def test_deep_bvh_stack_equals_brute_force(ctx):
+    sc = _stacked_scene()
+    plain, fast = ctx.upload_scene(sc), ctx.upload_scene(sc, accel=True)
+    args = (17, 20_000, 2_000, 6)
+    g0, c0, s0 = ctx.trace_photons(plain, *args, max_paths=400_000)
+    g1, c1, s1 = ctx.trace_photons(fast, *args, max_paths=400_000)
+    assert s0 == s1
+    assert np.array_equal(_np(g0).view(np.uint32), _np(g1).view(np.uint32))
+    assert np.array_equal(_np(c0)[:, : s0[2]].view(np.uint32), _np(c1)[:, : s1[2]].view(np.uint32))
+    # photons landed on the slab (its y) from both paths alike
+    assert np.any(np.abs(_np(g1)[1] - 0.5) < 1e-6)
+    aabb = sc.aabb()
+    gg = ctx.grid_build(g1, aabb, cell_size=0.05)
+    gc = ctx.grid_build(c1, aabb, cell_size=0.05)
+    cam = cornell_camera(96, 64)
+    r0, u0, h0 = ctx.render_frame(plain, gg, gc, cam, 3, 20, 0.05, 20, 0.05)
+    r1, u1, h1 = ctx.render_frame(fast, gg, gc, cam, 3, 20, 0.05, 20, 0.05)
+    assert h0 == h1
+    assert np.array_equal(_np(r0).view(np.uint32), _np(r1).view(np.uint32))
