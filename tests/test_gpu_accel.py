@@ -30,12 +30,13 @@ def test_trace_bvh_equals_brute_force(ctx, seed):
     assert np.array_equal(_np(c0)[:, : s0[2]].view(np.uint32), _np(c1)[:, : s1[2]].view(np.uint32))
 
 
-@pytest.mark.parametrize("seed", [5, 13])
-def test_trace_bvh_matches_oracle(ctx, seed):
+@pytest.mark.parametrize("seed,ng,nc", [(5, 4000, 400), (13, 4000, 400), (13, 1000, 20000)])
+def test_trace_bvh_matches_oracle(ctx, seed, ng, nc):
+    # (1000, 20000): the caustic map fills only in the caustic phase
     sc = random_scene(seed, n_spheres=3, tri_range=(200, 4000))
     fast = ctx.upload_scene(sc, accel=True)
-    g, c, st = ctx.trace_photons(fast, seed, 4000, 400, 10, max_paths=400_000)
-    og, oc, ost = O.trace_photons(sc, seed, 4000, 400, 10, max_paths=400_000)
+    g, c, st = ctx.trace_photons(fast, seed, ng, nc, 10, max_paths=400_000)
+    og, oc, ost = O.trace_photons(sc, seed, ng, nc, 10, max_paths=400_000)
     assert st[1:] == ost[1:]  # paths: the device traces whole batches of paths
     assert np.array_equal(_np(g).view(np.uint32), og.view(np.uint32))
     assert np.array_equal(_np(c)[:, : st[2]].view(np.uint32), oc[:, : st[2]].view(np.uint32))

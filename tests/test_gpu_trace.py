@@ -23,7 +23,11 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.mark.parametrize("seed,ng,nc,mb", [(1, 20000, 2000, 10), (7, 5000, 500, 8), (3, 3000, 50, 1)])
+# (5, 2000, 9000, 10): the global map fills in the first batch and the
+# caustic map only after it, in the caustic phase (emission filter, survivor
+# queue, persistent continuation, one staging slot per path).
+@pytest.mark.parametrize("seed,ng,nc,mb", [(1, 20000, 2000, 10), (7, 5000, 500, 8), (3, 3000, 50, 1),
+                                           (5, 2000, 9000, 10)])
 def test_trace_parity(ctx, dscene, seed, ng, nc, mb):
     g, c, st = ctx.trace_photons(dscene, seed, ng, nc, mb, max_paths=2_000_000)
     og, oc, ost = O.trace_photons(dscene.scene, seed, ng, nc, mb, max_paths=2_000_000)
