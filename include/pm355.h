@@ -117,7 +117,9 @@ pm_status pm_grid_destroy(pm_grid* grid);
  * heap's PhotonMaxHeapItem::idx), d_dist = sqrtf(d2); slots past d_count[q]
  * hold idx 0xFFFFFFFF and dist +inf. 1 <= k <= 2048; k > 256 (the shipped
  * renderer uses k = 2000 and 400) runs a one-wave-per-workgroup kernel that
- * sorts in LDS. */
+ * sorts in LDS. The grid holds fewer than 2^30 photons (positions are read
+ * with 32-bit byte offsets); larger grids are rejected with
+ * PM_ERR_UNSUPPORTED. */
 pm_status pm_grid_knn(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq, float radius,
                       int32_t k, uint32_t flags, uint32_t* d_idx, float* d_dist, uint32_t* d_count);
 
@@ -256,7 +258,9 @@ typedef struct {
  * max_paths paths (the reference spins forever when a map cannot fill,
  * PhotonTracing.cu:344). Output blocks are photons9 with plane stride
  * n_global / n_caustic. Photon power = intensity / n_global for both maps
- * (PhotonTracing.cu:369). 1 <= max_bounces <= 16 (reference: 10). */
+ * (PhotonTracing.cu:369). 1 <= max_bounces <= 16 (reference: 10); at most
+ * 65536 models. stats->paths counts the paths traced, which may exceed the
+ * last path that stored a kept photon (whole batches are traced). */
 pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, uint32_t n_global,
                            uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths, float* d_global9,
                            float* d_caustic9, pm_trace_stats* stats);
