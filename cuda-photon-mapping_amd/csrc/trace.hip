@@ -511,7 +511,10 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     const uint64_t want = (uint64_t)n_global + n_caustic;
     const uint32_t batch_g = (uint32_t)std::max<uint64_t>(
         std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(want, 65536), kGlobalBatch), max_paths), 1);
-    const uint32_t batch_c = (uint32_t)std::max<uint64_t>(std::min<uint64_t>(kCausticBatch, max_paths), 1);
+    // (16 paths per requested photon bound the queue of small traces; a
+    // scene with a lower caustic yield just takes more batches)
+    const uint32_t batch_c = (uint32_t)std::max<uint64_t>(
+        std::min<uint64_t>(std::min<uint64_t>(kCausticBatch, std::max<uint64_t>(16 * want, 65536)), max_paths), 1);
     const uint32_t npath = std::max(batch_g, batch_c);
     const size_t slots = std::max((size_t)batch_g * B, (size_t)batch_c);
     const uint32_t nseg = pm::div_up(batch_c, kEmitBlock);
