@@ -151,6 +151,81 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
     }
 }
 
+// walk with 128-candidate steps: lane l holds candidates l and l + 64 of
+// the step, so a step's scalar cursor and bookkeeping serve two candidates
+// per lane. f(va, sa, d2a, vb, sb, d2b).
+template <class F>
+__device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
+                                      F&& f) {
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
+    const int nseg = linear ? ny * nz : nx * ny * nz;
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(P.px, P.n), ry = plane_rsrc(P.py, P.n), rz = plane_rsrc(P.pz, P.n);
+    for (int sbase = 0; sbase < nseg; sbase += 64) {
+        const int s = sbase + (int)lane;
+        uint32_t b = 0, len = 0;
+        if (s < nseg) {
+            if (linear) {
+                int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
+                uint32_t c0 = pmd::cell_index(P.G, B.lo[0], y, z);
+                uint32_t c1 = c0 + (uint32_t)(nx - 1);
+                b = P.starts[c0];
+                len = P.starts[c1] + P.sizes[c1] - b;
+            } else {
+                int x = B.lo[0] + s % nx, t = s / nx;
+                int y = B.lo[1] + t % ny, z = B.lo[2] + t / ny;
+                uint32_t c = pmd::cell_index(P.G, x, y, z);
+                b = P.starts[c];
+                len = P.sizes[c];
+            }
+        }
+        const int ns = min(64, nseg - sbase);
+        uint64_t rows = wballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
+        if (!rows) continue;
+        int cs = __builtin_ctzll(rows);
+        rows &= rows - 1;
+        uint32_t rb = __builtin_amdgcn_readlane(b, cs), rl = __builtin_amdgcn_readlane(len, cs), cc = 0;
+        bool va0 = lane < rl, vb0 = lane + 64 < rl;
+        uint32_t o0 = (rb + lane) * 4u;
+        float xa0 = bload(rx, o0), ya0 = bload(ry, o0), za0 = bload(rz, o0);
+        float xb0 = bload(rx, o0 + 256u), yb0 = bload(ry, o0 + 256u), zb0 = bload(rz, o0 + 256u);
+        for (;;) {
+            bool has_next = true;
+            uint32_t ncc = cc + 128, nrb = rb, nrl = rl;
+            if (ncc >= rl) {
+                if (!rows) {
+                    has_next = false;
+                } else {
+                    int ncs = __builtin_ctzll(rows);
+                    rows &= rows - 1;
+                    nrb = __builtin_amdgcn_readlane(b, ncs);
+                    nrl = __builtin_amdgcn_readlane(len, ncs);
+                    ncc = 0;
+                }
+            }
+            // (after the last step this reads past the row once, harmlessly)
+            const bool va1 = ncc + lane < nrl, vb1 = ncc + 64 + lane < nrl;
+            const uint32_t o1 = (nrb + ncc + lane) * 4u;
+            const float xa1 = bload(rx, o1), ya1 = bload(ry, o1), za1 = bload(rz, o1);
+            const float xb1 = bload(rx, o1 + 256u), yb1 = bload(ry, o1 + 256u), zb1 = bload(rz, o1 + 256u);
+            f(va0, o0 >> 2, dist2(xa0, ya0, za0, qx, qy, qz), vb0, (o0 >> 2) + 64u, dist2(xb0, yb0, zb0, qx, qy, qz));
+            if (!has_next) break;
+            cc = __builtin_amdgcn_readfirstlane(ncc);
+            rb = __builtin_amdgcn_readfirstlane(nrb);
+            rl = __builtin_amdgcn_readfirstlane(nrl);
+            va0 = va1;
+            vb0 = vb1;
+            o0 = o1;
+            xa0 = xa1;
+            ya0 = ya1;
+            za0 = za1;
+            xb0 = xb1;
+            yb0 = yb1;
+            zb0 = zb1;
+        }
+    }
+}
+
 // Radiance estimate of Raytracer.cuh:110-138 over the m kept photons held in
 // v (sorted); lanes own elements j*64 + lane.
 template <int R>
@@ -300,26 +375,30 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
     const uint32_t k = (uint32_t)P.k;
     uint32_t cnt = 0, found = 0;
     uint64_t khi = ~0ull;
-    walk(P, lane, B, qx, qy, qz, [&](bool valid, uint32_t slot, float d2) {
+    walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
         // each ballot is taken of a plain compare (no bool round trip): an
         // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
-        const bool in = d2 <= (valid ? T : -1.0f);
-        const uint64_t bin = wballot(in);
-        found += (uint32_t)__popcll(bin);
-        uint64_t key = make_key(d2, slot);
-        const bool below = key < khi;
-        bool acc = in && below;
-        uint64_t bal = bin & wballot(below);
-        uint32_t na = (uint32_t)__popcll(bal);
-        if (cnt + na > (uint32_t)CAP) {
+        const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
+        const uint64_t ba = wballot(ina), bb = wballot(inb);
+        found += (uint32_t)(__popcll(ba) + __popcll(bb));
+        const uint64_t ka = make_key(d2a, sa), kb = make_key(d2b, sb);
+        bool la = ka < khi, lb = kb < khi;
+        uint64_t bala = ba & wballot(la), balb = bb & wballot(lb);
+        uint32_t na = (uint32_t)__popcll(bala), nb = (uint32_t)__popcll(balb);
+        if (cnt + na + nb > (uint32_t)CAP) {
+            // leaves at most CAP / 2 keys, room for this step's 128
             wave_sync();
             cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
-            acc = acc && key < khi;
-            bal = wballot(acc);
-            na = (uint32_t)__popcll(bal);
+            la = ka < khi;
+            lb = kb < khi;
+            bala = ba & wballot(la);
+            balb = bb & wballot(lb);
+            na = (uint32_t)__popcll(bala);
+            nb = (uint32_t)__popcll(balb);
         }
-        if (acc) L.buf[cnt + mbcnt(bal)] = key;
-        cnt += na;
+        if (ina && la) L.buf[cnt + mbcnt(bala)] = ka;
+        if (inb && lb) L.buf[cnt + na + mbcnt(balb)] = kb;
+        cnt += na + nb;
     });
     wave_sync();
     finish<CAP>(P, L, lane, q, cnt, found, khi);
