@@ -166,20 +166,22 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
     const Rec* __restrict__ A0, const Rec* __restrict__ A1, const Rec* __restrict__ A2, Rec* __restrict__ B0,
     Rec* __restrict__ B1, Rec* __restrict__ B2, uint32_t n, const uint32_t* __restrict__ node_in,
     uint32_t* __restrict__ node_out, const uint2* __restrict__ seg, const int8_t* __restrict__ plane,
-    const uint2* __restrict__ med, uint64_t* __restrict__ state, uint32_t* __restrict__ counter, uint32_t epoch,
-    uint32_t n_tiles) {
-    __shared__ uint32_t s_tile;
+    const uint2* __restrict__ med, uint64_t* __restrict__ state, uint32_t* __restrict__ counter, uint32_t n_tiles) {
+    __shared__ uint32_t s_tile, s_epoch;
     __shared__ uint32_t s_wagg[kPItems * (kPThreads / 64)];
     __shared__ uint32_t s_wpre[kPItems * (kPThreads / 64)];
     __shared__ uint32_t s_prefix;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t == 0) {
+        // the epoch is advanced by k_child_planes, the next launch: a tile
+        // here may finish its look-back before earlier tiles have read it
+        s_epoch = pm::lookback_epoch(counter);
         uint32_t id = atomicAdd(counter, 1u);
         if (id == n_tiles - 1) atomicExch(counter, 0u);  // every tile holds its id: reset for the next launch
         s_tile = id;
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = s_tile, epoch = s_epoch;
     const uint32_t total = 3u * n;  // n <= 2^30
 
     // position of item k: tile row k, wave, lane (rows of 256 consecutive
@@ -324,8 +326,11 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
 
 __global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
                                                      const Rec* __restrict__ A2, uint32_t first, uint32_t count,
-                                                     const uint2* __restrict__ seg, int8_t* __restrict__ plane) {
+                                                     const uint2* __restrict__ seg, int8_t* __restrict__ plane,
+                                                     uint32_t* __restrict__ lookback_counter) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    // the level partition before this launch has completed: advance its epoch
+    if (t == 0) pm::lookback_advance(lookback_counter, pm::lookback_epoch(lookback_counter));
     if (t >= count) return;
     uint32_t node = first + t;
     uint2 s = seg[node];
@@ -662,20 +667,17 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(count, 256)), dim3(256), 0, s, A[0], A[1], A[2], first,
                            count, cap, seg, plane, T->node_idx, T->node_plane, med);
         if (lvl + 1 == L) break;  // every segment of the last level has size <= 1
-        {
-            const uint32_t tiles = pm::div_up(3ull * n, kPTile);
-            uint64_t* lstate;
-            uint32_t* lcounter;
-            uint32_t lepoch;
-            if ((st = pm::lookback_begin(ctx, tiles, &lstate, &lcounter, &lepoch)) != PM_OK) return bail(st);
-            hipLaunchKernelGGL(k_level_partition, dim3(tiles), dim3(kPThreads), 0, s, A[0], A[1], A[2], B[0], B[1],
-                               B[2], n, node_of, node_of2, seg, plane, med, lstate, lcounter, lepoch, tiles);
-        }
+        const uint32_t tiles = pm::div_up(3ull * n, kPTile);
+        uint64_t* lstate;
+        uint32_t* lcounter;
+        if ((st = pm::lookback_begin(ctx, tiles, &lstate, &lcounter)) != PM_OK) return bail(st);
+        hipLaunchKernelGGL(k_level_partition, dim3(tiles), dim3(kPThreads), 0, s, A[0], A[1], A[2], B[0], B[1], B[2],
+                           n, node_of, node_of2, seg, plane, med, lstate, lcounter, tiles);
         for (int a = 0; a < 3; ++a) std::swap(A[a], B[a]);
         std::swap(node_of, node_of2);
         uint32_t cf = (1u << (lvl + 1)) - 1, cc = 1u << (lvl + 1);
         hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
-                           plane);
+                           plane, lcounter);
     }
     hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx, cap,
                        T->photons);

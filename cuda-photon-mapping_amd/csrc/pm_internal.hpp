@@ -23,8 +23,9 @@ struct pm_ctx {
     // no clearing pass is needed between launches.
     uint64_t* scan_state = nullptr;
     size_t scan_state_tiles = 0;
-    uint32_t* scan_counter = nullptr;  // ordered tile ids, reset by the last tile
-    uint32_t scan_epoch = 0;
+    // [0]: ordered tile ids, reset by each launch's last tile; [1]: the
+    // epoch of the next look-back launch (device-side, starts at 1)
+    uint32_t* scan_counter = nullptr;
     int cu_count = 256;
     // Frame state of pm_render_frame (hit slots and query block survive the
     // gathers, which use `scratch`).
@@ -85,9 +86,20 @@ static inline size_t carve_size(std::initializer_list<size_t> bytes) {
 
 // Decoupled look-back state for one single-pass launch of `tiles` tiles (the
 // context's epoch-tagged 64-bit tile words {epoch:30, status:2, value:32}
-// and its ordered tile counter, which the last tile resets). One launch at a
-// time per context: launches on the context's stream never overlap.
-pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter, uint32_t* epoch);
+// and its counter words: the ordered tile counter, which the last tile
+// resets, and the launch epoch). One launch at a time per context: launches
+// on the context's stream never overlap. Each launch reads the epoch
+// (lookback_epoch, before taking a tile id) and advances it exactly once,
+// when no tile of the launch can still be about to read it.
+pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter);
+__device__ __forceinline__ uint32_t lookback_epoch(uint32_t* counter) {
+    return __hip_atomic_load(counter + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// (after 2^30 - 1 launches the epoch wraps to 1)
+__device__ __forceinline__ void lookback_advance(uint32_t* counter, uint32_t epoch) {
+    __hip_atomic_store(counter + 1, epoch + 1 < (1u << 30) ? epoch + 1 : 1u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Scan over u32 (exclusive or inclusive), optional u64 device total.
 pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total);

@@ -1,5 +1,8 @@
 // Single-pass u32 prefix scan: wave64 shuffle scan inside a 4096-element
-// tile, decoupled look-back across tiles (Merrill & Garland 2016). Replaces
+// tile, decoupled look-back across tiles (Merrill & Garland 2016). Tile
+// words carry an epoch held in device memory (read by every tile, advanced
+// by the last), so consecutive launches need no clearing pass and a launch
+// replayed from a captured graph sees a fresh epoch. Replaces
 // the reference's 44-block Blelloch scan with its software grid barrier
 // (PrefixScanSum.cuh:12-279, CudaGridSynchronizer.cuh:14-27), which needs all
 // blocks co-resident; here a tile only waits on tiles that took a lower
@@ -30,13 +33,14 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
 template <bool kInclusive>
 __global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                         uint32_t n, uint64_t* __restrict__ state,
-                                                        uint32_t* __restrict__ counter, uint32_t epoch,
-                                                        uint32_t n_tiles, uint64_t* __restrict__ total) {
-    __shared__ uint32_t s_tile;
+                                                        uint32_t* __restrict__ counter, uint32_t n_tiles,
+                                                        uint64_t* __restrict__ total) {
+    __shared__ uint32_t s_tile, s_epoch;
     __shared__ uint32_t s_wave[kThreads / 64];
     __shared__ uint32_t s_prefix;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t == 0) {
+        s_epoch = pm::lookback_epoch(counter);
         uint32_t id = atomicAdd(counter, 1u);
         // Every tile has taken its id once the last id is handed out: reset
         // the counter for the next launch on this context.
@@ -44,7 +48,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t* __restric
         s_tile = id;
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = s_tile, epoch = s_epoch;
     const size_t base = (size_t)tile * kTile + (size_t)t * kItems;
 
     uint32_t v[kItems];
@@ -124,7 +128,13 @@ __global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t* __restric
         }
         if (lane == 0) {
             s_prefix = prefix;
-            if (total && tile == n_tiles - 1) *total = (uint64_t)(uint32_t)(prefix + agg);
+            if (tile == n_tiles - 1) {
+                if (total) *total = (uint64_t)(uint32_t)(prefix + agg);
+                // every tile has published (the look-back above reached an
+                // inclusive tile through all of them), so every tile has
+                // read this launch's epoch: advance it for the next launch
+                pm::lookback_advance(counter, epoch);
+            }
         }
     }
     __syncthreads();
@@ -158,8 +168,10 @@ __global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
 namespace pm {
 
-pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter, uint32_t* epoch) {
+pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter) {
     if (tiles > ctx->scan_state_tiles) {
+        // (growth is the only host-side step: warm a context up before
+        // capturing its calls into a graph)
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->scan_state) PM_HIP(ctx, hipFree(ctx->scan_state));
         ctx->scan_state = nullptr;
@@ -167,17 +179,9 @@ pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t
         PM_HIP(ctx, hipMalloc(&ctx->scan_state, want * sizeof(uint64_t)));
         PM_HIP(ctx, hipMemset(ctx->scan_state, 0, want * sizeof(uint64_t)));
         ctx->scan_state_tiles = want;
-        ctx->scan_epoch = 0;
-    }
-    ctx->scan_epoch++;
-    if (ctx->scan_epoch >= (1u << 30)) {
-        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        PM_HIP(ctx, hipMemset(ctx->scan_state, 0, ctx->scan_state_tiles * sizeof(uint64_t)));
-        ctx->scan_epoch = 1;
     }
     *state = ctx->scan_state;
     *counter = ctx->scan_counter;
-    *epoch = ctx->scan_epoch;
     return PM_OK;
 }
 
@@ -192,15 +196,14 @@ pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, b
     uint32_t tiles = div_up(n, kTile);
     uint64_t* state;
     uint32_t* counter;
-    uint32_t epoch;
-    pm_status st = lookback_begin(ctx, tiles, &state, &counter, &epoch);
+    pm_status st = lookback_begin(ctx, tiles, &state, &counter);
     if (st != PM_OK) return st;
     if (inclusive)
         hipLaunchKernelGGL(k_scan_u32<true>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, state, counter,
-                           epoch, tiles, total);
+                           tiles, total);
     else
         hipLaunchKernelGGL(k_scan_u32<false>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, state, counter,
-                           epoch, tiles, total);
+                           tiles, total);
     PM_CHECK_LAUNCH(ctx, "k_scan_u32");
     return PM_OK;
 }
