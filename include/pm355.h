@@ -65,7 +65,15 @@ typedef struct pm_server pm_server;
 /* ---- context (replaces the reference's global device state: GlobalParams
  * __constant__ mirrors GlobalParams.cu:1-58, the heap pointers of
  * set_photon_heap_parameters PhotonMaxHeap.cu:9-17, and checkCudaErrors
- * Defines.cuh:36) ---------------------------------------------------------- */
+ * Defines.cuh:36). Calls are asynchronous on the context's stream.
+ * Graph capture: pm_scan_u32, pm_grid_knn, pm_grid_gather_radiance,
+ * pm_radiance and pm_grid_query_order make no host synchronisation and no
+ * allocation once the context has served a call of the same size, so they
+ * can be captured into a HIP graph and replayed (the look-back scans keep
+ * their epoch in device memory); a captured graph borrows the context's
+ * scratch, so a later call that grows it invalidates the graph. The builds,
+ * the tracer, pm_gather_batch, frames and the server allocate or read
+ * counts back to the host and are not capturable. ------------------------- */
 const char* pm_version(void);
 pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out);
 pm_status pm_ctx_destroy(pm_ctx* ctx);
