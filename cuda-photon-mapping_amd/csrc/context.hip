@@ -77,9 +77,9 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
     c->device = device;
     c->stream = (hipStream_t)hip_stream;
     c->cu_count = prop.multiProcessorCount;
-    const uint32_t words[2] = {0u, 1u};  // tile counter, first look-back epoch
+    const uint32_t first_epoch = 1u;  // tile counter 0, first look-back epoch 1
     if (hipMalloc(&c->scan_counter, 256) != hipSuccess || hipMemset(c->scan_counter, 0, 256) != hipSuccess ||
-        hipMemcpy(c->scan_counter, words, sizeof words, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->scan_counter + pm::kEpochWord, &first_epoch, 4, hipMemcpyHostToDevice) != hipSuccess) {
         if (c->scan_counter) (void)hipFree(c->scan_counter);
         delete c;
         return PM_ERR_NOMEM;

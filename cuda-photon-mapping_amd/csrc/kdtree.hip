@@ -146,9 +146,10 @@ __device__ __forceinline__ int side_of(const Rec& e, int p, uint2 m) {
 //   left: from + Lb,   median: mid,   right: mid + 1 + (j - from - Lb - Mb).
 // This replaces flags -> device scan -> scatter (three passes over 3n
 // positions) by one.
-constexpr int kPThreads = 256;
-constexpr int kPItems = 8;                      // rows of 256 positions per tile
-constexpr uint32_t kPTile = kPThreads * kPItems;  // 2048
+constexpr int kPThreads = 512;                 // (kPItems * kPThreads / 64 wave rows <= 64: one wave scans them)
+constexpr int kPItems = 8;                      // rows of kPThreads positions per tile
+constexpr uint32_t kPTile = kPThreads * kPItems;  // 4096
+static_assert(kPItems * (kPThreads / 64) <= 64, "the wave rows of a tile are scanned by one wave");
 constexpr uint32_t kSegH = 1u << 31, kSegM = 1u << 30, kSegL = (1u << 30) - 1;
 constexpr uint64_t kStAgg = 1, kStIncl = 2;
 
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
     const uint32_t tile = s_tile, epoch = s_epoch;
     const uint32_t total = 3u * n;  // n <= 2^30
 
-    // position of item k: tile row k, wave, lane (rows of 256 consecutive
+    // position of item k: tile row k, wave, lane (rows of kPThreads consecutive
     // positions). All loads of the tile's items are issued before any item
     // is scanned: positions, then their nodes' segments, planes and medians.
     uint32_t val[kPItems], incl[kPItems], node[kPItems];
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
         if (lane == 63) s_wagg[k * (kPThreads / 64) + wave] = x;
     }
     __syncthreads();
-    // exclusive prefixes of the 32 wave rows, in tile order (row-major)
+    // exclusive prefixes of the 64 wave rows, in tile order (row-major)
     if (wave == 0) {
         constexpr uint32_t nw = kPItems * (kPThreads / 64);
         uint32_t x = lane < nw ? s_wagg[lane] : 0u;

@@ -23,7 +23,7 @@ struct pm_ctx {
     // no clearing pass is needed between launches.
     uint64_t* scan_state = nullptr;
     size_t scan_state_tiles = 0;
-    // [0]: ordered tile ids, reset by each launch's last tile; [1]: the
+    // [0]: ordered tile ids, reset by each launch's last tile; [32]: the
     // epoch of the next look-back launch (device-side, starts at 1)
     uint32_t* scan_counter = nullptr;
     int cu_count = 256;
@@ -92,12 +92,17 @@ static inline size_t carve_size(std::initializer_list<size_t> bytes) {
 // (lookback_epoch, before taking a tile id) and advances it exactly once,
 // when no tile of the launch can still be about to read it.
 pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter);
-__device__ __forceinline__ uint32_t lookback_epoch(uint32_t* counter) {
-    return __hip_atomic_load(counter + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// The epoch word sits 128 bytes past the tile counter, on a cache line of
+// its own: every tile reads it with a plain load (an L2 hit after the first;
+// it was written by an earlier launch), while the counter line takes one
+// device-scope atomic per tile, and those serialise.
+constexpr int kEpochWord = 32;
+__device__ __forceinline__ uint32_t lookback_epoch(const uint32_t* counter) {
+    return *(const volatile uint32_t*)(counter + kEpochWord);
 }
 // (after 2^30 - 1 launches the epoch wraps to 1)
 __device__ __forceinline__ void lookback_advance(uint32_t* counter, uint32_t epoch) {
-    __hip_atomic_store(counter + 1, epoch + 1 < (1u << 30) ? epoch + 1 : 1u, __ATOMIC_RELAXED,
+    __hip_atomic_store(counter + kEpochWord, epoch + 1 < (1u << 30) ? epoch + 1 : 1u, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
