@@ -352,8 +352,9 @@ constexpr int kLocalThreads = 256;
 
 struct LocalLds {
     Rec rec[kLocal];          // the subtree's records in axis-0 order
-    uint16_t ord[3][kLocal];  // per axis: local record index at each position
-    uint16_t tmp[kLocal];
+    // per axis: local record index at each position, in buffers ordb[oi[a]];
+    // the fourth buffer receives the next partition and the roles rotate
+    uint16_t ordb[4][kLocal];
     uint16_t pref[kLocal];
     uint16_t node[kLocal];    // local node (at the current depth) of each position, 0xFFFF = final
     ushort2 seg[2][kLocal];   // node segments, current and next depth
@@ -403,10 +404,13 @@ __device__ __forceinline__ void local_scan(LocalLds& L, uint32_t m) {
 }
 
 // find_largest_plane over local segment [f, t) (PhotonMap.cuh:161-178)
-__device__ __forceinline__ int8_t local_plane(const LocalLds& L, uint32_t f, uint32_t t) {
-    float d0 = fabsf(L.rec[L.ord[0][t - 1]].c[0] - L.rec[L.ord[0][f]].c[0]);
-    float d1 = fabsf(L.rec[L.ord[1][t - 1]].c[1] - L.rec[L.ord[1][f]].c[1]);
-    float d2 = fabsf(L.rec[L.ord[2][t - 1]].c[2] - L.rec[L.ord[2][f]].c[2]);
+__device__ __forceinline__ int8_t local_plane(const LocalLds& L, const int (&oi)[3], uint32_t f, uint32_t t) {
+    const uint16_t* o0 = L.ordb[oi[0]];
+    const uint16_t* o1 = L.ordb[oi[1]];
+    const uint16_t* o2 = L.ordb[oi[2]];
+    float d0 = fabsf(L.rec[o0[t - 1]].c[0] - L.rec[o0[f]].c[0]);
+    float d1 = fabsf(L.rec[o1[t - 1]].c[1] - L.rec[o1[f]].c[1]);
+    float d2 = fabsf(L.rec[o2[t - 1]].c[2] - L.rec[o2[f]].c[2]);
     int8_t p = 0;
     float mx = d0;
     if (d1 > mx) {
@@ -433,9 +437,9 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
     // records to axis-0 positions through pos0
     for (uint32_t j = t; j < m; j += kLocalThreads) {
         L.rec[j] = A0[rs.x + j];
-        L.ord[0][j] = (uint16_t)j;
-        L.ord[1][j] = (uint16_t)(pos0[A1[rs.x + j].id] - rs.x);
-        L.ord[2][j] = (uint16_t)(pos0[A2[rs.x + j].id] - rs.x);
+        L.ordb[0][j] = (uint16_t)j;
+        L.ordb[1][j] = (uint16_t)(pos0[A1[rs.x + j].id] - rs.x);
+        L.ordb[2][j] = (uint16_t)(pos0[A2[rs.x + j].id] - rs.x);
         L.node[j] = 0;
     }
     if (t == 0) {
@@ -444,6 +448,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
     }
     __syncthreads();
     int cur = 0;
+    int oi[3] = {0, 1, 2}, spare = 3;  // (workgroup-uniform)
     for (uint32_t d = 0; d < depth_left; ++d) {
         const uint32_t nd = 1u << d;
         const uint32_t gfirst = (root + 1) * nd - 1;  // global index of local node 0 at this depth
@@ -455,7 +460,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
             if (s.y > s.x) {
                 const int8_t p = L.plane[cur][i];
                 const uint32_t mid = s.x + (s.y - s.x) / 2;
-                const Rec r = L.rec[L.ord[p][mid]];
+                const Rec r = L.rec[L.ordb[p == 0 ? oi[0] : p == 1 ? oi[1] : oi[2]][mid]];
                 node_idx[gfirst + i] = r.id;
                 node_plane[gfirst + i] = p;
                 if (!last) L.med[i] = make_uint2(__float_as_uint(r.c[p]), r.id);
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
                 const uint32_t i = L.node[j];
                 uint8_t f = 0;
                 if (i != 0xFFFFu) {
-                    const int sd = side_of(L.rec[L.ord[a][j]], L.plane[cur][i], L.med[i]);
+                    const int sd = side_of(L.rec[L.ordb[oi[a]][j]], L.plane[cur][i], L.med[i]);
                     f = sd == 0;
                     if (sd == 1) L.mpos[i][a] = (uint16_t)j;
                 }
@@ -499,11 +504,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
                         dst = mp == j ? mid : mid + 1 + ((j - from) - before_l - (mp < j ? 1u : 0u));
                     }
                 }
-                L.tmp[dst] = L.ord[a][j];
+                L.ordb[spare][dst] = L.ordb[oi[a]][j];
             }
             __syncthreads();
-            for (uint32_t j = t; j < m; j += kLocalThreads) L.ord[a][j] = L.tmp[j];
-            __syncthreads();
+            const int was = oi[a];
+            oi[a] = spare;
+            spare = was;
         }
         // node of every position at the next depth
         for (uint32_t j = t; j < m; j += kLocalThreads) {
@@ -516,7 +522,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
         // child planes (k_child_planes)
         for (uint32_t c = t; c < 2 * nd; c += kLocalThreads) {
             const ushort2 s = L.seg[cur ^ 1][c];
-            L.plane[cur ^ 1][c] = s.y > s.x ? local_plane(L, s.x, s.y) : (int8_t)0;
+            L.plane[cur ^ 1][c] = s.y > s.x ? local_plane(L, oi, s.x, s.y) : (int8_t)0;
         }
         __syncthreads();
         cur ^= 1;
