@@ -69,12 +69,13 @@ def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
 
 def cpu_stages(threads):
     """Per-stage timings of the oracle on the host cores (the reference CPU
-    path restated): photon tracing of the Cornell box (1 thread, as the
-    oracle's tracer is serial), a 2M-photon grid build and kd-tree build
-    (1 thread), kNN + radiance at k = 64 on 1 thread and on all threads, and
-    a 320x240 frame (100k photons, k = 50, r = 0.05) on all threads. Each
-    stage runs once on a bounded input; the units say what was counted."""
-    import numpy as np
+    path restated), each the median of three runs on identical inputs:
+    photon tracing of the Cornell box (1 thread: the oracle's tracer is
+    serial), a 2M-photon grid build and kd-tree build (1 thread), the kNN
+    gather at k = 64 (1 thread and all threads), the radiance estimate over
+    its result (1 thread), and a 320x240 frame (100k photons, k = 50,
+    r = 0.05) on all threads. The units say what was counted."""
+    import statistics
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -82,35 +83,45 @@ def cpu_stages(threads):
     from pm355.scenes import cornell_box, cornell_camera
     from pm355.workloads import gather_microbench
 
-    out = {"threads_all": threads, "nproc": os.cpu_count()}
+    def med(fn, reps=3):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    out = {"threads_all": threads, "nproc": os.cpu_count(), "runs_per_stage": 3}
     try:
         with open("/proc/cpuinfo") as f:
             out["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
     sc = cornell_box()
-    t0 = time.perf_counter()
-    g, c, st = pyoracle.trace_photons(sc, 7, 100_000, 10_000, 8)
-    dt = time.perf_counter() - t0
+    res = {}
+
+    def trace():
+        res["trace"] = pyoracle.trace_photons(sc, 7, 100_000, 10_000, 8)
+
+    dt = med(trace)
+    st = res["trace"][2]
     out["trace_photons_per_s_1thread"] = (st[1] + st[2]) / dt
     ph, q, qn, aabb = gather_microbench(seed=3, n_photons=2_000_000, n_queries=20_000)
-    t0 = time.perf_counter()
+    out["grid_build_2M_ms_1thread"] = med(lambda: pyoracle.OracleGrid(ph, aabb, cell_size=0.01)) * 1e3
     og = pyoracle.OracleGrid(ph, aabb, cell_size=0.01)
-    out["grid_build_2M_ms_1thread"] = (time.perf_counter() - t0) * 1e3
-    t0 = time.perf_counter()
-    pyoracle.kdtree_build(ph[0:3])
-    out["kdtree_build_2M_ms_1thread"] = (time.perf_counter() - t0) * 1e3
+    out["kdtree_build_2M_ms_1thread"] = med(lambda: pyoracle.kdtree_build(ph[0:3])) * 1e3
     for name, th in (("1thread", 1), ("all", threads)):
         nq = 5_000 if th == 1 else 20_000
-        t0 = time.perf_counter()
-        oi, od, oc = og.knn(q[:, :nq], 0.01, 64, threads=th)
-        pyoracle.radiance(oi, od, oc, og.photons, qn[:, :nq], 64)
-        out[f"knn_radiance_queries_per_s_{name}"] = nq / (time.perf_counter() - t0)
+        out[f"knn_queries_per_s_{name}"] = nq / med(lambda: og.knn(q[:, :nq], 0.01, 64, threads=th))
+    oi, od, oc = og.knn(q[:, :5_000], 0.01, 64, threads=threads)
+    out["radiance_queries_per_s_1thread"] = 5_000 / med(
+        lambda: pyoracle.radiance(oi, od, oc, og.photons, qn[:, :5_000], 64))
+    g, c, _ = res["trace"]
     gg = pyoracle.OracleGrid(g, sc.aabb(), cell_size=0.05)
     gc = pyoracle.OracleGrid(c, sc.aabb(), cell_size=0.05)
-    t0 = time.perf_counter()
-    pyoracle.render(sc, gg, gc, cornell_camera(320, 240), 4, 50, 0.05, 50, 0.05, threads=threads)
-    out["frame_320x240_ms_all"] = (time.perf_counter() - t0) * 1e3
+    cam = cornell_camera(320, 240)
+    out["frame_320x240_ms_all"] = med(
+        lambda: pyoracle.render(sc, gg, gc, cam, 4, 50, 0.05, 50, 0.05, threads=threads)) * 1e3
     return out
 
 
