@@ -380,7 +380,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the photon path "
-              f"runs on a single MI355X; multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+              f"runs on a single MI355X; the photon path shards by query, tile and frame over a replicated photon map)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
