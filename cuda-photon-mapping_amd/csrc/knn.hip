@@ -949,15 +949,19 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
     if (!ctx) return PM_ERR_INVALID;
     if (n_jobs == 0) return PM_OK;
     if (!jobs) return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: null job table");
-    std::vector<KnnParams> hp(n_jobs);
-    std::vector<uint32_t> qbase(n_jobs), cell_base(n_jobs);
+    // Jobs with k <= 256 share one launch of the batch kernel; larger k
+    // (one wave per query over a 2048/4096-entry LDS buffer) run as their own
+    // launches of the single-map kernel, first, on the same stream.
+    std::vector<KnnParams> hp;
+    std::vector<std::pair<const pm_grid*, KnnParams>> large;
+    std::vector<uint32_t> qbase, cell_base;
+    hp.reserve(n_jobs);
     uint64_t total = 0, cells = 0;
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const pm_gather_job& J = jobs[j];
-        KnnParams& P = hp[j];
+        KnnParams P;
         pm_status st = setup(ctx, J.grid, J.d_queries3, J.nq, J.radius, J.k, flags, P, "pm_gather_batch");
         if (st != PM_OK) return st;
-        if (J.k > 256) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_gather_batch: job %u k=%d above 256", j, J.k);
         if (J.nq > 0) {
             if ((J.d_idx == nullptr) != (J.d_dist == nullptr))
                 return pm::fail(ctx, PM_ERR_INVALID, "pm_gather_batch: job %u needs both d_idx and d_dist", j);
@@ -972,13 +976,23 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
         P.nrm = J.d_rgb3 ? J.d_normals3 : nullptr;
         P.rgb = J.d_rgb3;
         P.order = nullptr;
-        qbase[j] = (uint32_t)total;
-        cell_base[j] = (uint32_t)cells;
+        if (J.k > 256) {
+            if (J.nq > 0) large.emplace_back(J.grid, P);
+            continue;
+        }
+        qbase.push_back((uint32_t)total);
+        cell_base.push_back((uint32_t)cells);
+        hp.push_back(P);
         total += J.nq;
         cells += J.grid->desc.n_cells;
         if (total > 0xFFFFFFFFull) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_gather_batch: over 2^32 queries");
     }
+    for (auto& L : large) {
+        pm_status st = launch(ctx, L.first, L.second, flags & PM_KNN_SORT_QUERIES);
+        if (st != PM_OK) return st;
+    }
     if (total == 0) return PM_OK;
+    n_jobs = (uint32_t)hp.size();
     const bool sorted = (flags & PM_KNN_SORT_QUERIES) && cells <= 0xFFFFFFFFull;
     const uint32_t nt = (uint32_t)total;
     const size_t rs = sorted ? pm::radix_sort_scratch_bytes(nt) : 0;

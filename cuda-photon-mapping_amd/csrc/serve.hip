@@ -125,8 +125,6 @@ extern "C" pm_status pm_server_submit(pm_server* S, const pm_job_desc* desc, uin
         return pm::fail(ctx, PM_ERR_INVALID, "pm_server_submit: incomplete job");
     if (d.params.flags & PM_RENDER_REFERENCE_CAUSTIC)
         return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_server_submit: reference-caustic mode renders one frame at a time");
-    if (d.params.k_global > 256 || d.params.k_caustic > 256)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_server_submit: batched gathers take k <= 256");
     auto j = std::make_unique<Job>();
     j->id = S->next_id++;
     j->d = d;

@@ -168,6 +168,22 @@ def cornell_camera(width=320, height=240) -> np.ndarray:
     return camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, width, height)
 
 
+def reference_camera(width=1024, height=768) -> np.ndarray:
+    """The reference's camera for its CornellBox-Sphere scene, at its default
+    1024 x 768 canvas (kernel.cu:14-15, 29-30)."""
+    return camera((-0.00999999046, 0.795000017, 2.35000001), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0,
+                  width, height)
+
+
+# The reference's frame on its shipped scene: photon counts
+# (Initializer.cu:145-146), photon bounces (PHOTON_BOUNCE_NUM,
+# PhotonTracing.cu:20), gather k (kernel.cu:25-26), gather radii
+# (Raytracer.cuh:187, 200), eye-ray depth (kernel.cu:23), canvas
+# (kernel.cu:14-15).
+REFERENCE_FRAME = dict(n_global=1_000_000, n_caustic=100_000, max_bounces=10, max_depth=4, k_global=2000,
+                       r_global=0.1, k_caustic=400, r_caustic=0.2, width=1024, height=768)
+
+
 def random_scene(seed: int, n_spheres=None, tri_range=(200, 20000)) -> Scene:
     """A scene of the batched-workload distribution: the open unit box of
     cornell_box (5 walls and the ceiling light) plus 1-8 spheres, each

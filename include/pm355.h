@@ -157,14 +157,16 @@ pm_status pm_grid_query_order(pm_ctx* ctx, const pm_grid* grid, const float* d_q
  * job's outputs are exactly those of pm_grid_knn (d_idx + d_dist, d_count)
  * and/or pm_grid_gather_radiance (d_rgb3 from d_normals3) on its own.
  * flags: PM_KNN_SORT_QUERIES orders all queries by (job, cell) with one
- * sort. Calls on one context are ordered on its stream; the job table is
- * read before the call returns. */
+ * sort. Jobs with k > 256 (the reference frame's k = 2000 / 400) run as
+ * their own launches of the large-k gather (one wave per query) ahead of the
+ * shared launch. Calls on one context are ordered on its stream; the job
+ * table is read before the call returns. */
 typedef struct {
     const pm_grid* grid;
     const float* d_queries3;  /* [3][nq] */
     uint32_t nq;
     float radius;
-    int32_t k;                /* 1..256 */
+    int32_t k;                /* 1..2048 */
     uint32_t* d_idx;          /* optional [nq][k] grid slots, with d_dist */
     float* d_dist;            /* optional [nq][k] */
     uint32_t* d_count;        /* optional [nq] */
@@ -339,7 +341,7 @@ pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* 
  * d_rgb8 + f * W * H * 3. Each pm_server_step admits queued jobs whose
  * device footprint fits the budget, then renders the next frame of every
  * active job with all of their gathers in one pm_gather_batch. A job's frames
- * are identical alone and in any mix. k <= 256; no reference-caustic mode. */
+ * are identical alone and in any mix. No reference-caustic mode. */
 typedef struct {
     const pm_scene* scene;
     float aabb[6];            /* grid bounds */

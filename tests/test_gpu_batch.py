@@ -1,7 +1,8 @@
 """GPU: the batched gather (one persistent launch over many jobs) returns for
 every job exactly what the single-job calls return, and what the oracle
 returns: jobs of different photon counts, grids (linear and Morton, cell
-sizes), radii, k and query counts, with and without the (job, cell) order."""
+sizes), radii, k (1 to 2000) and query counts, with and without the
+(job, cell) order."""
 import numpy as np
 import pytest
 
@@ -18,7 +19,9 @@ def _jobs(seed):
     for j, (n, nq, r, k, cell_mul, layout) in enumerate([
             (20_000, 3000, 0.05, 50, 1.0, 0), (200_000, 5000, 0.02, 64, 1.0, 0), (5_000, 0, 0.05, 8, 1.0, 0),
             (60_000, 777, 0.03, 1, 0.5, 1), (150_000, 4096, 0.01, 256, 2.0, 0), (10_000, 33, 0.2, 100, 0.5, 0),
-            (400_000, 6000, 0.015, 64, 1.0, 1), (1_000, 100, 0.5, 16, 1.0, 0)]):
+            (400_000, 6000, 0.015, 64, 1.0, 1), (1_000, 100, 0.5, 16, 1.0, 0),
+            # k above 256 (the reference frame's 400 / 2000): own launches in the same call
+            (100_000, 500, 0.05, 400, 1.0, 0), (300_000, 300, 0.08, 2000, 1.0, 1)]):
         ph, q, qn, aabb = gather_microbench(seed=seed * 100 + j, n_photons=n, n_queries=max(nq, 1),
                                             n_clusters=int(rng.integers(1, 16)))
         specs.append((ph, q[:, :nq].copy(), qn[:, :nq].copy(), aabb, r, k, r * cell_mul, layout))
@@ -47,7 +50,8 @@ def test_batch_equals_single_jobs(ctx, flags):
 
 
 def test_batch_matches_oracle(ctx):
-    specs = _jobs(5)[:4]
+    specs = _jobs(5)
+    specs = specs[:4] + specs[8:9]
     grids = [ctx.grid_build(ph, aabb, cell_size=cell, layout=layout) for ph, _, _, aabb, _, _, cell, layout in specs]
     outs = ctx.gather_batch([(g, q, None, r, k) for g, (_, q, _, _, r, k, _, _) in zip(grids, specs)],
                             flags=1, want_knn=True)
