@@ -30,6 +30,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host threads the CPU baselines use: OMP_NUM_THREADS, else all cores
+    (at most 64)."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(threads, 64))
+
+
 def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
     """Oracle (C++ restatement of the reference CPU path) on the host cores:
     grid build once, then kNN + radiance on a bounded sample of the queries."""
@@ -38,8 +45,7 @@ def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
 
     import pyoracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 64))
+    threads = host_threads()
     t0 = time.perf_counter()
     og = pyoracle.OracleGrid(ph, aabb, cell_size=r)
     t_build = time.perf_counter() - t0
@@ -224,6 +230,89 @@ def frame_bench(ctx, n_frames):
     }
 
 
+def reference_frame_bench(ctx, n_frames, cpu_threads=0):
+    """The reference application's own frame: its CornellBox-Sphere scene
+    (2188 triangles, per-model BVHs), 1M global + 100k caustic photons
+    (<= 10 bounces), both grids with the reference's cell sizing, then a
+    1024x768 frame with k = 2000 within r = 0.1 (global) and k = 400 within
+    r = 0.2 (caustic), max depth 4. Reports p50/p99 frame latency and stage
+    medians (HIP events). With cpu_threads > 0 the oracle is timed on a
+    bounded sample of the same frame: a 1/10 photon trace (100k + 10k,
+    scaled x10) and a 256x192 render over the full-size maps (1/16 of the
+    pixels, scaled x16)."""
+    import numpy as np
+    import torch
+
+    from pm355.objload import load_obj
+    from pm355.scenes import REFERENCE_FRAME as RF
+    from pm355.scenes import reference_camera
+
+    scene = load_obj(os.path.join(ROOT, "tests", "golden", "cornell_box_sphere"), "CornellBox-Sphere.obj")
+    ds = ctx.upload_scene(scene, accel=True)
+    aabb = scene.aabb()
+    cam = reference_camera(RF["width"], RF["height"])
+    stream = torch.cuda.current_stream(ctx.device)
+    args = (RF["max_depth"], RF["k_global"], RF["r_global"], RF["k_caustic"], RF["r_caustic"])
+
+    def frame(seed, keep=False):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record(stream)
+        g, c, st = ctx.trace_photons(ds, seed, RF["n_global"], RF["n_caustic"], RF["max_bounces"])
+        ev[1].record(stream)
+        gg = ctx.grid_build(g, aabb)
+        gc = ctx.grid_build(c, aabb)
+        ev[2].record(stream)
+        rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, *args)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        gg.close()
+        gc.close()
+        sm = [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])]
+        return st, hits, sm, ((g, c) if keep else None)
+
+    _, _, _, maps = frame(1, keep=True)
+    lat, stages = [], []
+    for i in range(n_frames):
+        t0 = time.perf_counter()
+        st, hits, sm, _ = frame(200 + i)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        stages.append(sm)
+    stages = np.median(np.array(stages), 0)
+    out = {
+        "workload": "reference frame: CornellBox-Sphere OBJ/MTL (2188 triangles, BVHs), 1M global + 100k caustic "
+                    "photons (<=10 bounces), grids with the reference sizing, 1024x768, k=2000 r=0.1 global, "
+                    "k=400 r=0.2 caustic, max depth 4",
+        "frames": n_frames,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "stage_ms": {"trace": stages[0], "grid_builds": stages[1], "render": stages[2]},
+        "eye_ray_hits": int(hits),
+        "gather_queries_per_s": 2 * int(hits) / (stages[2] * 1e-3),
+        "photons_traced_per_s": (int(st[1]) + int(st[2])) / (stages[0] * 1e-3),
+    }
+    if cpu_threads > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        t0 = time.perf_counter()
+        O.trace_photons(scene, 1, RF["n_global"] // 10, RF["n_caustic"] // 10, RF["max_bounces"])
+        trace_s = (time.perf_counter() - t0) * 10
+        gh, ch = (m.cpu().numpy() for m in maps)
+        t0 = time.perf_counter()
+        og, oc = O.OracleGrid(gh, aabb), O.OracleGrid(ch, aabb)
+        grid_s = time.perf_counter() - t0
+        small = reference_camera(RF["width"] // 4, RF["height"] // 4)
+        t0 = time.perf_counter()
+        O.render(scene, og, oc, small, *args, threads=cpu_threads)
+        render_s = (time.perf_counter() - t0) * 16
+        out["cpu_oracle"] = {"frame_ms_estimate": (trace_s + grid_s + render_s) * 1e3,
+                             "stage_ms_estimate": {"trace": trace_s * 1e3, "grid_builds": grid_s * 1e3,
+                                                   "render": render_s * 1e3},
+                             "threads": {"trace": 1, "grid_builds": 1, "render": cpu_threads},
+                             "sample": "trace 100k + 10k photons (x10), render 256x192 over the full maps (x16)"}
+    return out
+
+
 def batch_bench(ctx, n_jobs, reps=2):
     """Plain per-job form vs the batched persistent form on the many-job
     gather mix (photon counts log-uniform 10k-4M, one query per pixel of a
@@ -367,6 +456,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=12, help="per-frame Cornell-box frames for p50/p99 (0 = skip)")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
+    ap.add_argument("--ref-frames", type=int, default=5,
+                    help="frames of the reference application's CornellBox-Sphere frame (0 = skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the byte-counter scale reference) and exit")
     ap.add_argument("--cell-over-r", type=float, default=0.0, help="grid cell size / r (0 = autotune)")
@@ -618,6 +709,10 @@ def main():
     if args.frames > 0:
         log(f"per-frame form: {args.frames} Cornell-box frames")
         res["per_frame"] = frame_bench(ctx, args.frames)
+    if args.ref_frames > 0:
+        log(f"reference frame: {args.ref_frames} CornellBox-Sphere frames")
+        res["reference_frame"] = reference_frame_bench(ctx, args.ref_frames,
+                                                       0 if args.no_cpu_baseline else host_threads())
     if args.batch_jobs > 0:
         log(f"batched form: {args.batch_jobs}-job gather mix")
         res["batched"] = batch_bench(ctx, args.batch_jobs)

@@ -426,12 +426,7 @@ __global__ __launch_bounds__(256) void k_knn_stream1024(KnnParams P) {
 // the final selection narrows the buffer to exactly the k smallest keys,
 // which a bitonic network sorts in LDS.
 template <int CAP>
-__global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
-    __shared__ WaveLds<CAP> L;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    if (blk >= P.nq) return;
-    const uint32_t q = P.order ? P.order[blk] : blk;
+__device__ __forceinline__ void large_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
     Box B;
     float qx, qy, qz;
     box_of(P, q, B, qx, qy, qz);
@@ -516,6 +511,201 @@ __global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
             P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
             P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
         }
+    }
+}
+
+template <int CAP>
+__global__ __launch_bounds__(64) void k_knn_large(KnnParams P) {
+    __shared__ WaveLds<CAP> L;
+    const uint32_t blk = P.remap ? pmd::xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (blk >= P.nq) return;
+    large_one<CAP>(P, L, threadIdx.x, P.order ? P.order[blk] : blk);
+}
+
+// The queries a k_gather_wide launch handed over (list[0, *count)), one wave
+// per workgroup over a fixed grid.
+template <int CAP>
+__global__ __launch_bounds__(64) void k_knn_large_list(KnnParams P, const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ count) {
+    __shared__ WaveLds<CAP> L;
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        large_one<CAP>(P, L, threadIdx.x, list[i]);
+        wave_sync();
+    }
+}
+
+// Radiance for 256 < k <= 2048 without a k-key buffer (the kernel behind the
+// reference frame's k = 2000 / 400 gathers). One wave per query, three walks
+// of the candidate box:
+//   1. count the photons within r and histogram their d2 into 256 bins (8
+//      per octave below the radius threshold, ascending in d2), keeping the
+//      largest d2;
+//   2. only if more than k are within r: collect the keys (d2, slot) of the
+//      bin holding the k-th smallest into a 512-key LDS buffer and sort it in
+//      registers; the (k - below)-th key there is the k-th key K* overall;
+//   3. sum the estimate over the photons within r with key <= K* (all of them
+//      when at most k are within r), the filter radius being the largest
+//      kept distance.
+// The kept set is exactly the k smallest (d2, slot) keys, as in the buffered
+// kernels; only the order of the radiance sum differs (walk order). A query
+// whose crossing bin holds more than 512 keys is appended to `ovf_list` for
+// k_knn_large_list.
+constexpr uint32_t kWideCap = 512;
+struct WideLds {
+    union {
+        uint32_t hist[256];
+        uint64_t buf[kWideCap];
+    };
+};
+
+__device__ __forceinline__ uint32_t wide_bin(float d2, uint32_t bT20) {
+    // d2 <= T, so bits(d2) >> 20 <= bT20; d2 <= T / 2^32 (and 0) share bin 0
+    const uint32_t h = bT20 - (__float_as_uint(d2) >> 20);
+    return 255u - (h < 255u ? h : 255u);
+}
+
+__global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __restrict__ ovf_list,
+                                                     uint32_t* __restrict__ ovf_count) {
+    __shared__ WideLds lds[kWaves];
+    uint32_t q;
+    if (!query_of_wave(P, q)) return;  // wave-uniform; no workgroup barriers below
+    const uint32_t lane = threadIdx.x & 63;
+    WideLds& L = lds[threadIdx.x >> 6];
+    Box B;
+    float qx, qy, qz;
+    box_of(P, q, B, qx, qy, qz);
+    const float T = P.T;
+    const uint32_t k = (uint32_t)P.k;
+    const uint32_t bT20 = __float_as_uint(T) >> 20;
+
+    for (uint32_t e = lane; e < 256; e += 64) L.hist[e] = 0;
+    wave_sync();
+    uint32_t found = 0;
+    float dmax = 0.f;
+    walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t, float d2a, bool vb, uint32_t, float d2b) {
+        const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
+        found += (uint32_t)(__popcll(wballot(ina)) + __popcll(wballot(inb)));
+        if (ina) {
+            atomicAdd(&L.hist[wide_bin(d2a, bT20)], 1u);
+            dmax = fmaxf(dmax, d2a);
+        }
+        if (inb) {
+            atomicAdd(&L.hist[wide_bin(d2b, bT20)], 1u);
+            dmax = fmaxf(dmax, d2b);
+        }
+    });
+    wave_sync();
+    found = __builtin_amdgcn_readfirstlane(found);
+    const uint32_t m = found < k ? found : k;
+    uint64_t kstar = ~0ull;
+    float rmax2;
+    if (found > k) {
+        // lane l holds bins 4l .. 4l+3; the crossing bin is the first whose
+        // inclusive count from the smallest d2 reaches k
+        const uint32_t h0 = L.hist[4 * lane], h1 = L.hist[4 * lane + 1], h2 = L.hist[4 * lane + 2],
+                       h3 = L.hist[4 * lane + 3];
+        const uint32_t incl = wave_incl_scan(h0 + h1 + h2 + h3, lane);
+        const int ls = __builtin_ctzll(wballot(incl >= k));
+        uint32_t below = __shfl(incl - (h0 + h1 + h2 + h3), ls, 64);
+        const uint32_t c0 = __shfl(h0, ls, 64), c1 = __shfl(h1, ls, 64), c2 = __shfl(h2, ls, 64);
+        uint32_t bstar = 4u * (uint32_t)ls, nb = c0;
+        if (below + c0 < k) {
+            below += c0;
+            ++bstar;
+            nb = c1;
+            if (below + c1 < k) {
+                below += c1;
+                ++bstar;
+                nb = c2;
+                if (below + c2 < k) {
+                    below += c2;
+                    ++bstar;
+                    nb = __shfl(h3, ls, 64);
+                }
+            }
+        }
+        bstar = __builtin_amdgcn_readfirstlane(bstar);
+        below = __builtin_amdgcn_readfirstlane(below);
+        nb = __builtin_amdgcn_readfirstlane(nb);
+        wave_sync();  // the histogram is read before the buffer (same LDS) is written
+        if (nb > kWideCap) {
+            if (lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = q;
+            return;
+        }
+        uint32_t cnt = 0;
+        walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
+            const bool ina = d2a <= (va ? T : -1.0f) && wide_bin(d2a, bT20) == bstar;
+            const bool inb = d2b <= (vb ? T : -1.0f) && wide_bin(d2b, bT20) == bstar;
+            const uint64_t ba = wballot(ina), bb = wballot(inb);
+            const uint32_t na = (uint32_t)__popcll(ba);
+            if (ina) L.buf[cnt + mbcnt(ba)] = make_key(d2a, sa);
+            if (inb) L.buf[cnt + na + mbcnt(bb)] = make_key(d2b, sb);
+            cnt += na + (uint32_t)__popcll(bb);
+        });
+        wave_sync();
+        uint64_t v[kWideCap / 64];
+#pragma unroll
+        for (int j = 0; j < (int)(kWideCap / 64); ++j) {
+            const uint32_t e = (uint32_t)j * 64 + lane;
+            v[j] = e < cnt ? L.buf[e] : ~0ull;
+        }
+        bitonic_sort<kWideCap / 64>(v, lane);
+        const uint32_t r = k - below - 1;  // rank of K* in the bin (< nb)
+        const int jr = (int)(r >> 6);
+        uint64_t pick = 0;
+#pragma unroll
+        for (int j = 0; j < (int)(kWideCap / 64); ++j)
+            if (j == jr) pick = v[j];
+        const uint32_t lo = __shfl((uint32_t)pick, (int)(r & 63), 64);
+        const uint32_t hi = __shfl((uint32_t)(pick >> 32), (int)(r & 63), 64);
+        kstar = ((uint64_t)hi << 32) | lo;
+        rmax2 = __uint_as_float(hi);
+    } else {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, d, 64));
+        rmax2 = dmax;
+    }
+    if (P.out_count && lane == 0) P.out_count[q] = m;
+
+    // Raytracer.cuh:110-138 over the kept photons, in walk order
+    const float alpha = 1.818f, beta = 1.953f;
+    const float pi_f = 3.141592654f;
+    float res0 = 0.f, res1 = 0.f, res2 = 0.f;
+    const float rmax = sqrtf(rmax2);
+    if (m > 0) {
+        const float filter_r = __fmul_rn(rmax, rmax);
+        const float denom = 1.f - expf(-beta);
+        const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
+        auto add = [&](uint32_t idx, float d2) {
+            const float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+            const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
+            if (cosNL > 0.f) {
+                const float d = sqrtf(d2);
+                const float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                const float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                res0 += P.power[idx] * w;
+                res1 += P.power[P.n + idx] * w;
+                res2 += P.power[2 * (size_t)P.n + idx] * w;
+            }
+        };
+        walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
+            if (d2a <= (va ? T : -1.0f) && make_key(d2a, sa) <= kstar) add(sa, d2a);
+            if (d2b <= (vb ? T : -1.0f) && make_key(d2b, sb) <= kstar) add(sb, d2b);
+        });
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        res0 += __shfl_xor(res0, d, 64);
+        res1 += __shfl_xor(res1, d, 64);
+        res2 += __shfl_xor(res2, d, 64);
+    }
+    if (lane == 0) {
+        float coef = 1.f;
+        if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
+        P.rgb[q] = m > 0 ? res0 * coef : 0.f;
+        P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
+        P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
     }
 }
 
@@ -819,19 +1009,49 @@ pm_status query_order(pm_ctx* ctx, const pm_grid* grid, const KnnParams& P, uint
 
 pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags) {
     if (P.nq == 0) return PM_OK;
-    if (flags & PM_KNN_SORT_QUERIES) {
-        // the order lives in the scratch block carved by query_order
-        size_t rs = pm::radix_sort_scratch_bytes(P.nq);
+    // radiance (and counts) without neighbour lists for k > 256: the
+    // three-walk kernel, with an overflow list behind it
+    const bool wide = P.k > 256 && !P.out_idx && P.rgb;
+    const bool sort = (flags & PM_KNN_SORT_QUERIES) != 0;
+    uint32_t* ovf_list = nullptr;
+    uint32_t* ovf_count = nullptr;
+    if (sort || wide) {
+        // the order lives in the scratch block carved by query_order; the
+        // overflow list follows it
+        const size_t rs = sort ? pm::radix_sort_scratch_bytes(P.nq) : 0;
+        const size_t a = sort ? (size_t)P.nq * 4 : 0;
         void* work = nullptr;
-        pm_status st = pm::scratch_reserve(ctx, pm::carve_size({(size_t)P.nq * 4, (size_t)P.nq * 4, (size_t)P.nq * 4, rs}),
-                                           &work);
+        pm_status st = pm::scratch_reserve(
+            ctx, pm::carve_size({a, a, a, rs, wide ? (size_t)P.nq * 4 : 0, wide ? (size_t)64 : 0}), &work);
         if (st != PM_OK) return st;
         pm::Carve cv{static_cast<char*>(work)};
-        cv.take<uint32_t>(P.nq);
-        cv.take<uint32_t>(P.nq);
-        uint32_t* order = cv.take<uint32_t>(P.nq);
-        if ((st = query_order(ctx, grid, P, order)) != PM_OK) return st;
-        P.order = order;
+        cv.take<uint32_t>(a / 4);
+        cv.take<uint32_t>(a / 4);
+        uint32_t* order = cv.take<uint32_t>(a / 4);
+        cv.take<char>(rs);
+        if (wide) {
+            ovf_list = cv.take<uint32_t>(P.nq);
+            ovf_count = cv.take<uint32_t>(16);
+        }
+        if (sort) {
+            if ((st = query_order(ctx, grid, P, order)) != PM_OK) return st;
+            P.order = order;
+        }
+    }
+    if (wide) {
+        pm_status st = pm::fill_u32(ctx, ovf_count, 0u, 1);
+        if (st != PM_OK) return st;
+        uint32_t nblk = pm::div_up(P.nq, kWaves);
+        if (P.remap == 2) nblk = pm::div_up(nblk, 8 * kXcdChunk) * 8 * kXcdChunk;
+        hipLaunchKernelGGL(k_gather_wide, dim3(nblk), dim3(256), 0, ctx->stream, P, ovf_list, ovf_count);
+        PM_CHECK_LAUNCH(ctx, "k_gather_wide");
+        const uint32_t lblocks = std::min<uint32_t>(P.nq, (uint32_t)std::max(ctx->cu_count, 1) * 8u);
+        if (P.k > 1024)
+            hipLaunchKernelGGL(k_knn_large_list<4096>, dim3(lblocks), dim3(64), 0, ctx->stream, P, ovf_list, ovf_count);
+        else
+            hipLaunchKernelGGL(k_knn_large_list<2048>, dim3(lblocks), dim3(64), 0, ctx->stream, P, ovf_list, ovf_count);
+        PM_CHECK_LAUNCH(ctx, "k_knn_large_list");
+        return PM_OK;
     }
     if (P.k > 1024) {
         hipLaunchKernelGGL(k_knn_large<4096>, dim3(P.nq), dim3(64), 0, ctx->stream, P);
