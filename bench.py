@@ -254,13 +254,13 @@ def reference_frame_bench(ctx, n_frames, cpu_threads=0):
     stream = torch.cuda.current_stream(ctx.device)
     args = (RF["max_depth"], RF["k_global"], RF["r_global"], RF["k_caustic"], RF["r_caustic"])
 
-    def frame(seed, keep=False):
+    def frame(seed, keep=False, cell_over_r=0.0):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         ev[0].record(stream)
         g, c, st = ctx.trace_photons(ds, seed, RF["n_global"], RF["n_caustic"], RF["max_bounces"])
         ev[1].record(stream)
-        gg = ctx.grid_build(g, aabb)
-        gc = ctx.grid_build(c, aabb)
+        gg = ctx.grid_build(g, aabb, cell_size=cell_over_r * RF["r_global"])
+        gc = ctx.grid_build(c, aabb, cell_size=cell_over_r * RF["r_caustic"])
         ev[2].record(stream)
         rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, *args)
         ev[3].record(stream)
@@ -278,6 +278,16 @@ def reference_frame_bench(ctx, n_frames, cpu_threads=0):
         lat.append((time.perf_counter() - t0) * 1e3)
         stages.append(sm)
     stages = np.median(np.array(stages), 0)
+    # the same frames with grid cells of r/2 (the gather's tuned cell size;
+    # neighbour sets and images are unchanged by the cell size)
+    frame(1, cell_over_r=0.5)
+    tlat, tstages = [], []
+    for i in range(n_frames):
+        t0 = time.perf_counter()
+        _, _, sm, _ = frame(200 + i, cell_over_r=0.5)
+        tlat.append((time.perf_counter() - t0) * 1e3)
+        tstages.append(sm)
+    tstages = np.median(np.array(tstages), 0)
     out = {
         "workload": "reference frame: CornellBox-Sphere OBJ/MTL (2188 triangles, BVHs), 1M global + 100k caustic "
                     "photons (<=10 bounces), grids with the reference sizing, 1024x768, k=2000 r=0.1 global, "
@@ -289,6 +299,8 @@ def reference_frame_bench(ctx, n_frames, cpu_threads=0):
         "eye_ray_hits": int(hits),
         "gather_queries_per_s": 2 * int(hits) / (stages[2] * 1e-3),
         "photons_traced_per_s": (int(st[1]) + int(st[2])) / (stages[0] * 1e-3),
+        "cell_half_r": {"p50_ms": float(np.percentile(tlat, 50)), "p99_ms": float(np.percentile(tlat, 99)),
+                        "stage_ms": {"trace": tstages[0], "grid_builds": tstages[1], "render": tstages[2]}},
     }
     if cpu_threads > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -458,6 +470,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched gather mix (0 = skip)")
     ap.add_argument("--ref-frames", type=int, default=5,
                     help="frames of the reference application's CornellBox-Sphere frame (0 = skip)")
+    ap.add_argument("--ref-frame-probe", action="store_true",
+                    help="only render the reference frame (3 frames per grid cell size, no CPU part) and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the byte-counter scale reference) and exit")
     ap.add_argument("--cell-over-r", type=float, default=0.0, help="grid cell size / r (0 = autotune)")
@@ -501,6 +515,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"trace_probe": {"paths": int(st[0]), "global": int(st[1]), "caustic": int(st[2])}}),
               flush=True)
+        ctx.close()
+        return
+
+    if args.ref_frame_probe:
+        ctx = pm355.Context(0)
+        print(json.dumps({"reference_frame": reference_frame_bench(ctx, 3)}), flush=True)
         ctx.close()
         return
 

@@ -152,3 +152,28 @@ def test_shipped_served_frames(ctx, scene):
     (out,), *_ = _serve(ctx, [spec])
     ref = _one_at_a_time(ctx, spec)
     assert np.array_equal(out, ref.reshape(out.shape))
+
+
+def test_render_obj_cli(ctx, scene, tmp_path, capsys):
+    """The headless renderer writes the canvas render_frame returns for the
+    same scene, photon maps and parameters."""
+    from pm355.image import read_ppm
+    from pm355.render_obj import main
+
+    obj = os.path.join(GOLDEN, "cornell_box_sphere", "CornellBox-Sphere.obj")
+    out = str(tmp_path / "f.ppm")
+    W, H = 96, 72
+    assert main([obj, out, "--width", str(W), "--height", str(H), "--global-photons", "50000",
+                 "--caustic-photons", "5000", "--seed", "5", "--pfm"]) == 0
+    assert os.path.exists(str(tmp_path / "f.pfm"))
+    ds = ctx.upload_scene(scene, accel=True)
+    g, c, st = ctx.trace_photons(ds, 5, 50_000, 5_000, RF["max_bounces"])
+    aabb = scene.aabb()
+    gg, gc = ctx.grid_build(g, aabb), ctx.grid_build(c, aabb)
+    _, rgb8, _ = ctx.render_frame(ds, gg, gc, reference_camera(W, H), RF["max_depth"], RF["k_global"],
+                                  RF["r_global"], RF["k_caustic"], RF["r_caustic"])
+    ctx.sync()
+    assert np.array_equal(read_ppm(out)[::-1].reshape(-1), _np(rgb8))
+    assert '"eye_ray_hits"' in capsys.readouterr().out
+    gg.close()
+    gc.close()
