@@ -76,6 +76,63 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_o
     return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
 }
 
+// Lower bound of |v - p| along one axis over the photons hashed into cell c
+// (cells 0 and g-1 also hold every photon clamped into them). The cell's
+// bounds are widened by a relative 1e-6, far above the rounding of the hash
+// (p - mn) / cell and of the bounds themselves.
+__device__ __forceinline__ float axis_gap(float v, int c, int g, float mn, float cell) {
+    const float lo = mn + (float)c * cell, hi = mn + (float)(c + 1) * cell;
+    const float e = (fabsf(lo) + fabsf(hi) + fabsf(mn)) * 1e-6f;
+    if (c > 0 && v < lo - e) return (lo - e) - v;
+    if (c < g - 1 && v > hi + e) return v - (hi + e);
+    return 0.f;
+}
+
+// Photon run s of the box (linear layout: the cells of one (y, z) row;
+// Morton: one cell) as [b, b + len) in grid-slot order, pruned to the cells
+// that can hold a photon within the radius: a row whose (y, z) gap alone
+// exceeds it is skipped, and the x extent is cut to the cells meeting the
+// slab |x - qx| <= sqrt(T - gap_yz^2). Every bound is widened so that a
+// photon passing the FP32 test d2 <= T is never cut: the squared gaps are
+// compared with T (1 + 1e-3) (rounding of d2 and of the gaps is a few ulps),
+// and the slab half-width is sqrt(T (1 + 1e-3) - gap_yz^2 + 1e-3 T) plus a
+// relative 1e-6 before it is hashed with the grid's own monotone cell_coord.
+// The kept candidates are a subset of the reference's box holding all of
+// its candidates within r, so results are unchanged.
+__device__ __forceinline__ bool row_span(const GridGeom& G, const uint32_t* __restrict__ starts,
+                                         const uint32_t* __restrict__ sizes, const Box& B, int s, float qx, float qy,
+                                         float qz, float T, uint32_t& b, uint32_t& len) {
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    const float thr = T * 1.001f;
+    b = 0;
+    len = 0;
+    if (G.layout == PM_LAYOUT_LINEAR) {
+        const int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
+        const float gy = axis_gap(qy, y, G.g[1], G.mn[1], G.cell), gz = axis_gap(qz, z, G.g[2], G.mn[2], G.cell);
+        const float dyz2 = gy * gy + gz * gz;
+        if (!(dyz2 <= thr)) return false;
+        const float dx = sqrtf(thr - dyz2 + 1e-3f * T);
+        const float e = (fabsf(qx) + dx) * 1e-6f;
+        const int xl = max(B.lo[0], pmd::cell_coord(qx - dx - e, G.mn[0], G.cell, G.g[0]));
+        const int xh = min(B.hi[0], pmd::cell_coord(qx + dx + e, G.mn[0], G.cell, G.g[0]));
+        if (xl > xh) return false;
+        const uint32_t c0 = pmd::cell_index(G, xl, y, z);
+        const uint32_t c1 = c0 + (uint32_t)(xh - xl);
+        b = starts[c0];
+        len = starts[c1] + sizes[c1] - b;
+    } else {
+        const int x = B.lo[0] + s % nx, t = s / nx;
+        const int y = B.lo[1] + t % ny, z = B.lo[2] + t / ny;
+        const float gx = axis_gap(qx, x, G.g[0], G.mn[0], G.cell), gy = axis_gap(qy, y, G.g[1], G.mn[1], G.cell),
+                    gz = axis_gap(qz, z, G.g[2], G.mn[2], G.cell);
+        if (!(gx * gx + gy * gy + gz * gz <= thr)) return false;
+        const uint32_t c = pmd::cell_index(G, x, y, z);
+        b = starts[c];
+        len = sizes[c];
+    }
+    return true;
+}
+
 // Visit every candidate of the box, 64 per step, calling
 // f(valid, slot, d2) on all lanes (valid = lane holds a candidate; d2 is
 // computed on every lane). Runs of photons (rows) are held one per lane; a
@@ -93,21 +150,7 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
     for (int sbase = 0; sbase < nseg; sbase += 64) {
         const int s = sbase + (int)lane;
         uint32_t b = 0, len = 0;
-        if (s < nseg) {
-            if (linear) {
-                int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
-                uint32_t c0 = pmd::cell_index(P.G, B.lo[0], y, z);
-                uint32_t c1 = c0 + (uint32_t)(nx - 1);
-                b = P.starts[c0];
-                len = P.starts[c1] + P.sizes[c1] - b;
-            } else {
-                int x = B.lo[0] + s % nx, t = s / nx;
-                int y = B.lo[1] + t % ny, z = B.lo[2] + t / ny;
-                uint32_t c = pmd::cell_index(P.G, x, y, z);
-                b = P.starts[c];
-                len = P.sizes[c];
-            }
-        }
+        if (s < nseg) row_span(P.G, P.starts, P.sizes, B, s, qx, qy, qz, P.T, b, len);
         const int ns = min(64, nseg - sbase);
         // Non-empty rows as a lane mask; the cursor visits them in order.
         uint64_t rows = wballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
@@ -164,21 +207,7 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
     for (int sbase = 0; sbase < nseg; sbase += 64) {
         const int s = sbase + (int)lane;
         uint32_t b = 0, len = 0;
-        if (s < nseg) {
-            if (linear) {
-                int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
-                uint32_t c0 = pmd::cell_index(P.G, B.lo[0], y, z);
-                uint32_t c1 = c0 + (uint32_t)(nx - 1);
-                b = P.starts[c0];
-                len = P.starts[c1] + P.sizes[c1] - b;
-            } else {
-                int x = B.lo[0] + s % nx, t = s / nx;
-                int y = B.lo[1] + t % ny, z = B.lo[2] + t / ny;
-                uint32_t c = pmd::cell_index(P.G, x, y, z);
-                b = P.starts[c];
-                len = P.sizes[c];
-            }
-        }
+        if (s < nseg) row_span(P.G, P.starts, P.sizes, B, s, qx, qy, qz, P.T, b, len);
         const int ns = min(64, nseg - sbase);
         uint64_t rows = wballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
         if (!rows) continue;
@@ -906,27 +935,29 @@ __global__ __launch_bounds__(256) void k_query_keys(GridGeom G, const float* __r
 __global__ __launch_bounds__(256) void k_box_stats(GridGeom G, const uint32_t* __restrict__ starts,
                                                   const uint32_t* __restrict__ sizes, const float* __restrict__ qx,
                                                   const float* __restrict__ qy, const float* __restrict__ qz,
-                                                  uint32_t nq, int rc, unsigned long long* __restrict__ out) {
+                                                  uint32_t nq, int rc, float T, unsigned long long* __restrict__ out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cells = 0, cands = 0;
     if (i < nq) {
         const float qv[3] = {qx[i], qy[i], qz[i]};
-        int lo[3], hi[3];
+        Box B;
         for (int a = 0; a < 3; ++a) {
             int c = pmd::cell_coord(qv[a], G.mn[a], G.cell, G.g[a]);
-            lo[a] = max(0, c - rc);
-            hi[a] = min(G.g[a] - 1, c + rc);
+            B.lo[a] = max(0, c - rc);
+            B.hi[a] = min(G.g[a] - 1, c + rc);
         }
-        cells = (unsigned long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
-        for (int z = lo[2]; z <= hi[2]; ++z)
-            for (int y = lo[1]; y <= hi[1]; ++y) {
-                if (G.layout == PM_LAYOUT_LINEAR) {
-                    uint32_t c0 = pmd::cell_index(G, lo[0], y, z), c1 = c0 + (uint32_t)(hi[0] - lo[0]);
-                    cands += starts[c1] + sizes[c1] - starts[c0];
-                } else {
-                    for (int x = lo[0]; x <= hi[0]; ++x) cands += sizes[pmd::cell_index(G, x, y, z)];
-                }
-            }
+        // the runs the gather walks (row_span): cells and photons visited
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        const int nseg = G.layout == PM_LAYOUT_LINEAR ? ny * nz : nx * ny * nz;
+        // (C_q stays the box's cell count in the linear layout, an upper
+        // bound of the start / size words its row spans read)
+        for (int s = 0; s < nseg; ++s) {
+            uint32_t b, len;
+            const bool kept = row_span(G, starts, sizes, B, s, qv[0], qv[1], qv[2], T, b, len);
+            cands += len;
+            if (G.layout != PM_LAYOUT_LINEAR) cells += kept;
+        }
+        if (G.layout == PM_LAYOUT_LINEAR) cells = (unsigned long long)nx * ny * nz;
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1144,7 +1175,7 @@ extern "C" pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const f
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, ctx->stream, acc);
     if (nq > 0)
         hipLaunchKernelGGL(k_box_stats, dim3(pm::div_up(nq, 256)), dim3(256), 0, ctx->stream, P.G, P.starts, P.sizes,
-                           P.qx, P.qy, P.qz, nq, P.rc, acc);
+                           P.qx, P.qy, P.qz, nq, P.rc, P.T, acc);
     PM_CHECK_LAUNCH(ctx, "k_box_stats");
     unsigned long long h[2] = {0, 0};
     PM_HIP(ctx, hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

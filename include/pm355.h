@@ -176,8 +176,12 @@ typedef struct {
 
 pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* h_jobs, uint32_t n_jobs, uint32_t flags);
 
-/* Per-query candidate statistics of the gather box (cells visited C_q and
- * photons in them P_q, summed), used for the algorithmic-byte count. */
+/* Per-query candidate statistics of the gather (cells C_q whose bounds it
+ * reads and photons P_q it tests, summed), used for the algorithmic-byte
+ * count. The gather visits the reference's box (cell +- ceil(r/cell),
+ * PhotonGrid.cu:262-272) pruned to the rows (linear layout) or cells (Morton)
+ * that can hold a photon within r; C_q of the linear layout is the box's
+ * cell count. */
 pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
                             float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands);
 
