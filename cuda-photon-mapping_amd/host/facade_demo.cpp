@@ -3,6 +3,7 @@
 // PhotonMap and query it, and write every result to <outdir> as raw little-endian arrays
 // for the parity test to compare with the oracle.
 //   facade_demo <outdir> <n_photons> <n_queries> <seed>
+// and, in `render` mode, the reference application's frame on an OBJ scene.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -19,7 +20,61 @@ static void write_file(const std::string& path, const void* p, size_t bytes) {
     std::fclose(f);
 }
 
+// The reference application's flow on an OBJ scene (kernel.cu:8-40,
+// Initializer.cu:179-201, Drawer.cu:18-101) through the façade: load_obj,
+// DeviceScene with BVHs, trace_photons, both PhotonGrids with the reference
+// sizing, render_frame with the reference's k / radii / depth and camera;
+// writes the photon maps, the uchar3 canvas and frame.ppm.
+//   facade_demo render <obj> <outdir> <W> <H> <n_global> <n_caustic> <seed>
+static int render_main(int argc, char** argv) {
+    if (argc != 9) {
+        std::fprintf(stderr, "usage: %s render obj outdir W H n_global n_caustic seed\n", argv[0]);
+        return 2;
+    }
+    const std::string out = argv[3];
+    const int W = std::atoi(argv[4]), H = std::atoi(argv[5]);
+    const uint32_t ng = (uint32_t)std::strtoul(argv[6], nullptr, 10), nc = (uint32_t)std::strtoul(argv[7], nullptr, 10);
+    const uint64_t seed = std::strtoull(argv[8], nullptr, 10);
+    try {
+        const pm355::HostScene hs = pm355::load_obj(argv[2]);
+        pm355::Context ctx(0);
+        pm355::DeviceScene ds(ctx, hs, true);
+        pm355::PhotonArrays maps = pm355::trace_photons(ctx, ds.get(), ng, nc, seed, 10);
+        pm355::AABB box;
+        hs.aabb(box.corners);
+        pm355::PhotonGrid gg(ctx, (int)ng, box), gc(ctx, (int)nc, box);
+        gg.build(maps.global9.data());
+        gc.build(maps.caustic9.data());
+        const pm_camera cam = pm355::reference_camera(W, H);
+        pm_render_params p{};
+        p.max_depth = 4;
+        p.k_global = 2000;
+        p.r_global = 0.1f;
+        p.k_caustic = 400;
+        p.r_caustic = 0.2f;
+        pm355::DeviceBuffer<uint8_t> rgb8((size_t)W * H * 3);
+        const pm_frame_stats st = pm355::render_frame(ctx, ds.get(), gg, &gc, cam, p, nullptr, rgb8.data());
+        ctx.sync();
+        const std::vector<uint8_t> h8 = rgb8.to_host();
+        const std::vector<float> g9 = maps.global9.to_host(), c9 = maps.caustic9.to_host();
+        write_file(out + "/global9.f32", g9.data(), g9.size() * 4);
+        write_file(out + "/caustic9.f32", c9.data(), c9.size() * 4);
+        write_file(out + "/rgb8.u8", h8.data(), h8.size());
+        pm355::write_ppm(out + "/frame.ppm", h8.data(), W, H);
+        std::printf("facade_demo render ok: models=%zu tris=%u global=%u caustic=%u hits=%llu\n", hs.models.size(),
+                    hs.n_tris(), maps.stats.n_global, maps.stats.n_caustic, (unsigned long long)st.hits);
+    } catch (const pm355::Error& e) {
+        std::fprintf(stderr, "pm355 error %d: %s\n", e.code, e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "render") return render_main(argc, argv);
     if (argc != 5) {
         std::fprintf(stderr, "usage: %s outdir n_photons n_queries seed\n", argv[0]);
         return 2;

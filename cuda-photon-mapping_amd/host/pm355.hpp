@@ -11,6 +11,10 @@
 //                                             -> pm355::trace_photons
 //   Raytracer::render_gpu + RaytracePlanner (Raytracer.cuh:245,
 //        RaytracePlanner.cu:18)              -> pm355::render_frame
+//   loadOBJ / loadMaterial (ObjLoader.cpp:4-306), Camera (Camera.cuh:43-66)
+//        and the GL canvas (Window.cuh)      -> scene_io.hpp: load_obj,
+//                                               make_camera, write_ppm;
+//                                               pm355::DeviceScene
 // Differences from the reference are those of the C ABI: batched queries,
 // borrowed inputs, status codes raised as pm355::Error (never exit()), and
 // deterministic results.
@@ -18,11 +22,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/pm355.h"
+#include "scene_io.hpp"
 
 namespace pm355 {
 
@@ -227,6 +233,39 @@ public:
 
 private:
     pm_accel* a_ = nullptr;
+};
+
+// A HostScene (scene_io.hpp: load_obj) in device memory as a pm_scene
+// (initialize_scene_and_photon_maps, Initializer.cu:179-201): triangles and
+// normals copied once, model records kept on the host, light intensity 1
+// and default IOR 1 (Light.cuh:8, kernel.cu:24); optional per-model BVHs.
+class DeviceScene {
+public:
+    DeviceScene(Context& ctx, const HostScene& s, bool bvh = true)
+        : tris_(s.tris9.size()), normals_(s.normals9.size()), models_(s.models) {
+        tris_.from_host(s.tris9);
+        normals_.from_host(s.normals9);
+        rec_ = pm_scene{};
+        rec_.d_tris9 = tris_.data();
+        rec_.d_normals9 = normals_.data();
+        rec_.n_tris = s.n_tris();
+        rec_.h_models = models_.data();
+        rec_.n_models = (int32_t)models_.size();
+        rec_.light_model = s.light_model;
+        for (int a = 0; a < 3; ++a) rec_.light_intensity[a] = 1.f;
+        rec_.default_refr_index = 1.f;
+        if (bvh) {
+            accel_ = std::make_unique<SceneAccel>(ctx, rec_);
+            rec_.accel = accel_->get();
+        }
+    }
+    const pm_scene& get() const { return rec_; }
+
+private:
+    DeviceBuffer<float> tris_, normals_;
+    std::vector<pm_model> models_;
+    pm_scene rec_{};
+    std::unique_ptr<SceneAccel> accel_;
 };
 
 // trace_photons (PhotonTracing.cu:379-422): returns the global ("diffuse")

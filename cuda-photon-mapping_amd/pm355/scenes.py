@@ -9,6 +9,7 @@ reference's Model + Material (Model.cuh:24-30, Material.cuh:4-31).
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -139,7 +140,8 @@ def camera(position, direction, up, vfov_deg, width, height) -> np.ndarray:
     u = np.asarray(up, f32)
     aspect = f32(width) / f32(height)
     theta = f32(f32(vfov_deg) * f32(3.141592653589793)) / f32(180.0)
-    h = f32(np.tan(theta / f32(2)))
+    # tan in double, rounded once (tanf implementations differ by an ulp)
+    h = f32(math.tan(float(theta / f32(2))))
     vh = f32(2.0) * h
     vw = aspect * vh
 

@@ -36,3 +36,35 @@ def test_facade_demo_matches_oracle(tmp_path):
     assert np.array_equal(rd("kdknn_count.u32", np.uint32), qc)
     assert np.array_equal(rd("kdknn_idx.u32", np.uint32).reshape(nq, 32), qi)
     assert np.array_equal(rd("kdknn_dist.f32", np.float32).reshape(nq, 32).view(np.uint32), qd.view(np.uint32))
+
+
+def test_facade_render_obj_equals_python_path(ctx, tmp_path):
+    """`facade_demo render`: the C++ loader, DeviceScene, trace_photons, two
+    PhotonGrids and render_frame on the shipped CornellBox-Sphere give the
+    photon maps and canvas of the Python path (pm355.objload + Context),
+    which the shipped-scene tests check against the oracle."""
+    from conftest import GOLDEN
+    from pm355.image import read_ppm
+    from pm355.objload import load_obj
+    from pm355.scenes import reference_camera
+
+    exe = os.path.join(PKG, "facade_demo")
+    obj = os.path.join(GOLDEN, "cornell_box_sphere", "CornellBox-Sphere.obj")
+    W, H, ng, nc, seed = 128, 96, 100_000, 10_000, 3
+    r = subprocess.run([exe, "render", obj, str(tmp_path), str(W), str(H), str(ng), str(nc), str(seed)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    scene = load_obj(obj)
+    ds = ctx.upload_scene(scene, accel=True)
+    g, c, st = ctx.trace_photons(ds, seed, ng, nc, 10)
+    aabb = scene.aabb()
+    gg, gc = ctx.grid_build(g, aabb), ctx.grid_build(c, aabb)
+    _, rgb8, _ = ctx.render_frame(ds, gg, gc, reference_camera(W, H), 4, 2000, 0.1, 400, 0.2)
+    ctx.sync()
+    rd = lambda name, dt: np.fromfile(os.path.join(tmp_path, name), dtype=dt)
+    assert np.array_equal(rd("global9.f32", np.uint32), g.cpu().numpy().reshape(-1).view(np.uint32))
+    assert np.array_equal(rd("caustic9.f32", np.uint32), c.cpu().numpy().reshape(-1).view(np.uint32))
+    assert np.array_equal(rd("rgb8.u8", np.uint8), rgb8.cpu().numpy())
+    assert np.array_equal(read_ppm(os.path.join(tmp_path, "frame.ppm"))[::-1].reshape(-1), rgb8.cpu().numpy())
+    gg.close()
+    gc.close()

@@ -220,3 +220,72 @@ def test_ppm_and_pfm_orientation(tmp_path):
     assert data.startswith(hdr)
     px = np.frombuffer(data[len(hdr):], "<f4").reshape(H, W, 3)
     assert np.array_equal(px[0, 0], planes[:, 0]) and np.array_equal(px[2, 4], planes[:, 14])
+
+
+@pytest.fixture(scope="module")
+def cpp_dump(tmp_path_factory):
+    """tests/objload_dump.cpp (the C++ loader of host/scene_io.hpp) built
+    with g++ for the host."""
+    import subprocess
+
+    from conftest import ROOT
+
+    d = tmp_path_factory.mktemp("cppobj")
+    exe = str(d / "objload_dump")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-o", exe,
+                    os.path.join(ROOT, "tests", "objload_dump.cpp")], check=True)
+    return exe
+
+
+def _cpp_load(exe, obj, out, W=1024, H=768):
+    import subprocess
+
+    from pm355.scenes import PM_CAMERA_DTYPE, PM_MODEL_DTYPE
+
+    os.makedirs(out, exist_ok=True)
+    r = subprocess.run([exe, obj, out, str(W), str(H)], capture_output=True, text=True)
+    if r.returncode != 0:
+        return r.stderr
+    rd = lambda n, dt: np.fromfile(os.path.join(out, n), dtype=dt)
+    return dict(tris9=rd("tris9.f32", np.float32).reshape(-1, 9), normals9=rd("normals9.f32", np.float32).reshape(-1, 9),
+                models=rd("models.bin", PM_MODEL_DTYPE), light=int(rd("light.i32", np.int32)[0]),
+                names=open(os.path.join(out, "names.txt")).read().split("\n")[:-1],
+                camera=rd("camera.bin", PM_CAMERA_DTYPE)[0])
+
+
+def _same(c, s):
+    assert c["names"] == s.names and c["light"] == s.light_model
+    assert np.array_equal(c["tris9"].view(np.uint32), s.tris9.view(np.uint32))
+    assert np.array_equal(c["normals9"].view(np.uint32), s.normals9.view(np.uint32))
+    assert c["models"].tobytes() == s.models.tobytes()
+
+
+def test_cpp_loader_equals_python_on_shipped_scene(cpp_dump, shipped, tmp_path):
+    c = _cpp_load(cpp_dump, OBJ, str(tmp_path))
+    _same(c, shipped)
+    for W, H in ((1024, 768), (96, 72)):
+        c = _cpp_load(cpp_dump, OBJ, str(tmp_path / f"c{W}"), W, H)
+        assert c["camera"].tobytes() == np.ascontiguousarray(reference_camera(W, H)).tobytes()
+
+
+@pytest.mark.parametrize("obj", [
+    "mtllib s.mtl\n" + _QUAD + "g a\nusemtl m1\nf 1//1 2//1 3//1 4//1\ng a\nf -4//-1 -3//-1 -2//-1\n"
+    "o light\nusemtl m2\nf 1/1/1 2/1/1 3/1/1\n",
+    "mtllib s.mtl\n" + _QUAD + "g x\nf 1//1 2//1 3//1\n#\nf 1//1 2//1 3//1\n#\nf 1//1 2//1 3//1\ng light\nf 1//1 2//1 3//1\n",
+    "mtllib s.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\n  \nv 5 5 5\nvn 0 0 1\ng light\nf 1//1 2//1 5//1\n",
+    "mtllib s.mtl\nv 0 0 0\nv 2 0 0\nv 0 0 2\ng light\nusemtl nosuch\nf 1 2 3\nf 1 3 2\n",
+    "mtllib s.mtl\r\nv 0 0 0\r\nv 1 0 0\r\nv 0 1 0\r\nvn 0 0 1\r\n\r\ng light\r\nusemtl m1\r\nf 1//1 2//1 3//1\r\n",
+])
+def test_cpp_loader_equals_python_rules(cpp_dump, tmp_path, obj):
+    path = _write(tmp_path, obj)
+    _same(_cpp_load(cpp_dump, path, str(tmp_path / "o")), load_obj(path))
+
+
+@pytest.mark.parametrize("obj", ["v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\nv 2 2 0\ng light\nf 1 2 3 4 5\n",
+                                 "v 0 0 0\nv 1 0 0\nv 0 1 0\ng light\nf 0 1 2\n",
+                                 "v 0 0 0\nv 1 0 0\nv 0 1 0\ng lamp\nf 1 2 3\n"])
+def test_cpp_loader_rejects_like_python(cpp_dump, tmp_path, obj):
+    path = _write(tmp_path, obj)
+    assert isinstance(_cpp_load(cpp_dump, path, str(tmp_path / "o")), str)
+    with pytest.raises(ValueError):
+        load_obj(path)
