@@ -8,28 +8,6 @@ namespace {
 
 using pmd::GridGeom;
 
-// Per-photon cell id (the sort key) and the per-cell histogram
-// (count_photons_per_cell, PhotonGrid.cu:58-68).
-__global__ __launch_bounds__(256) void k_grid_hash(const float* __restrict__ px, const float* __restrict__ py,
-                                                  const float* __restrict__ pz, uint32_t n, GridGeom G,
-                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ counts) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int x = pmd::cell_coord(px[i], G.mn[0], G.cell, G.g[0]);
-    int y = pmd::cell_coord(py[i], G.mn[1], G.cell, G.g[1]);
-    int z = pmd::cell_coord(pz[i], G.mn[2], G.cell, G.g[2]);
-    uint32_t c = pmd::cell_index(G, x, y, z);
-    keys[i] = c;
-    // Photons arrive in emission order, so a wave often holds runs of the
-    // same cell: one atomic per wave when all its lanes share a cell.
-    uint64_t same = __ballot(__shfl(c, 0, 64) == c);
-    if (same == __ballot(true)) {
-        if (__lane_id() == (uint32_t)__builtin_ctzll(same)) atomicAdd(&counts[c], (uint32_t)__popcll(same));
-    } else {
-        atomicAdd(&counts[c], 1u);
-    }
-}
-
 // Cell index of every photon (the hash of PhotonGrid.cu:152-174 without the
 // counting atomics: counts come from the sorted keys). The same pass copies
 // each photon into a 48-byte record (three float4), so the scatter below
@@ -184,9 +162,6 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     if (!(cell_size > 0.f) && desired <= 0)
         return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: need cell_size > 0 or desired_per_cell > 0");
 
-    const float* px = d_photons9;
-    const float* py = d_photons9 + n;
-    const float* pz = d_photons9 + 2 * (size_t)n;
     const uint32_t blocks = pm::div_up(n, 256);
 
     HostDims d;
@@ -194,19 +169,11 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
         return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
 
-    // scratch: keys, sorted keys, sorted order, radix work, max counter
+    // scratch: keys, sorted keys, sorted order, radix work, max counter, records
     size_t rs = pm::radix_sort_scratch_bytes(n);
     void* work = nullptr;
     pm_status st;
-    size_t need_counts = (size_t)d.n_cells * 4;
-    if (!(cell_size > 0.f)) {
-        // analyze_density (PhotonGrid.cu:92-141): count once at the reference
-        // size and shrink the cells when the fullest one exceeds 5 * desired.
-        HostDims d2;
-        float cell2 = reference_cell(n, aabb, std::max(1, desired / 5));
-        if (make_dims(aabb, cell2, layout, d2)) need_counts = std::max(need_counts, (size_t)d2.n_cells * 4);
-    }
-    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, need_counts, (size_t)n * 48});
+    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, (size_t)n * 48});
     if ((st = pm::scratch_reserve(ctx, total, &work)) != PM_OK) return st;
     pm::Carve cv{static_cast<char*>(work)};
     uint32_t* keys = cv.take<uint32_t>(n);
@@ -214,68 +181,80 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     uint32_t* order = cv.take<uint32_t>(n);
     void* rwork = cv.take<char>(rs);
     uint32_t* dmax = cv.take<uint32_t>(64);
-    uint32_t* tmp_counts = cv.take<uint32_t>(need_counts / 4);
     float4* rec = cv.take<float4>((size_t)n * 3);
 
+    // One build at dims d: keys + records, key sort, cell bounds, scatter.
+    auto build_once = [&](const HostDims& dd, pm_grid*& out_g) -> pm_status {
+        pm_grid* g = new pm_grid();
+        size_t bytes = pm::carve_size({(size_t)dd.n_cells * 4, (size_t)dd.n_cells * 4, (size_t)n * 36, (size_t)n * 4});
+        hipError_t e = hipMalloc(&g->mem, bytes);
+        if (e != hipSuccess) {
+            delete g;
+            return pm::hip_fail(ctx, e, "hipMalloc(grid)");
+        }
+        pm::Carve gc{static_cast<char*>(g->mem)};
+        uint32_t* starts = gc.take<uint32_t>(dd.n_cells);
+        uint32_t* sizes = gc.take<uint32_t>(dd.n_cells);
+        float* photons = gc.take<float>((size_t)n * 9);
+        uint32_t* perm = gc.take<uint32_t>(n);
+
+        GridGeom G = geom_of(aabb, dd, layout);
+        auto bail = [&](pm_status s) {
+            (void)hipFree(g->mem);
+            delete g;
+            return s;
+        };
+        pm_status bs;
+        hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
+        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_keys launch"));
+        if ((bs = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(dd.n_cells), rwork)) !=
+            PM_OK)
+            return bail(bs);
+        hipLaunchKernelGGL(k_cell_bounds, dim3(pm::div_up(dd.n_cells, 256)), dim3(256), 0, ctx->stream, skeys, n,
+                           dd.n_cells, starts, sizes);
+        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_cell_bounds launch"));
+        hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, rec, n, order, photons, perm);
+        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
+
+        pm_grid_desc& D = g->desc;
+        for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
+        D.cell_size = dd.cell;
+        for (int a = 0; a < 3; ++a) D.dims[a] = dd.g[a];
+        D.layout = layout;
+        D.n_cells = dd.n_cells;
+        D.n_photons = n;
+        D.d_cell_starts = starts;
+        D.d_cell_sizes = sizes;
+        D.d_photons = photons;
+        D.d_perm = perm;
+        out_g = g;
+        return PM_OK;
+    };
+
+    pm_grid* g = nullptr;
+    if ((st = build_once(d, g)) != PM_OK) return st;
     if (!(cell_size > 0.f)) {
-        GridGeom G = geom_of(aabb, d, layout);
-        if ((st = pm::fill_u32(ctx, tmp_counts, 0u, d.n_cells)) != PM_OK) return st;
+        // analyze_density (PhotonGrid.cu:92-141): at the reference size, the
+        // fullest cell (read from the built grid's sizes) decides whether the
+        // cells shrink to the size for desired / 5; then the grid is built
+        // again at that size.
         if ((st = pm::fill_u32(ctx, dmax, 0u, 1)) != PM_OK) return st;
-        hipLaunchKernelGGL(k_grid_hash, dim3(blocks), dim3(256), 0, ctx->stream, px, py, pz, n, G, keys, tmp_counts);
-        PM_CHECK_LAUNCH(ctx, "k_grid_hash");
         hipLaunchKernelGGL(k_max_u32, dim3(std::min<uint32_t>(pm::div_up(d.n_cells, 256), 4096)), dim3(256), 0,
-                           ctx->stream, tmp_counts, d.n_cells, dmax);
+                           ctx->stream, g->desc.d_cell_sizes, d.n_cells, dmax);
         PM_CHECK_LAUNCH(ctx, "k_max_u32");
         uint32_t mx = 0;
         PM_HIP(ctx, hipMemcpyAsync(&mx, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (mx > 5u * (uint32_t)desired) {
-            cell = reference_cell(n, aabb, desired / 5);
+            (void)hipFree(g->mem);
+            delete g;
+            g = nullptr;
+            cell = reference_cell(n, aabb, std::max(1, desired / 5));
             if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
                 return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
+            if ((st = build_once(d, g)) != PM_OK) return st;
         }
     }
-
-    pm_grid* g = new pm_grid();
-    size_t bytes = pm::carve_size({(size_t)d.n_cells * 4, (size_t)d.n_cells * 4, (size_t)n * 36, (size_t)n * 4});
-    hipError_t e = hipMalloc(&g->mem, bytes);
-    if (e != hipSuccess) {
-        delete g;
-        return pm::hip_fail(ctx, e, "hipMalloc(grid)");
-    }
-    pm::Carve gc{static_cast<char*>(g->mem)};
-    uint32_t* starts = gc.take<uint32_t>(d.n_cells);
-    uint32_t* sizes = gc.take<uint32_t>(d.n_cells);
-    float* photons = gc.take<float>((size_t)n * 9);
-    uint32_t* perm = gc.take<uint32_t>(n);
-
-    GridGeom G = geom_of(aabb, d, layout);
-    auto bail = [&](pm_status s) {
-        (void)hipFree(g->mem);
-        delete g;
-        return s;
-    };
-    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
-    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_keys launch"));
-    if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(d.n_cells), rwork)) != PM_OK)
-        return bail(st);
-    hipLaunchKernelGGL(k_cell_bounds, dim3(pm::div_up(d.n_cells, 256)), dim3(256), 0, ctx->stream, skeys, n,
-                       d.n_cells, starts, sizes);
-    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_cell_bounds launch"));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, rec, n, order, photons, perm);
-    if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
-
-    pm_grid_desc& D = g->desc;
-    for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
-    D.cell_size = d.cell;
-    for (int a = 0; a < 3; ++a) D.dims[a] = d.g[a];
-    D.layout = layout;
-    D.n_cells = d.n_cells;
-    D.n_photons = n;
-    D.d_cell_starts = starts;
-    D.d_cell_sizes = sizes;
-    D.d_photons = photons;
-    D.d_perm = perm;
     *out = g;
     return PM_OK;
 }
