@@ -697,7 +697,11 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
     }
     if (P.out_count && lane == 0) P.out_count[q] = m;
 
-    // Raytracer.cuh:110-138 over the kept photons, in walk order
+    // Raytracer.cuh:110-138 over the kept photons. Their keys queue in the
+    // wave's LDS buffer (free again); every 64 queued, one lane per photon
+    // loads its direction and power, and a batch's loads are consumed only
+    // when the next batch is taken (or after the walk), some walk steps
+    // later, so the walk never waits on them.
     const float alpha = 1.818f, beta = 1.953f;
     const float pi_f = 3.141592654f;
     float res0 = 0.f, res1 = 0.f, res2 = 0.f;
@@ -706,22 +710,67 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
         const float filter_r = __fmul_rn(rmax, rmax);
         const float denom = 1.f - expf(-beta);
         const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
-        auto add = [&](uint32_t idx, float d2) {
-            const float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
-            const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
-            if (cosNL > 0.f) {
-                const float d = sqrtf(d2);
-                const float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
-                const float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
-                res0 += P.power[idx] * w;
-                res1 += P.power[P.n + idx] * w;
-                res2 += P.power[2 * (size_t)P.n + idx] * w;
+        const __amdgpu_buffer_rsrc_t rdx = plane_rsrc(P.dir, P.n), rdy = plane_rsrc(P.dir + P.n, P.n),
+                                     rdz = plane_rsrc(P.dir + 2 * (size_t)P.n, P.n);
+        const __amdgpu_buffer_rsrc_t rpr = plane_rsrc(P.power, P.n), rpg = plane_rsrc(P.power + P.n, P.n),
+                                     rpb = plane_rsrc(P.power + 2 * (size_t)P.n, P.n);
+        bool bv = false;
+        float bd2 = 0.f, bx = 0.f, by = 0.f, bz = 0.f, br = 0.f, bg = 0.f, bb = 0.f;
+        auto consume = [&]() {
+            if (bv) {
+                const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -bx), __fmul_rn(ny, -by)), __fmul_rn(nz, -bz));
+                if (cosNL > 0.f) {
+                    const float d = sqrtf(bd2);
+                    const float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                    const float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                    res0 += br * w;
+                    res1 += bg * w;
+                    res2 += bb * w;
+                }
             }
         };
+        auto take = [&](uint32_t avail) {  // lanes < avail take queue entries 0..avail-1
+            bv = lane < avail;
+            const uint64_t key = bv ? L.buf[lane] : 0ull;
+            const uint32_t off = (uint32_t)key * 4u;
+            bd2 = __uint_as_float((uint32_t)(key >> 32));
+            if (bv) {
+                bx = bload(rdx, off);
+                by = bload(rdy, off);
+                bz = bload(rdz, off);
+                br = bload(rpr, off);
+                bg = bload(rpg, off);
+                bb = bload(rpb, off);
+            }
+        };
+        uint32_t qn = 0;
+        wave_sync();  // the buffer's previous contents are dead
         walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
-            if (d2a <= (va ? T : -1.0f) && make_key(d2a, sa) <= kstar) add(sa, d2a);
-            if (d2b <= (vb ? T : -1.0f) && make_key(d2b, sb) <= kstar) add(sb, d2b);
+            const bool sela = d2a <= (va ? T : -1.0f) && make_key(d2a, sa) <= kstar;
+            const bool selb = d2b <= (vb ? T : -1.0f) && make_key(d2b, sb) <= kstar;
+            const uint64_t ba = wballot(sela), bbal = wballot(selb);
+            const uint32_t na = (uint32_t)__popcll(ba);
+            if (sela) L.buf[qn + mbcnt(ba)] = make_key(d2a, sa);
+            if (selb) L.buf[qn + na + mbcnt(bbal)] = make_key(d2b, sb);
+            qn += na + (uint32_t)__popcll(bbal);  // < 64 + 128: the queue holds < 64 between steps
+            while (qn >= 64) {
+                wave_sync();
+                consume();
+                take(64);
+                // entries 64 .. qn-1 (fewer than 128) move to the front
+                const uint64_t r0 = lane + 64 < qn ? L.buf[64 + lane] : 0ull;
+                const uint64_t r1 = lane + 128 < qn ? L.buf[128 + lane] : 0ull;
+                wave_sync();
+                if (lane + 64 < qn) L.buf[lane] = r0;
+                if (lane + 128 < qn) L.buf[64 + lane] = r1;
+                qn -= 64;
+                wave_sync();
+            }
         });
+        wave_sync();
+        consume();
+        take(qn);
+        consume();
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
