@@ -984,7 +984,8 @@ __global__ __launch_bounds__(256) void k_query_keys(GridGeom G, const float* __r
 __global__ __launch_bounds__(256) void k_box_stats(GridGeom G, const uint32_t* __restrict__ starts,
                                                   const uint32_t* __restrict__ sizes, const float* __restrict__ qx,
                                                   const float* __restrict__ qy, const float* __restrict__ qz,
-                                                  uint32_t nq, int rc, float T, unsigned long long* __restrict__ out) {
+                                                  uint32_t nq, int rc, float T, int reference,
+                                                  unsigned long long* __restrict__ out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cells = 0, cands = 0;
     if (i < nq) {
@@ -1000,13 +1001,27 @@ __global__ __launch_bounds__(256) void k_box_stats(GridGeom G, const uint32_t* _
         const int nseg = G.layout == PM_LAYOUT_LINEAR ? ny * nz : nx * ny * nz;
         // (C_q stays the box's cell count in the linear layout, an upper
         // bound of the start / size words its row spans read)
-        for (int s = 0; s < nseg; ++s) {
-            uint32_t b, len;
-            const bool kept = row_span(G, starts, sizes, B, s, qv[0], qv[1], qv[2], T, b, len);
-            cands += len;
-            if (G.layout != PM_LAYOUT_LINEAR) cells += kept;
+        if (reference) {
+            // the reference's whole box (PhotonGrid.cu:262-272)
+            cells = (unsigned long long)nx * ny * nz;
+            for (int z = B.lo[2]; z <= B.hi[2]; ++z)
+                for (int y = B.lo[1]; y <= B.hi[1]; ++y) {
+                    if (G.layout == PM_LAYOUT_LINEAR) {
+                        const uint32_t c0 = pmd::cell_index(G, B.lo[0], y, z), c1 = c0 + (uint32_t)(nx - 1);
+                        cands += starts[c1] + sizes[c1] - starts[c0];
+                    } else {
+                        for (int x = B.lo[0]; x <= B.hi[0]; ++x) cands += sizes[pmd::cell_index(G, x, y, z)];
+                    }
+                }
+        } else {
+            for (int s = 0; s < nseg; ++s) {
+                uint32_t b, len;
+                const bool kept = row_span(G, starts, sizes, B, s, qv[0], qv[1], qv[2], T, b, len);
+                cands += len;
+                if (G.layout != PM_LAYOUT_LINEAR) cells += kept;
+            }
+            if (G.layout == PM_LAYOUT_LINEAR) cells = (unsigned long long)nx * ny * nz;
         }
-        if (G.layout == PM_LAYOUT_LINEAR) cells = (unsigned long long)nx * ny * nz;
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1213,7 +1228,7 @@ extern "C" pm_status pm_radiance(pm_ctx* ctx, const pm_grid* grid, const uint32_
 }
 
 extern "C" pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
-                                       float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands) {
+                                       float radius, uint32_t flags, uint64_t* h_sum_cells, uint64_t* h_sum_cands) {
     if (!ctx) return PM_ERR_INVALID;
     KnnParams P;
     pm_status st = setup(ctx, grid, d_queries3, nq, radius, 1, 0, P, "pm_grid_box_stats");
@@ -1224,7 +1239,7 @@ extern "C" pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const f
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, ctx->stream, acc);
     if (nq > 0)
         hipLaunchKernelGGL(k_box_stats, dim3(pm::div_up(nq, 256)), dim3(256), 0, ctx->stream, P.G, P.starts, P.sizes,
-                           P.qx, P.qy, P.qz, nq, P.rc, P.T, acc);
+                           P.qx, P.qy, P.qz, nq, P.rc, P.T, (flags & PM_BOX_REFERENCE) ? 1 : 0, acc);
     PM_CHECK_LAUNCH(ctx, "k_box_stats");
     unsigned long long h[2] = {0, 0};
     PM_HIP(ctx, hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

@@ -33,6 +33,7 @@ KNN_TWO_PASS = 1 << 4
 KNN_LARGE_BUFFER = 1 << 5
 KNN_BATCH_QUEUE = 1 << 6
 KNN_XCD_CONTIGUOUS = 1 << 7
+BOX_REFERENCE = 1
 # gather variants (tuning flags): the default is the 512-key buffer with
 # XCD-interleaved chunks of workgroups
 KNN_VARIANTS = {"stream512_chunked": 0, "stream512": KNN_XCD_CONTIGUOUS, "stream1024": KNN_LARGE_BUFFER,
@@ -81,7 +82,7 @@ _SIGS = {
     "pm_radiance": (_I32, [_P, _P, _P, _P, _P, _P, _U32, _I32, _P]),
     "pm_grid_gather_radiance": (_I32, [_P, _P, _P, _P, _U32, _F32, _I32, _U32, _P, _P]),
     "pm_grid_query_order": (_I32, [_P, _P, _P, _U32, _P]),
-    "pm_grid_box_stats": (_I32, [_P, _P, _P, _U32, _F32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pm_grid_box_stats": (_I32, [_P, _P, _P, _U32, _F32, _U32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "pm_kdtree_build": (_I32, [_P, _P, _U32, C.POINTER(_P)]),
     "pm_kdtree_describe": (_I32, [_P, _P]),
     "pm_kdtree_copy_to_host": (_I32, [_P, _P, _P, _P]),
@@ -452,12 +453,14 @@ class Context:
     def server(self, budget_bytes: int = 0) -> "Server":
         return Server(self, budget_bytes)
 
-    def box_stats(self, grid: "Grid", queries3, radius: float):
+    def box_stats(self, grid: "Grid", queries3, radius: float, reference: bool = False):
+        """(sum C_q, sum P_q): the gather's visited cells / candidates, or with
+        reference=True the reference's whole candidate box."""
         q = _f32_dev(queries3, self.device)
         c = C.c_uint64()
         p = C.c_uint64()
-        self.check(self.lib.pm_grid_box_stats(self.h, grid.h, _ptr(q), q.shape[1], float(radius), C.byref(c),
-                                              C.byref(p)))
+        self.check(self.lib.pm_grid_box_stats(self.h, grid.h, _ptr(q), q.shape[1], float(radius),
+                                              BOX_REFERENCE if reference else 0, C.byref(c), C.byref(p)))
         return int(c.value), int(p.value)
 
 

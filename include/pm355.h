@@ -176,14 +176,15 @@ typedef struct {
 
 pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* h_jobs, uint32_t n_jobs, uint32_t flags);
 
-/* Per-query candidate statistics of the gather (cells C_q whose bounds it
- * reads and photons P_q it tests, summed), used for the algorithmic-byte
- * count. The gather visits the reference's box (cell +- ceil(r/cell),
- * PhotonGrid.cu:262-272) pruned to the rows (linear layout) or cells (Morton)
- * that can hold a photon within r; C_q of the linear layout is the box's
- * cell count. */
+/* Per-query candidate statistics, summed: cells C_q and photons P_q.
+ * flags = PM_BOX_REFERENCE: the reference's candidate box (cell +- ceil(r/cell),
+ * clamped, PhotonGrid.cu:262-272), the C_q / P_q of the algorithmic-byte
+ * count. flags = 0: what the gather visits, that box pruned to the rows
+ * (linear layout) or cells (Morton) that can hold a photon within r (C_q of
+ * the linear layout stays the box's cell count). */
+enum { PM_BOX_REFERENCE = 1u };
 pm_status pm_grid_box_stats(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq,
-                            float radius, uint64_t* h_sum_cells, uint64_t* h_sum_cands);
+                            float radius, uint32_t flags, uint64_t* h_sum_cells, uint64_t* h_sum_cands);
 
 /* ---- photon kd-tree: cpm::PhotonMap(Type, Photon*, size_t) ->
  * PhotonMap::fill_balanced (PhotonMap.cuh:189-283, 373-375). Same split

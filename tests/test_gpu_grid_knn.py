@@ -251,6 +251,8 @@ def test_box_stats_bracket_the_oracle(ctx, layout, cell):
     og.knn(q, 0.01, 64)
     cells, cands = ctx.box_stats(g, q, 0.01)
     st = og.last_stats
+    # the reference box itself equals the oracle's counting pass exactly
+    assert ctx.box_stats(g, q, 0.01, reference=True) == (st["cells"], st["cands"])
     assert cells <= st["cells"] and (layout == 1 or cells == st["cells"])
     assert st["found"] <= cands <= st["cands"]
     if cell < 0.01:
