@@ -338,13 +338,70 @@ __device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds<CAP>& 
     if (P.rgb) radiance_epilogue<R>(P, v, m, lane, q);
 }
 
+// Radiance estimate (Raytracer.cuh:110-138) over the m kept keys left in
+// the LDS buffer, unsorted: the filter radius is their largest distance and
+// the sum runs in buffer order (lane-strided, then a wave reduction).
+template <int CAP>
+__device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLds<CAP>& L, uint32_t lane, uint32_t q,
+                                                uint32_t m) {
+    const float alpha = 1.818f, beta = 1.953f;
+    const float pi_f = 3.141592654f;
+    float dmax = 0.f;
+    for (uint32_t e = lane; e < m; e += 64) dmax = fmaxf(dmax, __uint_as_float((uint32_t)(L.buf[e] >> 32)));
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, d, 64));
+    const float rmax = sqrtf(dmax);
+    float res0 = 0.f, res1 = 0.f, res2 = 0.f;
+    if (m > 0) {
+        const float filter_r = __fmul_rn(rmax, rmax);
+        const float denom = 1.f - expf(-beta);
+        const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
+        for (uint32_t e = lane; e < m; e += 64) {
+            const uint64_t v = L.buf[e];
+            const uint32_t idx = (uint32_t)v;
+            const float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+            const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
+            if (cosNL > 0.f) {
+                const float d = sqrtf(__uint_as_float((uint32_t)(v >> 32)));
+                const float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
+                const float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                res0 += P.power[idx] * w;
+                res1 += P.power[P.n + idx] * w;
+                res2 += P.power[2 * (size_t)P.n + idx] * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        res0 += __shfl_xor(res0, d, 64);
+        res1 += __shfl_xor(res1, d, 64);
+        res2 += __shfl_xor(res2, d, 64);
+    }
+    if (lane == 0) {
+        float coef = 1.f;
+        if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
+        P.rgb[q] = m > 0 ? res0 * coef : 0.f;
+        P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
+        P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
+    }
+}
+
 template <int CAP>
 __device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q, uint32_t cnt,
                                        uint32_t found, uint64_t khi) {
     const uint32_t k = (uint32_t)P.k;
     const uint32_t m = found < k ? found : k;
-    // the sort holds at most 256 keys
-    cnt = buf_select<CAP>(L, lane, cnt, k, 256u, P.bT, khi);
+    if (!P.out_idx) {
+        // no neighbour lists: narrow to exactly the k smallest and estimate
+        // from the buffer without sorting
+        cnt = buf_select<CAP>(L, lane, cnt, k, k, P.bT, khi);
+        wave_sync();
+        if (P.out_count && lane == 0) P.out_count[q] = m;
+        if (P.rgb) radiance_buffer<CAP>(P, L, lane, q, m);
+        return;
+    }
+    // narrowed to the smallest sort that holds k keys (64, 128 or 256)
+    cnt = buf_select<CAP>(L, lane, cnt, k, k <= 64 ? 64u : k <= 128 ? 128u : 256u, P.bT, khi);
     wave_sync();
     if (cnt <= 64)
         sort_and_emit<1, CAP>(P, L, lane, q, cnt, m);
