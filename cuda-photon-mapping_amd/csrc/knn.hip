@@ -545,6 +545,11 @@ __device__ __forceinline__ void large_one(const KnnParams& P, WaveLds<CAP>& L, u
     const uint32_t m = found < k ? found : k;
     cnt = buf_select<CAP>(L, lane, cnt, k, k, P.bT, khi);  // exactly min(k, found) keys remain
     wave_sync();
+    if (!P.out_idx) {  // radiance / counts only: no sort needed
+        if (P.out_count && lane == 0) P.out_count[q] = m;
+        if (P.rgb) radiance_buffer<CAP>(P, L, lane, q, m);
+        return;
+    }
     uint32_t n2 = 64;
     while (n2 < cnt) n2 <<= 1;
     for (uint32_t e = cnt + lane; e < n2; e += 64) L.buf[e] = ~0ull;
