@@ -181,6 +181,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         return _lib
     if not os.path.exists(path):
         raise ImportError(f"libpm355.so not built at {path}; run __graft_entry__.build()")
+    # torch first: its HIP runtime must be the process's one before the
+    # library binds to a HIP runtime of its own
+    import torch  # noqa: F401
+
     lib = C.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
