@@ -1168,7 +1168,9 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
     if (P.nq == 0) return PM_OK;
     // radiance (and counts) without neighbour lists for k > 256: the
     // three-walk kernel, with an overflow list behind it
-    const bool wide = P.k > 256 && !P.out_idx && P.rgb;
+    // (256 < k <= 512 radiance-only: one walk with the 1024-key buffer)
+    const bool mid = P.k > 256 && P.k <= 512 && !P.out_idx && P.rgb;
+    const bool wide = P.k > 512 && !P.out_idx && P.rgb;
     const bool sort = (flags & PM_KNN_SORT_QUERIES) != 0;
     uint32_t* ovf_list = nullptr;
     uint32_t* ovf_count = nullptr;
@@ -1208,6 +1210,13 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
         else
             hipLaunchKernelGGL(k_knn_large_list<2048>, dim3(lblocks), dim3(64), 0, ctx->stream, P, ovf_list, ovf_count);
         PM_CHECK_LAUNCH(ctx, "k_knn_large_list");
+        return PM_OK;
+    }
+    if (mid) {
+        uint32_t nblk = pm::div_up(P.nq, kWaves);
+        if (P.remap == 2) nblk = pm::div_up(nblk, 8 * kXcdChunk) * 8 * kXcdChunk;
+        hipLaunchKernelGGL(k_knn_stream1024, dim3(nblk), dim3(256), 0, ctx->stream, P);
+        PM_CHECK_LAUNCH(ctx, "k_knn_stream1024");
         return PM_OK;
     }
     if (P.k > 1024) {
