@@ -623,8 +623,8 @@ def main():
     order_ms = timed(lambda: ctx.query_order(g, d_q), reps)
     gather_ms = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags), reps)
 
-    # Algorithmic bytes of one fused gather launch, the per-query figure of
-    # the measurement plan: 12 (query) + 8*C_q (start + size) + 12*P_q
+    # Algorithmic bytes of one fused gather launch, per query:
+    # 12 (query) + 8*C_q (start + size) + 12*P_q
     # (candidate positions) + 8*kept (index + distance) + 24*kept (power +
     # direction of the kept photons) + 12 (normal) + 12 (RGB out), with C_q
     # and P_q the cells and photons of the reference's candidate box at the

@@ -37,7 +37,7 @@ enum {
     PM_ERR_HIP = -2,         /* HIP runtime error */
     PM_ERR_NOMEM = -3,       /* device allocation failed */
     PM_ERR_CAPACITY = -4,    /* an output or internal capacity was exceeded */
-    PM_ERR_UNSUPPORTED = -5, /* parameter outside what this build supports */
+    PM_ERR_UNSUPPORTED = -5, /* parameter outside the supported range */
     PM_ERR_NO_DEVICE = -6    /* no usable gfx950 device */
 };
 
