@@ -73,3 +73,46 @@ def test_gather_and_scan_replay_from_graph():
         del graph
         g.close()
         ctx.close()
+
+
+@pytest.mark.parametrize("k,r", [(400, 0.05), (2000, 0.08)])
+def test_large_k_gather_replays_from_graph(k, r):
+    """Radiance gathers above k = 256 (the single-walk 1024-key kernel, and
+    the three-walk kernel with its overflow list, whose counter a fill kernel
+    clears) replay from a captured graph with the eager results; a cluster
+    of exact duplicates sends queries through the overflow list."""
+    import torch
+
+    import pm355
+    from pm355 import _ptr
+    from pm355.workloads import gather_microbench
+
+    ph, q, qn, aabb = gather_microbench(seed=k, n_photons=300_000, n_queries=3_000, n_clusters=8)
+    ph[0:3, 1000:2500] = ph[0:3, -1:]
+    q[:, 0] = ph[0:3, -1]
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ctx = pm355.Context(0)
+        lib = ctx.lib
+        g = ctx.grid_build(ph, aabb, cell_size=r / 2)
+        dq = torch.from_numpy(q).cuda()
+        dn = torch.from_numpy(qn).cuda()
+        nq = dq.shape[1]
+        ref = ctx.gather_radiance(g, dq, dn, r, k, flags=pm355.KNN_SORT_QUERIES)
+        s.synchronize()
+        rgb = torch.zeros_like(ref)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=s):
+            rc = lib.pm_grid_gather_radiance(ctx.h, g.h, _ptr(dq), _ptr(dn), nq, C.c_float(r), k,
+                                             pm355.KNN_SORT_QUERIES, _ptr(rgb), None)
+        assert rc == pm355.PM_OK
+        for rep in range(3):
+            rgb.fill_(-1.0)
+            other = ctx.gather_radiance(g, dq, dn, r, k, flags=pm355.KNN_SORT_QUERIES)
+            graph.replay()
+            s.synchronize()
+            assert torch.equal(other.nan_to_num(7.0), ref.nan_to_num(7.0)), rep
+            assert torch.equal(rgb.nan_to_num(7.0), ref.nan_to_num(7.0)), f"replay {rep} differs from eager"
+        del graph
+        g.close()
+        ctx.close()
