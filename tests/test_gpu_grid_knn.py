@@ -290,3 +290,46 @@ def test_full_size_gather_properties(ctx):
     assert np.array_equal(gc_h[sel], oc)
     assert np.array_equal(gi_h, oi)
     assert np.array_equal(gd_h[sel].view(np.uint32), od.view(np.uint32))
+
+
+@pytest.mark.parametrize("layout,cell", [(0, 0.005), (0, 0.01), (1, 0.005)])
+def test_knn_parity_across_clamped_boundary(ctx, layout, cell):
+    """Photons and queries just outside the grid's AABB: the boundary cells
+    hold everything clamped into them, so the walk's pruning must treat them
+    as unbounded on the outer side (photons within r across the boundary are
+    still found, as the oracle finds them)."""
+    rng = np.random.default_rng(21)
+    n = 200_000
+    ph = np.zeros((9, n), np.float32)
+    ph[0:3] = rng.random((3, n), dtype=np.float32)
+    m = 40_000  # a shell just outside the box on every face
+    face = rng.integers(0, 3, m)
+    side = rng.integers(0, 2, m)
+    off = rng.uniform(0.0, 0.012, m).astype(np.float32)
+    for a in range(3):
+        sel = face == a
+        ph[a, :m][sel] = np.where(side[sel] == 0, -off[sel], 1.0 + off[sel])
+    ph[3:6] = 1e-6
+    ph[6:9] = np.array([0, -1, 0], np.float32)[:, None]
+    nq = 4000
+    q = rng.random((3, nq), dtype=np.float32)
+    qa = rng.integers(0, 3, nq)
+    qs = rng.integers(0, 2, nq)
+    qo = rng.uniform(-0.01, 0.01, nq).astype(np.float32)  # inside or outside, near a face
+    for a in range(3):
+        sel = qa == a
+        q[a][sel] = np.where(qs[sel] == 0, qo[sel], 1.0 + qo[sel])
+    qn = np.zeros_like(q)
+    qn[1] = 1.0
+    aabb = (0, 0, 0, 1, 1, 1)
+    g = ctx.grid_build(ph, aabb, cell_size=cell, layout=layout)
+    og = O.OracleGrid(ph, aabb, cell_size=cell, layout=layout)
+    for k in (16, 64):
+        oi, od, oc = og.knn(q, 0.01, k)
+        gi, gd, gc = ctx.knn(g, q, 0.01, k)
+        ctx.sync()
+        _compare_knn(gi, gd, gc, oi, od, oc)
+        assert oc.sum() > nq  # the shell is found
+    cells, cands = ctx.box_stats(g, q, 0.01)
+    og.knn(q, 0.01, 64)
+    assert og.last_stats["found"] <= cands <= og.last_stats["cands"]
