@@ -289,3 +289,96 @@ def test_cpp_loader_rejects_like_python(cpp_dump, tmp_path, obj):
     assert isinstance(_cpp_load(cpp_dump, path, str(tmp_path / "o")), str)
     with pytest.raises(ValueError):
         load_obj(path)
+
+
+def _random_obj(rng, path):
+    """A random OBJ/MTL pair and the scene it must load as: groups of
+    triangles and quads in the v//vn, v/vt/vn and negative-index forms,
+    vertices declared between groups, materials with every keyword."""
+    n_mat = int(rng.integers(2, 5))
+    mats = []
+    mtl = []
+    for i in range(n_mat):
+        m = dict(diffuse=rng.random(3).astype(np.float32), specular=rng.random(3).astype(np.float32),
+                 emission=np.zeros(3, np.float32), shininess=np.float32(rng.integers(0, 64)),
+                 refr_index=np.float32(1.0), opaque=np.float32(1.0))
+        lines = [f"newmtl mat{i}", "Kd %r %r %r" % tuple(float(x) for x in m["diffuse"]),
+                 "Ks %r %r %r" % tuple(float(x) for x in m["specular"]), f"Ns {int(m['shininess'])}"]
+        if rng.random() < 0.5:
+            m["refr_index"] = np.float32(1.5)
+            lines.append("Ni 1.5")
+        if rng.random() < 0.5:
+            tr = np.float32(rng.integers(1, 9) / 10)
+            m["opaque"] = np.float32(np.float32(1.0) - tr)
+            lines.append(f"Tr {float(tr)!r}")
+        mats.append(m)
+        mtl += lines + [""]
+    light = dict(diffuse=np.full(3, 0.5, np.float32), specular=np.ones(3, np.float32),
+                 emission=np.full(3, 4.0, np.float32), shininess=np.float32(16), refr_index=np.float32(1),
+                 opaque=np.float32(1))
+    mtl += ["newmtl lamp", "Kd 0.5 0.5 0.5", "Ke 4 4 4"]
+    obj = ["mtllib r.mtl"]
+    pos, nrm = [], []
+    tris, nrms, models, names = [], [], [], []
+    n_groups = int(rng.integers(2, 6))
+    for gi in range(n_groups + 1):
+        is_light = gi == n_groups
+        nv = int(rng.integers(4, 12))
+        base = len(pos)
+        for _ in range(nv):
+            v = (rng.random(3) * 4 - 2).astype(np.float32)
+            pos.append(v)
+            obj.append("v %r %r %r" % tuple(float(x) for x in v))
+        nb = len(nrm)
+        nn = rng.standard_normal((2, 3))
+        for v in nn:
+            v = (v / np.linalg.norm(v)).astype(np.float32)
+            nrm.append(v)
+            obj.append("vn %r %r %r" % tuple(float(x) for x in v))
+        name = "light" if is_light else f"grp{gi}"
+        obj.append(f"g {name}")
+        mi = int(rng.integers(0, n_mat))
+        obj.append("usemtl lamp" if is_light else f"usemtl mat{mi}")
+        t0 = len(tris)
+        for _ in range(int(rng.integers(1, 6))):
+            quad = rng.random() < 0.4
+            idx = rng.choice(nv, 4 if quad else 3, replace=False) + base
+            ni = [int(rng.integers(0, 2)) + nb for _ in idx]
+            form = int(rng.integers(0, 3))
+            toks = []
+            for p_i, n_i in zip(idx, ni):
+                if form == 0:
+                    toks.append(f"{p_i + 1}//{n_i + 1}")
+                elif form == 1:
+                    toks.append(f"{p_i + 1}/1/{n_i + 1}")
+                else:  # negative: counted back from the vertices read so far
+                    toks.append(f"{p_i - len(pos)}//{n_i - len(nrm)}")
+            obj.append("f " + " ".join(toks))
+            order = [(0, 1, 3), (1, 2, 3)] if quad else [(0, 1, 2)]
+            for a, b, c in order:
+                tris.append(np.concatenate([pos[idx[a]], pos[idx[b]], pos[idx[c]]]))
+                nrms.append(np.concatenate([nrm[ni[a]], nrm[ni[b]], nrm[ni[c]]]))
+        m = light if is_light else mats[mi]
+        models.append((t0, len(tris), m))
+        names.append(name)
+        obj.append("")
+    with open(os.path.join(path, "r.mtl"), "w") as f:
+        f.write("\n".join(mtl) + "\n")
+    with open(os.path.join(path, "r.obj"), "w") as f:
+        f.write("\n".join(obj) + "\n")
+    return np.array(tris, np.float32), np.array(nrms, np.float32), models, names
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_generated_obj_loads_as_built(cpp_dump, tmp_path, seed):
+    tris, nrms, models, names = _random_obj(np.random.default_rng(seed), str(tmp_path))
+    s = load_obj(str(tmp_path / "r.obj"))
+    assert s.names == names and s.light_model == len(names) - 1
+    assert np.array_equal(s.tris9, tris) and np.array_equal(s.normals9, nrms)
+    for rec, (t0, t1, m) in zip(s.models, models):
+        assert (rec["tri_begin"], rec["tri_end"]) == (t0, t1)
+        assert np.array_equal(rec["kd"], m["diffuse"]) and np.array_equal(rec["ks"], m["specular"])
+        assert np.array_equal(rec["ke"], m["emission"])
+        assert rec["shininess"] == m["shininess"] and rec["refr_index"] == m["refr_index"]
+        assert rec["opaque"] == m["opaque"]
+    _same(_cpp_load(cpp_dump, str(tmp_path / "r.obj"), str(tmp_path / "o")), s)
