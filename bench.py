@@ -717,9 +717,19 @@ def main():
         gr.close()
         del qs, ns
 
-    # kd-tree build over the same 16M photons (PhotonMap), HIP events
+    # kd-tree build over the same 16M photons (PhotonMap), HIP events, and
+    # the kd-tree's own kNN query (PhotonMap::locate_q) over the same 4M
+    # queries in cell order: neighbour lists (index + distance), k, r as above
     kd_ms = timed(lambda: ctx.kdtree_build(d_ph).close(), 3)
     res["components_ms"]["kdtree_build_16M"] = kd_ms
+    kt = ctx.kdtree_build(d_ph)
+    kd_knn_ms = timed(lambda: ctx.kdtree_knn(kt, d_qs, r, k), 2)
+    grid_knn_ms = timed(lambda: ctx.knn(g, d_qs, r, k, flags=vflags), 2)
+    kt.close()
+    res["components_ms"]["kdtree_knn_4M"] = kd_knn_ms
+    res["components_ms"]["grid_knn_4M"] = grid_knn_ms
+    res["kdtree_knn_queries_per_s"] = nq / (kd_knn_ms * 1e-3)
+    res["grid_knn_queries_per_s"] = nq / (grid_knn_ms * 1e-3)
 
     # Measured copy peak (a STREAM-like device copy of 1 GiB: bytes read +
     # written per second) beside the published 8 TB/s, and the build
