@@ -46,9 +46,12 @@ struct KdParams {
     uint32_t* out_count;
 };
 
+constexpr int kLdsStack = 12;  // top of each lane's depth-first stack, lane-interleaved in LDS
+
 struct KdLds {
     WaveLds<kCap> sel;
     uint32_t frontier[2][64];
+    uint32_t stk[kLdsStack][64];
 };
 
 __device__ __forceinline__ float kd_dist2(float px, float py, float pz, float qx, float qy, float qz) {
@@ -129,22 +132,34 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
         cur ^= 1;
         wave_sync();
     }
-    // Depth-first: lane l walks the subtree rooted at frontier[l].
-    uint32_t stack[kStack];
+    // Depth-first: lane l walks the subtree rooted at frontier[l]. The top
+    // kLdsStack entries of its stack live in LDS (one column per lane, no
+    // bank conflicts), deeper ones in a per-lane array.
+    uint32_t deep[kStack - kLdsStack];
     int sp = 0;
-    if (lane < fsize) stack[sp++] = L.frontier[cur][lane];
+    auto push = [&](uint32_t v) {
+        if (sp < kLdsStack)
+            L.stk[sp][lane] = v;
+        else
+            deep[sp - kLdsStack] = v;
+        ++sp;
+    };
+    if (lane < fsize) push(L.frontier[cur][lane]);
     for (;;) {
         if (!__builtin_amdgcn_ballot_w64(sp > 0)) break;
         uint32_t node = P.cap;
-        if (sp > 0) node = stack[--sp];
+        if (sp > 0) {
+            --sp;
+            node = sp < kLdsStack ? L.stk[sp][lane] : deep[sp - kLdsStack];
+        }
         bool valid, gl, gr;
         uint32_t pid = 0;
         float d2 = 0.f;
         visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
         feed(valid, pid, d2);
         // push far child first so the near child is visited next
-        if (gr && sp < kStack) stack[sp++] = 2 * node + 2;
-        if (gl && sp < kStack) stack[sp++] = 2 * node + 1;
+        if (gr && sp < kStack) push(2 * node + 2);
+        if (gl && sp < kStack) push(2 * node + 1);
     }
     wave_sync();
 
