@@ -148,6 +148,54 @@ GridGeom geom_of(const float aabb[6], const HostDims& d, int layout) {
     return G;
 }
 
+// A grid over no photons (a trace that stored nothing): every cell is
+// empty, so every gather finds nothing and every estimate is zero. The
+// reference's sizing rule divides by ceil(0 / desired) = 0 there
+// (PhotonGrid.cu:41-56); without a given cell size the grid is one cell.
+pm_status build_empty(pm_ctx* ctx, const float aabb[6], float cell_size, int layout, pm_grid** out) {
+    float cell = cell_size;
+    if (!(cell > 0.f)) {
+        cell = 0.f;
+        for (int a = 0; a < 3; ++a) cell = std::fmax(cell, std::fabs(aabb[a + 3] - aabb[a]));
+        if (!(cell > 0.f) || !std::isfinite(cell)) cell = 1.f;
+    }
+    HostDims d;
+    if (!make_dims(aabb, cell, layout, d))
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
+    pm_grid* g = new pm_grid();
+    // the photon planes and the permutation are empty; they point at a
+    // zeroed 256-byte block so no pointer is null
+    hipError_t e = hipMalloc(&g->mem, pm::carve_size({(size_t)d.n_cells * 4, (size_t)d.n_cells * 4, 256}));
+    if (e != hipSuccess) {
+        delete g;
+        return pm::hip_fail(ctx, e, "hipMalloc(grid)");
+    }
+    pm::Carve gc{static_cast<char*>(g->mem)};
+    uint32_t* starts = gc.take<uint32_t>(d.n_cells);
+    uint32_t* sizes = gc.take<uint32_t>(d.n_cells);
+    uint32_t* nothing = gc.take<uint32_t>(64);
+    pm_status st;
+    if ((st = pm::fill_u32(ctx, starts, 0u, d.n_cells)) != PM_OK ||
+        (st = pm::fill_u32(ctx, sizes, 0u, d.n_cells)) != PM_OK || (st = pm::fill_u32(ctx, nothing, 0u, 64)) != PM_OK) {
+        (void)hipFree(g->mem);
+        delete g;
+        return st;
+    }
+    pm_grid_desc& D = g->desc;
+    for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
+    D.cell_size = d.cell;
+    for (int a = 0; a < 3; ++a) D.dims[a] = d.g[a];
+    D.layout = layout;
+    D.n_cells = d.n_cells;
+    D.n_photons = 0;
+    D.d_cell_starts = starts;
+    D.d_cell_sizes = sizes;
+    D.d_photons = reinterpret_cast<const float*>(nothing);
+    D.d_perm = nothing;
+    *out = g;
+    return PM_OK;
+}
+
 }  // namespace
 
 extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
@@ -157,11 +205,11 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     if (!aabb || (n > 0 && !d_photons9)) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: null argument");
     if (layout != PM_LAYOUT_LINEAR && layout != PM_LAYOUT_MORTON)
         return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: bad layout %d", layout);
-    if (n == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: no photons to build grid");
     if (n > 0x7FFFFFFFu) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: n too large");
     if (!(cell_size > 0.f) && desired <= 0)
         return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: need cell_size > 0 or desired_per_cell > 0");
 
+    if (n == 0) return build_empty(ctx, aabb, cell_size, layout, out);
     const uint32_t blocks = pm::div_up(n, 256);
 
     HostDims d;

@@ -157,7 +157,8 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
         float d2 = 0.f;
         visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
         feed(valid, pid, d2);
-        // push far child first so the near child is visited next
+        // right child below the left one, so the left subtree is walked
+        // first; the order does not change the result (pruning uses r only)
         if (gr && sp < kStack) push(2 * node + 2);
         if (gl && sp < kStack) push(2 * node + 1);
     }

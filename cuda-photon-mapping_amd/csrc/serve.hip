@@ -163,7 +163,7 @@ extern "C" pm_status pm_server_step(pm_server* S, uint32_t* frames_done) {
         if (j.gg) pm_grid_destroy(j.gg);
         if (j.gc) pm_grid_destroy(j.gc);
         j.gg = j.gc = nullptr;
-        if ((st = pm_grid_build(ctx, j.g9, std::max(1u, ts.n_global), d.aabb, d.cell_size, 1000, PM_LAYOUT_LINEAR,
+        if ((st = pm_grid_build(ctx, j.g9, ts.n_global, d.aabb, d.cell_size, 1000, PM_LAYOUT_LINEAR,
                                 &j.gg)) != PM_OK)
             return st;
         if (ts.n_caustic > 0 &&

@@ -57,6 +57,16 @@ static int render_main(int argc, char** argv) {
         ctx.sync();
         const std::vector<uint8_t> h8 = rgb8.to_host();
         const std::vector<float> g9 = maps.global9.to_host(), c9 = maps.caustic9.to_host();
+        // persist both maps in the reference's photon-map file format and
+        // load the global one back (Initializer.cu:102-141)
+        pm355::save_photons(ctx, out + "/global_map.bin", maps.global9.data(), maps.stats.n_global);
+        pm355::save_photons(ctx, out + "/caustic_map.bin", maps.caustic9.data(), maps.stats.n_caustic);
+        uint32_t n_back = 0;
+        const pm355::DeviceBuffer<float> back = pm355::load_photons(ctx, out + "/global_map.bin", n_back);
+        if (n_back != maps.stats.n_global || back.to_host() != g9) {
+            std::fprintf(stderr, "photon file round trip differs\n");
+            return 1;
+        }
         write_file(out + "/global9.f32", g9.data(), g9.size() * 4);
         write_file(out + "/caustic9.f32", c9.data(), c9.size() * 4);
         write_file(out + "/rgb8.u8", h8.data(), h8.size());

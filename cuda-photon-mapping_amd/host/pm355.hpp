@@ -11,6 +11,8 @@
 //                                             -> pm355::trace_photons
 //   Raytracer::render_gpu + RaytracePlanner (Raytracer.cuh:245,
 //        RaytracePlanner.cu:18)              -> pm355::render_frame
+//   load (Initializer.cu:102-141)            -> pm355::load_photons /
+//                                               save_photons
 //   loadOBJ / loadMaterial (ObjLoader.cpp:4-306), Camera (Camera.cuh:43-66)
 //        and the GL canvas (Window.cuh)      -> scene_io.hpp: load_obj,
 //                                               make_camera, write_ppm;
@@ -280,6 +282,21 @@ inline PhotonArrays trace_photons(Context& ctx, const pm_scene& scene, uint32_t 
     ctx.check(pm_trace_photons(ctx.get(), &scene, seed, diffuse_num, specular_num, max_bounces, max_paths,
                                a.global9.data(), a.caustic9.data(), &a.stats));
     return a;
+}
+
+// load (Initializer.cu:102-141): a photon-map file (u64 count, 36-byte
+// records pos/power/inc_dir) into a device photons9 block; save_photons is
+// its writer, for persisting resident maps.
+inline DeviceBuffer<float> load_photons(Context& ctx, const std::string& path, uint32_t& count) {
+    uint64_t n = 0;
+    ctx.check(pm_photon_file_count(ctx.get(), path.c_str(), &n));
+    if (n > 0xFFFFFFFFull) throw Error(PM_ERR_UNSUPPORTED, "photon file too large: " + path);
+    DeviceBuffer<float> block((size_t)n * 9);
+    ctx.check(pm_photon_file_read(ctx.get(), path.c_str(), block.data(), (uint32_t)n, &count));
+    return block;
+}
+inline void save_photons(Context& ctx, const std::string& path, const float* d_photons9, uint32_t count) {
+    ctx.check(pm_photon_file_write(ctx.get(), path.c_str(), d_photons9, count));
 }
 
 // Raytracer::render_gpu with the RaytracePlanner (Raytracer.cuh:245-279).

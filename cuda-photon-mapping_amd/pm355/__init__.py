@@ -105,6 +105,9 @@ _SIGS = {
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
+    "pm_photon_file_count": (_I32, [_P, C.c_char_p, C.POINTER(C.c_uint64)]),
+    "pm_photon_file_read": (_I32, [_P, C.c_char_p, _P, _U32, C.POINTER(_U32)]),
+    "pm_photon_file_write": (_I32, [_P, C.c_char_p, _P, _U32]),
 }
 
 
@@ -377,6 +380,23 @@ class Context:
         order = torch.empty((q.shape[1],), dtype=torch.int32, device=self.device)
         self.check(self.lib.pm_grid_query_order(self.h, grid.h, _ptr(q), q.shape[1], _ptr(order)))
         return order
+
+    # -- photon-map files (Initializer.cu:102-141) --------------------------------
+    def photon_file_read(self, path: str):
+        """A photon-map file into a new device photons9 tensor [9, n]."""
+        import torch
+
+        n = C.c_uint64()
+        self.check(self.lib.pm_photon_file_count(self.h, os.fsencode(path), C.byref(n)))
+        ph = torch.empty((9, int(n.value)), dtype=torch.float32, device=self.device)
+        got = _U32()
+        self.check(self.lib.pm_photon_file_read(self.h, os.fsencode(path), _ptr(ph), int(n.value), C.byref(got)))
+        return ph[:, :got.value]
+
+    def photon_file_write(self, path: str, photons9):
+        ph = _f32_dev(photons9, self.device)
+        assert ph.dim() == 2 and ph.shape[0] == 9
+        self.check(self.lib.pm_photon_file_write(self.h, os.fsencode(path), _ptr(ph), ph.shape[1]))
 
     # -- scene / photon tracing ------------------------------------------------
     def upload_scene(self, scene, accel: bool = False) -> "DeviceScene":

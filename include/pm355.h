@@ -377,6 +377,24 @@ pm_status pm_server_stats(const pm_server* server, pm_server_info* out);
 /* per-frame latency in seconds (eligible -> completed), oldest first */
 pm_status pm_server_latencies(const pm_server* server, double* out, uint64_t cap, uint64_t* n);
 
+/* ---- photon-map files: the binary format of load() (Initializer.cu:102-141):
+ * a uint64 photon count, then per photon 9 FP32 values (pos, power,
+ * inc_dir), 36 bytes, little-endian. The reference only reads this format
+ * (into an AoS device array); here files load into and save from photons9
+ * blocks, for persisting resident maps across runs. Files stream through a
+ * pinned host buffer in chunks and are transposed on the device. The calls
+ * synchronise the context's stream.
+ *   pm_photon_file_count: the count stored in the file's header;
+ *   pm_photon_file_read:  the file into d_photons9 (plane stride = the
+ *                         file's count); PM_ERR_CAPACITY when the count
+ *                         exceeds capacity, PM_ERR_INVALID when the file is
+ *                         missing or truncated (then *n_read = 0);
+ *   pm_photon_file_write: n photons of d_photons9 (plane stride n). */
+pm_status pm_photon_file_count(pm_ctx* ctx, const char* path, uint64_t* n);
+pm_status pm_photon_file_read(pm_ctx* ctx, const char* path, float* d_photons9, uint32_t capacity,
+                              uint32_t* n_read);
+pm_status pm_photon_file_write(pm_ctx* ctx, const char* path, const float* d_photons9, uint32_t n);
+
 /* ---- model AABB: create_kd_tree(const Model*) (KdTree.cu:395-464), whose
  * only live effect is the model's bounding box (min/max over its vertices).
  * d_tris9: one triangle per 9 floats (v0.xyz v1.xyz v2.xyz); model m spans

@@ -110,6 +110,28 @@ int orc_trace_photons(const float* tris9, const float* normals9, uint32_t n_tris
                       uint64_t seed, uint32_t n_global, uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,
                       float* out_global9, float* out_caustic9, uint64_t* out_stats);
 
+/* One bounce of one light path, as orc_trace_photons decides it: the
+ * incoming ray (o, d), the hit point p, the normal n turned against d, the
+ * event dest (0 diffuse, 1 specular, 2 absorb, 3 refract: the Path values of
+ * PathOperator.cuh), stored (0 none, 1 global map, 2 caustic map), flags
+ * (bit 0: the critical-angle test ruled refraction out; bit 1: Ray::refract
+ * failed and the path ended), the outgoing ray (zero when the path ends
+ * there) and the refraction pair (n1, n2) consulted (0 on opaque models). */
+typedef struct {
+    uint32_t path;
+    int32_t bounce, model, dest, stored, flags;
+    float o[3], d[3], p[3], n[3], next_o[3], next_d[3];
+    float n1, n2;
+} orc_trace_event;
+
+/* Every bounce event of paths [0, n_paths) of the seed's numbering, in
+ * (path, bounce) order, into out[cap]; *n_events = total (returns -4 when it
+ * exceeds cap). n_global sets the photon power as in orc_trace_photons. */
+int orc_trace_events(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
+                     int32_t n_models, int32_t light_model, const float intensity[3], float default_ior,
+                     uint64_t seed, uint32_t n_global, int32_t max_bounces, uint64_t n_paths, orc_trace_event* out,
+                     uint64_t cap, uint64_t* n_events);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,3 +267,34 @@ def render(scene, og, oc, camera, max_depth=4, k_global=50, r_global=0.05, k_cau
     if rc != 0:
         raise ValueError(f"orc_render rc={rc}")
     return rgb, rgb8, int(st[0])
+
+
+TRACE_EVENT_DTYPE = np.dtype([
+    ("path", "<u4"), ("bounce", "<i4"), ("model", "<i4"), ("dest", "<i4"), ("stored", "<i4"), ("flags", "<i4"),
+    ("o", "<f4", 3), ("d", "<f4", 3), ("p", "<f4", 3), ("n", "<f4", 3), ("next_o", "<f4", 3), ("next_d", "<f4", 3),
+    ("n1", "<f4"), ("n2", "<f4"),
+])
+# event kinds (the Path values of PathOperator.cuh)
+EV_DIFFUSE, EV_SPECULAR, EV_ABSORB, EV_REFRACT = 0, 1, 2, 3
+
+
+def trace_events(scene, seed, n_paths, max_bounces=10, n_global=1, cap=None):
+    """Every bounce event of paths [0, n_paths) as the oracle tracer decides
+    them (TRACE_EVENT_DTYPE records in (path, bounce) order)."""
+    L = lib()
+    L.orc_trace_events.argtypes = [_P, _P, _U32, _P, _I32, _I32, _P, _F32, C.c_uint64, _U32, _I32, C.c_uint64,
+                                   _P, C.c_uint64, _P]
+    L.orc_trace_events.restype = _I32
+    tris = _c(scene.tris9, np.float32)
+    nrms = _c(scene.normals9, np.float32)
+    models = np.ascontiguousarray(scene.models)
+    inten = _c(scene.light_intensity, np.float32)
+    cap = int(cap or n_paths * (max_bounces + 1))
+    out = np.empty(cap, TRACE_EVENT_DTYPE)
+    n = np.zeros(1, np.uint64)
+    rc = L.orc_trace_events(tris.ctypes.data, nrms.ctypes.data, tris.shape[0], models.ctypes.data, len(models),
+                            int(scene.light_model), inten.ctypes.data, float(scene.default_refr_index), int(seed),
+                            int(n_global), int(max_bounces), int(n_paths), out.ctypes.data, cap, n.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"orc_trace_events rc={rc} (events {int(n[0])}, cap {cap})")
+    return out[:int(n[0])]

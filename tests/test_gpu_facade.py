@@ -65,6 +65,51 @@ def test_facade_render_obj_equals_python_path(ctx, tmp_path):
     assert np.array_equal(rd("global9.f32", np.uint32), g.cpu().numpy().reshape(-1).view(np.uint32))
     assert np.array_equal(rd("caustic9.f32", np.uint32), c.cpu().numpy().reshape(-1).view(np.uint32))
     assert np.array_equal(rd("rgb8.u8", np.uint8), rgb8.cpu().numpy())
+    # the maps persisted in the reference's photon-map file format
+    from pm355.workloads import read_photon_file
+
+    assert np.array_equal(read_photon_file(os.path.join(tmp_path, "global_map.bin")).view(np.uint32),
+                          g.cpu().numpy().view(np.uint32))
+    assert np.array_equal(read_photon_file(os.path.join(tmp_path, "caustic_map.bin")).view(np.uint32),
+                          c.cpu().numpy().view(np.uint32))
     assert np.array_equal(read_ppm(os.path.join(tmp_path, "frame.ppm"))[::-1].reshape(-1), rgb8.cpu().numpy())
     gg.close()
     gc.close()
+
+
+def test_photon_files_through_the_c_abi(ctx, tmp_path):
+    """pm_photon_file_read / _write against the independent numpy reader and
+    writer: the reference fixture global_map_test.txt (100 photons) loads
+    as numpy reads it; a 2.5M-photon block (three streaming chunks) round
+    trips bit for bit; errors for a missing file, a truncated file and a
+    capacity below the count."""
+    import ctypes as C
+
+    import torch
+    import pm355
+    from conftest import GOLDEN
+    from pm355.workloads import read_photon_file, write_photon_file
+
+    fx = os.path.join(GOLDEN, "global_map_test.bin")
+    got = ctx.photon_file_read(fx)
+    assert np.array_equal(got.cpu().numpy().view(np.uint32), read_photon_file(fx).view(np.uint32))
+    ph = torch.rand((9, 2_500_000), device="cuda")
+    p = os.path.join(tmp_path, "big.bin")
+    ctx.photon_file_write(p, ph)
+    assert os.path.getsize(p) == 8 + 36 * ph.shape[1]
+    assert np.array_equal(read_photon_file(p).view(np.uint32), ph.cpu().numpy().view(np.uint32))
+    back = ctx.photon_file_read(p)
+    assert torch.equal(back, ph)
+    q = os.path.join(tmp_path, "np.bin")
+    write_photon_file(q, ph[:, :1000].cpu().numpy())
+    assert torch.equal(ctx.photon_file_read(q), ph[:, :1000])
+    with pytest.raises(pm355.PMError):
+        ctx.photon_file_read(os.path.join(tmp_path, "missing.bin"))
+    with open(os.path.join(tmp_path, "short.bin"), "wb") as f:
+        f.write(open(p, "rb").read()[: 8 + 36 * 10 + 5])
+    with pytest.raises(pm355.PMError):
+        ctx.photon_file_read(os.path.join(tmp_path, "short.bin"))
+    out = torch.empty((9, 10), device="cuda")
+    n = C.c_uint32()
+    rc = ctx.lib.pm_photon_file_read(ctx.h, os.fsencode(p), out.data_ptr(), 10, C.byref(n))
+    assert rc == pm355.PM_ERR_CAPACITY and n.value == 0

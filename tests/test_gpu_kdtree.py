@@ -102,3 +102,18 @@ def test_kdtree_knn_ties(ctx):
     pos = np.random.default_rng(4).integers(0, 6, (3, 3000)).astype(np.float32) * 0.01
     q = np.random.default_rng(5).integers(0, 6, (3, 400)).astype(np.float32) * 0.01
     _kd_knn_check(ctx, pos, q, 0.02, 50)
+
+
+def test_kdtree_knn_deep_stack(ctx):
+    """A 21-level tree (2^20 photons) searched with r = 0.1: the subtree
+    roots of the breadth-first phase sit at depth <= 7, and each lane's
+    depth-first stack holds one pending sibling per level below its root, so
+    it grows past the 12 entries kept in LDS into the per-lane spill array.
+    Results equal the oracle's kd search and brute force."""
+    rng = np.random.default_rng(77)
+    pos = rng.random((3, 1 << 20), dtype=np.float32)
+    q = rng.random((3, 300), dtype=np.float32)
+    gi, gd, gc = _kd_knn_check(ctx, pos, q, 0.1, 64)
+    bi, bd, bc = O.knn_brute(pos, q, 0.1, 64)
+    assert np.array_equal(gc, bc) and np.array_equal(gi, bi)
+    assert np.all(gc == 64)

@@ -87,3 +87,43 @@ def test_batched_frames_equal_single_frames(ctx):
         assert hits == hits1
         assert np.array_equal(rgb.cpu().numpy().view(np.uint32), rgb1.cpu().numpy().view(np.uint32))
         assert np.array_equal(rgb8.cpu().numpy(), rgb81.cpu().numpy())
+
+
+@pytest.mark.parametrize("seed", [101, 202, 303])
+def test_c2_full_frame_against_oracle(ctx, seed):
+    """The per-frame configuration at full size: Cornell box with BVHs,
+    2M global + 500k caustic photons traced (<= 10 bounces), grids at
+    cell = r, a 1280x720 frame with k = 100 within r = 0.02, depth 4. The GPU
+    maps equal the oracle's maps bit for bit; the oracle renders its own maps;
+    the GPU frame's per-pixel RMSE against it stays within RMSE_MAX for every
+    seed and on average over the seeds."""
+    scene = cornell_box()
+    ds = ctx.upload_scene(scene, accel=True)
+    g, c, st = ctx.trace_photons(ds, seed, 2_000_000, 500_000, 10, max_paths=200_000_000)
+    og9, oc9, ost = O.trace_photons(scene, seed, 2_000_000, 500_000, 10, max_paths=200_000_000)
+    assert st[1:] == ost[1:] == (2_000_000, 500_000)
+    gh, ch = g.cpu().numpy(), c.cpu().numpy()
+    assert np.array_equal(gh.view(np.uint32), og9.view(np.uint32))
+    assert np.array_equal(ch.view(np.uint32), oc9.view(np.uint32))
+    aabb = scene.aabb()
+    gg = ctx.grid_build(g, aabb, cell_size=0.02)
+    gc = ctx.grid_build(c, aabb, cell_size=0.02)
+    cam = cornell_camera(1280, 720)
+    rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, 4, 100, 0.02, 100, 0.02)
+    ctx.sync()
+    og, oc = O.OracleGrid(og9, aabb, cell_size=0.02), O.OracleGrid(oc9, aabb, cell_size=0.02)
+    orgb, orgb8, ohits = O.render(scene, og, oc, cam, 4, 100, 0.02, 100, 0.02)
+    assert hits == ohits
+    got = rgb.cpu().numpy().astype(np.float64)
+    rmse = float(np.sqrt(np.mean((got - orgb) ** 2)))
+    _C2_RMSE.append(rmse)
+    assert rmse <= RMSE_MAX, rmse
+    assert np.abs(got - orgb).max() <= ABS_MAX
+    d8 = np.abs(rgb8.cpu().numpy().astype(np.int32) - orgb8.astype(np.int32))
+    assert d8.max() <= 1 and np.mean(d8 > 0) <= U8_STEP_FRAC
+    assert np.mean(_C2_RMSE) <= RMSE_MAX
+    for h in (gg, gc):
+        h.close()
+
+
+_C2_RMSE = []

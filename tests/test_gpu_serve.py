@@ -74,3 +74,50 @@ def test_budget_admits_jobs_as_memory_frees(ctx):
     wide, _, _, _ = _serve(ctx, specs)
     for a, b in zip(small, wide):
         assert np.array_equal(a, b)
+
+
+def _dark_scene():
+    """A light facing away from the only other surface: every photon
+    escapes, so a trace stores nothing."""
+    from pm355.scenes import PM_MODEL_DTYPE, Scene, _model, _quad
+
+    P = lambda *a: np.array(a, np.float32)
+    lt, ln = _quad(P(0.4, 0.5, 0.4), P(0.6, 0.5, 0.4), P(0.6, 0.5, 0.6), P(0.4, 0.5, 0.6), P(0, -1, 0))
+    ct, cn = _quad(P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1), P(0, -1, 0))
+    models = np.array([_model(0, 2, kd=(0.8, 0.8, 0.8), ke=(1, 1, 1), mid=0), _model(2, 4, kd=(0.7, 0.7, 0.7), mid=1)],
+                      PM_MODEL_DTYPE)
+    return Scene(np.concatenate([lt, ct]), np.concatenate([ln, cn]), models, light_model=0)
+
+
+def test_empty_photon_map(ctx):
+    """A trace that stores no photon gives an empty grid: gathers find
+    nothing and estimate zero, and a served job over it renders the same
+    frames as the calls one at a time (emission only)."""
+    import torch
+
+    aabb = (0, 0, 0, 1, 1, 1)
+    empty = ctx.grid_build(torch.zeros((9, 0), dtype=torch.float32, device="cuda"), aabb, cell_size=0.05)
+    assert empty.info.n_photons == 0
+    q = torch.rand((3, 500), device="cuda")
+    qn = torch.zeros_like(q)
+    qn[1] = 1
+    idx, dist, cnt = ctx.knn(empty, q, 0.05, 16)
+    rgb, c2 = ctx.gather_radiance(empty, q, qn, 0.05, 16, want_count=True)
+    ctx.sync()
+    assert not cnt.any() and not c2.any() and not rgb.any()
+    assert np.all(idx.cpu().numpy().view(np.uint32) == 0xFFFFFFFF)
+    sc = _dark_scene()
+    ds = ctx.upload_scene(sc)
+    cam = camera((0.5, 0.1, 1.2), (0.0, 0.6, -1.0), (0, 1, 0), 60.0, 64, 48)
+    spec = dict(dscene=ds, aabb=sc.aabb(), cell_size=0.05, n_global=1000, n_caustic=100, camera=cam, frames=2,
+                seed=5, max_depth=3, k_global=16, r_global=0.05, k_caustic=8, r_caustic=0.05, max_paths=20_000)
+    got = _serve(ctx, [spec])[0][0]
+    for f in range(2):
+        seed = (spec["seed"] + SEED_STEP * (f + 1)) & ((1 << 64) - 1)
+        g, c, st = ctx.trace_photons(ds, seed, 1000, 100, 10, max_paths=20_000)
+        assert st[1] == 0 and st[2] == 0
+        gg = ctx.grid_build(g[:, :0], spec["aabb"], cell_size=0.05)
+        _, rgb8, _ = ctx.render_frame(ds, gg, None, cam, 3, 16, 0.05, 8, 0.05)
+        ctx.sync()
+        assert np.array_equal(got[f], rgb8.cpu().numpy())
+    assert got.any()  # the light is in view
