@@ -37,10 +37,34 @@ def host_threads():
     return max(1, min(threads, 64))
 
 
-def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
-    """Oracle (C++ restatement of the reference CPU path) on the host cores:
-    grid build once, then kNN + radiance on a bounded sample of the queries."""
+def native_oracle():
+    """Builds the oracle with -march=native for this host (into a temporary
+    directory) and points pyoracle at it, for the CPU baselines; falls back
+    to the portable in-tree build. Returns the build used."""
+    import subprocess
+    import tempfile
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    d = tempfile.mkdtemp(prefix="pm355_oracle_native_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_DIR={d}"], check=True,
+                       capture_output=True, timeout=300)
+        pyoracle.LIB_PATH = os.path.join(d, "liboracle.so")
+        pyoracle._lib = None
+        return "g++ -O3 -march=native -ffp-contract=off -fopenmp, built on this host"
+    except (OSError, subprocess.SubprocessError) as e:
+        return f"g++ -O3 -ffp-contract=off -fopenmp, portable build (the -march=native build failed: {e})"
+
+
+def cpu_baseline(ph, q, qn, aabb, r, k, build, run_s=3.0, runs=5):
+    """Oracle (C++ restatement of the reference CPU path) on the host cores:
+    grid build once, then kNN + radiance over a seeded sample of the queries
+    sized for about run_s seconds, timed `runs` times; the median counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import statistics
+
     import numpy as np
 
     import pyoracle
@@ -49,38 +73,37 @@ def cpu_baseline(ph, q, qn, aabb, r, k, budget_s=12.0):
     t0 = time.perf_counter()
     og = pyoracle.OracleGrid(ph, aabb, cell_size=r)
     t_build = time.perf_counter() - t0
-    n = 20_000
-    sel = np.random.default_rng(0).choice(q.shape[1], n, replace=False)
-    t0 = time.perf_counter()
-    oi, od, oc = og.knn(q[:, sel], r, k, threads=threads)
-    pyoracle.radiance(oi, od, oc, og.photons, qn[:, sel], k)
-    dt = time.perf_counter() - t0
-    m = int(min(q.shape[1], max(n, n * (budget_s / max(dt, 1e-3)))))
-    if m > n:
-        sel = np.random.default_rng(1).choice(q.shape[1], m, replace=False)
+
+    def run(sel):
         t0 = time.perf_counter()
         oi, od, oc = og.knn(q[:, sel], r, k, threads=threads)
         pyoracle.radiance(oi, od, oc, og.photons, qn[:, sel], k)
-        dt = time.perf_counter() - t0
-        n = m
+        return time.perf_counter() - t0
+
+    pilot = np.random.default_rng(0).choice(q.shape[1], 20_000, replace=False)
+    dt = run(pilot)
+    n = int(min(q.shape[1], max(20_000, 20_000 * run_s / max(dt, 1e-3))))
+    sel = np.random.default_rng(1).choice(q.shape[1], n, replace=False)
+    ts = [run(sel) for _ in range(runs)]
     return {
-        "value": n / dt,
+        "value": n / statistics.median(ts),
         "unit": "queries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} of the 4M queries (seeded sample), kNN k={k} + radiance on the oracle grid; "
-                  f"oracle grid build of 16M photons took {t_build:.2f} s on 1 thread (not in value)",
+        "sample": f"{n} of the 4M queries (seeded sample), kNN k={k} + radiance on the oracle grid at cell = r, "
+                  f"median of {runs} runs ({', '.join(f'{t:.2f}' for t in ts)} s); oracle {build}; oracle grid "
+                  f"build of the 16M photons took {t_build:.2f} s on 1 thread (not in value)",
     }
 
 
-def cpu_stages(threads):
+def cpu_stages(threads, build):
     """Per-stage timings of the oracle on the host cores (the reference CPU
-    path restated), each the median of three runs on identical inputs:
-    photon tracing of the Cornell box (1 thread: the oracle's tracer is
-    serial), a 2M-photon grid build and kd-tree build (1 thread), the kNN
-    gather at k = 64 (1 thread and all threads), the radiance estimate over
-    its result (1 thread), and a 320x240 frame (100k photons, k = 50,
-    r = 0.05) on all threads. The units say what was counted."""
+    path restated), each the median of five runs on identical inputs:
+    photon tracing of the Cornell box (1 thread and all threads), a
+    2M-photon grid build and kd-tree build (1 thread), the kNN gather at
+    k = 64 (1 thread and all threads), the radiance estimate over its result
+    (1 thread), and a 320x240 frame (100k photons, k = 50, r = 0.05) on all
+    threads. The units say what was counted."""
     import statistics
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -89,7 +112,7 @@ def cpu_stages(threads):
     from pm355.scenes import cornell_box, cornell_camera
     from pm355.workloads import gather_microbench
 
-    def med(fn, reps=3):
+    def med(fn, reps=5):
         ts = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -97,10 +120,18 @@ def cpu_stages(threads):
             ts.append(time.perf_counter() - t0)
         return statistics.median(ts)
 
-    out = {"threads_all": threads, "nproc": os.cpu_count(), "runs_per_stage": 3}
+    out = {"threads_all": threads, "nproc": os.cpu_count(), "runs_per_stage": 5, "oracle_build": build}
     try:
+        info = {}
         with open("/proc/cpuinfo") as f:
-            out["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+            for ln in f:
+                if ":" in ln:
+                    key, val = ln.split(":", 1)
+                    info.setdefault(key.strip(), val.strip())
+        flags = set(info.get("flags", "").split())
+        out["cpu_model"] = info.get("model name", "")
+        out["cpu_family_model_stepping"] = [info.get("cpu family"), info.get("model"), info.get("stepping")]
+        out["cpu_simd"] = sorted(f for f in flags if f in ("avx2", "fma", "avx512f", "avx512vl", "avx512_bf16"))
     except OSError:
         pass
     sc = cornell_box()
@@ -109,9 +140,13 @@ def cpu_stages(threads):
     def trace():
         res["trace"] = pyoracle.trace_photons(sc, 7, 100_000, 10_000, 8)
 
+    pyoracle.set_threads(1)
     dt = med(trace)
+    pyoracle.set_threads(threads)
     st = res["trace"][2]
     out["trace_photons_per_s_1thread"] = (st[1] + st[2]) / dt
+    out["trace_photons_per_s_all"] = (st[1] + st[2]) / med(trace)
+    pyoracle.set_threads(0)
     ph, q, qn, aabb = gather_microbench(seed=3, n_photons=2_000_000, n_queries=20_000)
     out["grid_build_2M_ms_1thread"] = med(lambda: pyoracle.OracleGrid(ph, aabb, cell_size=0.01)) * 1e3
     og = pyoracle.OracleGrid(ph, aabb, cell_size=0.01)
@@ -205,11 +240,15 @@ def frame_bench(ctx, n_frames):
     vpath = os.path.join(ROOT, "profiles", "cornell_c2_trace_valu.json")
     if os.path.exists(vpath):
         # FP32 VALU operations of one C2 photon trace, counted by rocprofv3
-        # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32 x 64 lanes, FMA x 2)
+        # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32 x 64 lanes, FMA x 2), and
+        # whether the HIP sources changed since they were counted
+        from pm355 import profiling
+
         v = json.load(open(vpath))
         tflops = v["fp32_flop_per_trace"] / (stages[0] * 1e-3) / 1e12
         valu = {"fp32_tflops": tflops, "peak_tflops": 157.3, "frac": tflops / 157.3,
-                "source": "profiles/cornell_c2_trace_valu.json"}
+                "source": "profiles/cornell_c2_trace_valu.json",
+                "stale": v.get("source_hash") != profiling.source_hash()}
     return {
         "workload": "Cornell box (10 wall + 2x960 sphere + 2 light triangles, per-model BVHs), 2M global + 500k "
                     "caustic photons (<=10 bounces), grids cell=r and kd-trees built per frame, 1280x720, "
@@ -306,9 +345,11 @@ def reference_frame_bench(ctx, n_frames, cpu_threads=0):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
 
+        O.set_threads(cpu_threads)
         t0 = time.perf_counter()
         O.trace_photons(scene, 1, RF["n_global"] // 10, RF["n_caustic"] // 10, RF["max_bounces"])
         trace_s = (time.perf_counter() - t0) * 10
+        O.set_threads(0)
         gh, ch = (m.cpu().numpy() for m in maps)
         t0 = time.perf_counter()
         og, oc = O.OracleGrid(gh, aabb), O.OracleGrid(ch, aabb)
@@ -320,7 +361,7 @@ def reference_frame_bench(ctx, n_frames, cpu_threads=0):
         out["cpu_oracle"] = {"frame_ms_estimate": (trace_s + grid_s + render_s) * 1e3,
                              "stage_ms_estimate": {"trace": trace_s * 1e3, "grid_builds": grid_s * 1e3,
                                                    "render": render_s * 1e3},
-                             "threads": {"trace": 1, "grid_builds": 1, "render": cpu_threads},
+                             "threads": {"trace": cpu_threads, "grid_builds": 1, "render": cpu_threads},
                              "sample": "trace 100k + 10k photons (x10), render 256x192 over the full maps (x16)"}
     return out
 
@@ -483,6 +524,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
+    build = None
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the photon path "
               f"runs on a single MI355X; the photon path shards by query, tile and frame over a replicated photon map)", file=sys.stderr)
@@ -536,6 +578,9 @@ def main():
     stream = torch.cuda.current_stream(dev)
 
     def timed(fn, reps):
+        # one untimed call first: it grows scratch and allocates outputs
+        fn()
+        torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
@@ -558,7 +603,6 @@ def main():
         row = {}
         for name in variants:
             vf = pm355.KNN_VARIANTS[name]
-            ctx.gather_radiance(gt, qs, ns, r, k, flags=vf)
             row[name] = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k, flags=vf), 2)
             if best_ms is None or row[name] < best_ms:
                 best, best_ms = (frac, name), row[name]
@@ -566,9 +610,9 @@ def main():
         gt.close()
         del qs, ns
     frac, vname = best
-    # Timings within 2% are noise: then keep the configuration whose HBM
-    # traffic was profiled (profiles/), so the bench line can report it.
-    tpath0 = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_hbm_traffic.json")
+    # Timings within 2% are noise: then keep the configuration whose
+    # counters were profiled (profiles/), so the bench line can report them.
+    tpath0 = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_counters.json")
     if os.path.exists(tpath0) and not (args.cell_over_r or args.variant):
         t0 = json.load(open(tpath0))
         pf, pv = t0.get("cell_over_r"), t0.get("variant")
@@ -631,7 +675,8 @@ def main():
     # reference cell size s = r (exact counts, as the oracle's counting pass
     # gives them). The kernel itself walks that box pruned to what can reach
     # r at the tuned cell size ("walked" below), and most of its reads hit in
-    # L2 / MALL, so the fraction can exceed 1.
+    # L2 / MALL, so these bytes per second can exceed the HBM peak: they
+    # are reported beside the roofline, not as it.
     rgb, cnt = ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags, want_count=True)
     torch.cuda.synchronize()
     kept = int(cnt.to(torch.int64).sum().item())
@@ -639,18 +684,32 @@ def main():
     cells_ref, cands_ref = ctx.box_stats(g_ref, d_q, r, reference=True)
     g_ref.close()
     alg_bytes = nq * (12 + 12 + 12) + 8 * cells_ref + 12 * cands_ref + (8 + 24) * kept
-    achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
     cells, cands = ctx.box_stats(g, d_q, r)
     walked_bytes = nq * (12 + 12 + 12) + 8 * cells + 12 * cands + 24 * kept
 
-    # HBM bytes per dispatch of this kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE summary (scaled by a 1 GiB copy probe).
+    # Bytes per launch at each memory level, from the committed rocprofv3
+    # counter summary of this configuration (pm355.profiling; flagged stale
+    # when the HIP sources changed since it was measured): L2 read bytes
+    # (TCC_READ_SECTORS x 32) and HBM bytes (FETCH_SIZE / WRITE_SIZE scaled
+    # by a 1 GiB copy). The candidate positions of neighbouring cell-ordered
+    # queries are read from L2 / the Infinity Cache, so L2 is the level that
+    # serves the gather's bytes and the roofline is drawn there. Without a
+    # matching summary, the live walked-box bytes (every candidate position,
+    # cell bound and kept record the kernel reads) stand in for the L2 bytes.
+    from pm355 import profiling
+
+    cpath = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_counters.json")
+    cs = profiling.load(cpath, variant=vname, cell_over_r=float(frac))
+    t_s = gather_ms * 1e-3
+    if cs and "l2_read_bytes_per_dispatch" in cs:
+        l2_bytes, l2_src = cs["l2_read_bytes_per_dispatch"], "TCC_READ_SECTORS x 32 B (profiles/" + \
+            os.path.basename(cpath) + (", STALE: measured on other HIP sources)" if cs["stale"] else ")")
+    else:
+        l2_bytes, l2_src = walked_bytes, "walked-box bytes (no counter summary for this configuration)"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_hbm_traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath))
-        if t.get("variant") == vname and abs(t.get("cell_over_r", 1.0) - frac) < 1e-3:
-            traffic = t["hbm_read_bytes_per_dispatch"] + t["hbm_write_bytes_per_dispatch"]
+    if cs and "hbm_read_bytes_per_dispatch" in cs and "hbm_write_bytes_per_dispatch" in cs:
+        traffic = cs["hbm_read_bytes_per_dispatch"] + cs["hbm_write_bytes_per_dispatch"]
+    achieved = l2_bytes / t_s / 1e9
 
     value = nq / (ms_per_step * 1e-3)
     res = {
@@ -679,17 +738,31 @@ def main():
             "gather_variant": vname,
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": "l2",
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
+            "peak": profiling.L2_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved / profiling.L2_PEAK_GBS,
             "traffic": traffic,
             "kernel": {"stream512_chunked": "k_knn_stream512", "stream512": "k_knn_stream512",
                        "stream1024": "k_knn_stream1024", "two_pass": "k_knn_two_pass"}.get(vname, vname)
                       + f" (variant {vname}, cell {frac:.3f}r; fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
+            "l2_read_bytes": l2_bytes,
+            "l2_read_bytes_source": l2_src,
+            "peak_source": "aggregate L2 read bandwidth: 8 XCDs x 16 channels x 128 B/clk x 2.4 GHz",
+            "hbm_GBs": traffic / t_s / 1e9 if traffic else None,
+            "hbm_frac": traffic / t_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "l2_hit_rate": cs.get("l2_hit_rate") if cs else None,
+            "l1_hit_rate": cs.get("l1_hit_rate") if cs else None,
+            "l2_read_latency_cycles": cs.get("l2_read_latency_cycles") if cs else None,
+            "counters_stale": cs["stale"] if cs else None,
+            "walked_bytes": walked_bytes,
             "alg_bytes": alg_bytes,
+            "alg_bytes_definition": "sum over queries of 12 + 8 C_q + 12 P_q + 32 kept + 24 (reference box at "
+                                    "cell = r); its rate exceeds the HBM peak because neighbouring queries share "
+                                    "candidates in L2 / MALL",
+            "alg_GBs": alg_bytes / t_s / 1e9,
         },
         "components_ms": {
             "grid_build": build_ms,
@@ -709,7 +782,6 @@ def main():
         gr = ctx.grid_build(d_ph, aabb, cell_size=r)
         orr = ctx.query_order(gr, d_q).long()
         qs, ns = d_q[:, orr].contiguous(), d_n[:, orr].contiguous()
-        ctx.gather_radiance(gr, qs, ns, r, k, flags=vflags)
         ms_r = timed(lambda: ctx.gather_radiance(gr, qs, ns, r, k, flags=vflags), 3)
         b_r = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r).close(), 3)
         res["cell_equals_r"] = {"gather_ms": ms_r, "grid_build_ms": b_r, "step_queries_per_s_estimate":
@@ -740,7 +812,6 @@ def main():
     # level; the final record permutation).
     a = torch.empty(256 * 1024 * 1024, device=dev)
     b = torch.empty_like(a)
-    b.copy_(a)
     copy_ms = timed(lambda: b.copy_(a), 5)
     copy_gbs = 2 * a.numel() * 4 / (copy_ms * 1e-3) / 1e9
     del a, b
@@ -762,6 +833,8 @@ def main():
     if args.frames > 0:
         log(f"per-frame form: {args.frames} Cornell-box frames")
         res["per_frame"] = frame_bench(ctx, args.frames)
+    if not args.no_cpu_baseline:
+        build = native_oracle()
     if args.ref_frames > 0:
         log(f"reference frame: {args.ref_frames} CornellBox-Sphere frames")
         res["reference_frame"] = reference_frame_bench(ctx, args.ref_frames,
@@ -777,8 +850,8 @@ def main():
                          **serve_bench(ctx, n_jobs=args.serve_jobs)}
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k)
-        res["cpu_stages"] = cpu_stages(res["cpu_baseline"]["cores"])
+        res["cpu_baseline"] = cpu_baseline(ph, q, qn, aabb, r, k, build)
+        res["cpu_stages"] = cpu_stages(res["cpu_baseline"]["cores"], build)
     ctx.close()
     print(json.dumps(res), flush=True)
 

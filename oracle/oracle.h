@@ -105,6 +105,10 @@ int orc_render(const float* tris9, const float* normals9, uint32_t n_tris, const
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* Threads of the OpenMP parts that take no thread argument (the tracer);
+ * n <= 0 restores the default. Returns the count now in effect. */
+int orc_set_threads(int32_t n);
+
 int orc_trace_photons(const float* tris9, const float* normals9, uint32_t n_tris, const void* models_raw,
                       int32_t n_models, int32_t light_model, const float intensity[3], float default_ior,
                       uint64_t seed, uint32_t n_global, uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,

@@ -18,6 +18,9 @@
 #include <cfloat>
 #include <cmath>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "oracle.h"
 #include "oracle_math.hpp"
@@ -222,6 +225,16 @@ struct Tracer {
 
 static bool load_models(const void* models_raw, int32_t n_models, uint32_t n_tris, const float* tris9,
                         std::vector<Model>& M);
+
+extern "C" int orc_set_threads(int32_t n) {
+#ifdef _OPENMP
+    static const int dflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : dflt);
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 // One Philox4x32-10 block, for the generator's known-answer tests.
 extern "C" void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {

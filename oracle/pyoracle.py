@@ -175,6 +175,15 @@ def radius_threshold(r):
     return float(np.float32(lib().orc_radius_threshold(float(r))))
 
 
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops that take no thread argument
+    (the tracer); n <= 0 restores the default."""
+    L = lib()
+    L.orc_set_threads.argtypes = [_I32]
+    L.orc_set_threads.restype = _I32
+    return int(L.orc_set_threads(int(n)))
+
+
 def philox4x32_10(ctr, key):
     """One Philox4x32-10 block of the tracer's generator: 4 u32 counter words,
     2 u32 key words -> 4 u32."""

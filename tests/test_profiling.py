@@ -1,0 +1,55 @@
+"""CPU: the rocprofv3 counter summaries behind the benchmark's roofline
+(pm355.profiling) on synthetic counter-collection CSVs."""
+import csv
+import json
+import os
+
+from pm355 import profiling
+
+FIELDS = ["Correlation_Id", "Dispatch_Id", "Agent_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+
+
+def _write(d, rows):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "pmc_counter_collection.csv"), "w", newline="") as f:
+        w = csv.DictWriter(f, FIELDS)
+        w.writeheader()
+        for i, (kern, name, val) in enumerate(rows):
+            w.writerow(dict(Correlation_Id=i, Dispatch_Id=i // 10, Agent_Id=1, Kernel_Name=kern, Counter_Name=name,
+                            Counter_Value=val))
+
+
+def test_summarize_scales_and_rates(tmp_path):
+    k = "(anonymous namespace)::k_knn_stream512(KnnParams)"
+    # two dispatches of the kernel, one of another kernel
+    _write(tmp_path / "a", [(k, "TCC_READ_SECTORS_sum", 100), (k, "TCC_HIT_sum", 30), (k, "TCC_MISS_sum", 10)]
+           + [("other(int)", "TCC_READ_SECTORS_sum", 1e9)] * 7
+           + [(k, "TCC_READ_SECTORS_sum", 300), (k, "TCC_HIT_sum", 90), (k, "TCC_MISS_sum", 10)])
+    _write(tmp_path / "b", [(k, "FETCH_SIZE", 1000), (k, "WRITE_SIZE", 10)])
+    # the copy probe: 1 GiB read shows as half a GiB of FETCH_SIZE
+    _write(tmp_path / "c", [("__amd_rocclr_copyBuffer", "FETCH_SIZE", 524288),
+                            ("__amd_rocclr_copyBuffer", "WRITE_SIZE", 1048576)])
+    out = tmp_path / "s.json"
+    s = profiling.summarize("k_knn_stream512", [str(tmp_path / "a"), str(tmp_path / "b")], [str(tmp_path / "c")],
+                            {"variant": "v", "cell_over_r": 0.5}, str(out))
+    assert s["l2_read_bytes_per_dispatch"] == 32 * 200  # median of 100 and 300
+    assert abs(s["l2_hit_rate"] - 60 / 70) < 1e-12  # medians 60 hits, 10 misses
+    assert s["hbm_read_bytes_per_dispatch"] == 1000 * 1024 * 2
+    assert s["hbm_write_bytes_per_dispatch"] == 10 * 1024
+    assert json.load(open(out))["source_hash"] == profiling.source_hash()
+    got = profiling.load(str(out), variant="v", cell_over_r=0.5)
+    assert got is not None and got["stale"] is False
+    assert profiling.load(str(out), variant="w") is None
+    d = json.load(open(out))
+    d["source_hash"] = "0" * 16
+    json.dump(d, open(out, "w"))
+    assert profiling.load(str(out), variant="v")["stale"] is True
+
+
+def test_valu_summary(tmp_path):
+    _write(tmp_path / "v", [("k_trace_pool<false>", "SQ_INSTS_VALU_ADD_F32", 10),
+                            ("k_trace_pool<false>", "SQ_INSTS_VALU_FMA_F32", 5),
+                            ("k_trace_emit", "SQ_INSTS_VALU_MUL_F32", 1),
+                            ("k_trace_emit", "SQ_INSTS_VALU_TRANS_F32", 2)])
+    s = profiling.summarize_valu([str(tmp_path / "v")], 2, {}, str(tmp_path / "v.json"))
+    assert s["fp32_flop_per_trace"] == 64 * (10 + 1 + 2 + 2 * 5) / 2
