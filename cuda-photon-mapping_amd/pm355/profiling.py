@@ -24,6 +24,7 @@ import glob
 import hashlib
 import json
 import os
+import re
 import statistics
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -54,13 +55,14 @@ def _rows(d):
 
 def per_dispatch(dirs, kernel):
     """{counter: [value per dispatch of `kernel`]} over rocprofv3 output dirs."""
+    # the kernel's own name, bare or after a namespace / return type, followed
+    # by its argument list or template arguments
+    pat = re.compile(r"(^|[\s:])" + re.escape(kernel) + r"($|[(<])")
     acc = {}
     for d in dirs:
         vals = {}
         for r in _rows(d):
-            name = r.get("Kernel_Name", "")
-            if not (name == kernel or name.startswith(kernel + "(") or name.startswith(kernel + "<")
-                    or (" " + kernel + "(") in name or name.startswith("void " + kernel)):
+            if not pat.search(r.get("Kernel_Name", "")):
                 continue
             key = (r.get("Dispatch_Id"), r["Counter_Name"])
             vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
