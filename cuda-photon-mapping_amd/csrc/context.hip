@@ -24,6 +24,13 @@ pm_status hip_fail(pm_ctx* ctx, hipError_t e, const char* what) {
 }
 
 pm_status scratch_reserve(pm_ctx* ctx, size_t bytes, void** out) {
+    if (ctx->fixed_arena) {
+        if (bytes > ctx->fixed_arena_bytes)
+            return fail(ctx, PM_ERR_UNSUPPORTED, "scratch of %zu bytes exceeds the fixed block of %zu bytes", bytes,
+                        ctx->fixed_arena_bytes);
+        *out = ctx->fixed_arena;
+        return PM_OK;
+    }
     if (bytes > ctx->scratch_bytes) {
         if (ctx->scratch) {
             PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -84,6 +91,10 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
         delete c;
         return PM_ERR_NOMEM;
     }
+    // (a failure leaves only pm_grid_gather_radiance_noalloc unavailable,
+    // which reports it)
+    (void)pm::noalloc_setup(c);
+    c->err.clear();
     *out = c;
     return PM_OK;
 }

@@ -27,6 +27,15 @@ struct pm_ctx {
     // epoch of the next look-back launch (device-side, starts at 1)
     uint32_t* scan_counter = nullptr;
     int cu_count = 256;
+    // Fixed device memory that replaces the scratch arena and the look-back
+    // state for the duration of one call (the allocation-free entry point):
+    // while `fixed_arena` is set, scratch_reserve and lookback_begin hand
+    // these out and fail instead of allocating when they are too small.
+    void* fixed_arena = nullptr;
+    size_t fixed_arena_bytes = 0;
+    uint64_t* fixed_state = nullptr;
+    size_t fixed_state_tiles = 0;
+    uint32_t* fixed_counter = nullptr;
     // Frame state of pm_render_frame (hit slots and query block survive the
     // gathers, which use `scratch`).
     pm_frame* frame_state = nullptr;
@@ -119,6 +128,10 @@ pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint
                                 uint32_t* vals_out, uint32_t n, int key_bits, void* work);
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
+
+// Device address of the allocation-free gather's fixed scratch on the
+// context's device (taken once per device; called by pm_ctx_create).
+pm_status noalloc_setup(pm_ctx* ctx);
 
 // Per-model AABB over triangle ranges [tb[m], tb[m+1]) (device arrays).
 pm_status model_aabb_launch(pm_ctx* ctx, const float* tris9, const uint32_t* tb, int n_models, float* out6);

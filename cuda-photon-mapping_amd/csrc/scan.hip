@@ -169,6 +169,14 @@ __global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 namespace pm {
 
 pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t** counter) {
+    if (ctx->fixed_state) {
+        if (tiles > ctx->fixed_state_tiles)
+            return fail(ctx, PM_ERR_UNSUPPORTED, "look-back scan of %u tiles exceeds the fixed state of %zu tiles",
+                        tiles, ctx->fixed_state_tiles);
+        *state = ctx->fixed_state;
+        *counter = ctx->fixed_counter;
+        return PM_OK;
+    }
     if (tiles > ctx->scan_state_tiles) {
         // (growth is the only host-side step: warm a context up before
         // capturing its calls into a graph)

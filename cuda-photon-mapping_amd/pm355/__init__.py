@@ -81,6 +81,7 @@ _SIGS = {
     "pm_grid_knn": (_I32, [_P, _P, _P, _U32, _F32, _I32, _U32, _P, _P, _P]),
     "pm_radiance": (_I32, [_P, _P, _P, _P, _P, _P, _U32, _I32, _P]),
     "pm_grid_gather_radiance": (_I32, [_P, _P, _P, _P, _U32, _F32, _I32, _U32, _P, _P]),
+    "pm_grid_gather_radiance_noalloc": (_I32, [_P, _P, _P, _P, _U32, _F32, _I32, _U32, _P, _P]),
     "pm_grid_query_order": (_I32, [_P, _P, _P, _U32, _P]),
     "pm_grid_box_stats": (_I32, [_P, _P, _P, _U32, _F32, _U32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "pm_kdtree_build": (_I32, [_P, _P, _U32, C.POINTER(_P)]),
@@ -311,6 +312,17 @@ class Context:
         self.check(self.lib.pm_grid_gather_radiance(self.h, grid.h, _ptr(q), _ptr(nrm), nq, float(radius), int(k),
                                                     int(flags), _ptr(rgb), _ptr(cnt)))
         return (rgb, cnt) if want_count else rgb
+
+    def gather_radiance_noalloc(self, grid: "Grid", queries3, normals3, radius: float, k: int, flags: int = 0,
+                                rgb=None, count=None):
+        """pm_grid_gather_radiance_noalloc into caller-provided outputs
+        (rgb: (3, nq) float32, count: (nq,) int32 or None) on device."""
+        q = _f32_dev(queries3, self.device)
+        nrm = _f32_dev(normals3, self.device)
+        nq = q.shape[1]
+        self.check(self.lib.pm_grid_gather_radiance_noalloc(self.h, grid.h, _ptr(q), _ptr(nrm), nq, float(radius),
+                                                            int(k), int(flags), _ptr(rgb), _ptr(count)))
+        return rgb, count
 
     def gather_batch(self, jobs, flags: int = 0, want_knn: bool = False, want_count: bool = True):
         """Batched gather over many (grid, queries3, normals3, radius, k) jobs in

@@ -142,6 +142,19 @@ pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, const float*
                                   const float* d_normals3, uint32_t nq, float radius, int32_t k, uint32_t flags,
                                   float* d_rgb3, uint32_t* d_count);
 
+/* pm_grid_gather_radiance without allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that must not call hipMalloc, hosts that
+ * forbid it). Same arguments, kernels and results as pm_grid_gather_radiance;
+ * the call's scratch (query order, radix-sort passes, large-k overflow list,
+ * look-back scan state) lives in a fixed device array of about 26 MB per GPU,
+ * held until the process ends, and nq is limited to 1,048,576
+ * (PM_ERR_UNSUPPORTED above). Calls on one GPU share that array: calls that
+ * overlap on one GPU race. */
+pm_status pm_grid_gather_radiance_noalloc(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3,
+                                          const float* d_normals3, uint32_t nq, float radius, int32_t k,
+                                          uint32_t flags, float* d_rgb3, uint32_t* d_count);
+
 /* Cell-coherent processing order of a query block: d_order[i] = index of the
  * i-th query when queries are sorted (stably) by grid cell. Gathering the
  * queries in this order lets neighbouring waves share cells in cache; the
