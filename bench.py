@@ -592,7 +592,7 @@ def main():
     # Autotune: the grid's cell size relative to r and the gather variant,
     # jointly. Each (cell/r, variant) pair times the gather kernel alone on
     # the cell-ordered query block with HIP events on the launch stream.
-    fracs = [args.cell_over_r] if args.cell_over_r else [0.25, 1.0 / 3.0, 0.5, 2.0 / 3.0, 1.0]
+    fracs = [args.cell_over_r] if args.cell_over_r else [0.25, 1.0 / 3.0, 0.4, 0.5, 2.0 / 3.0, 1.0]
     variants = [args.variant] if args.variant else list(pm355.KNN_VARIANTS)
     tune, best, best_ms = {}, None, None
     for frac in fracs:

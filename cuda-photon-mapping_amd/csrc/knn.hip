@@ -61,6 +61,7 @@ struct KnnParams {
     uint32_t* out_count;
     float* rgb;  // 3 planes of nq (radiance only)
     int remap;
+    int h;  // inner box half-width in cells (0: walk the whole box at once)
 };
 
 struct Box {
@@ -86,6 +87,23 @@ __device__ __forceinline__ float axis_gap(float v, int c, int g, float mn, float
     if (c > 0 && v < lo - e) return (lo - e) - v;
     if (c < g - 1 && v > hi + e) return v - (hi + e);
     return 0.f;
+}
+
+// Lower bound of |v - p| along one axis over the photons hashed outside the
+// cells [lo, hi] (cells 0 and g-1 are unbounded outward, so a side at the
+// grid's edge has no photons beyond it). The cell edges are widened by a
+// relative 1e-6, far above the rounding of the hash and of the edges.
+__device__ __forceinline__ float inner_gap(float v, int lo, int hi, int g, float mn, float cell) {
+    float gap = __builtin_inff();
+    if (lo > 0) {
+        const float b = mn + (float)lo * cell;
+        gap = fminf(gap, v - (b + (fabsf(b) + fabsf(mn)) * 1e-6f));
+    }
+    if (hi < g - 1) {
+        const float b = mn + (float)(hi + 1) * cell;
+        gap = fminf(gap, (b - (fabsf(b) + fabsf(mn)) * 1e-6f) - v);
+    }
+    return fmaxf(gap, 0.f);
 }
 
 // Photon run s of the box (linear layout: the cells of one (y, z) row;
@@ -341,11 +359,20 @@ __device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds<CAP>& 
 // Radiance estimate (Raytracer.cuh:110-138) over the m kept keys left in
 // the LDS buffer, unsorted: the filter radius is their largest distance and
 // the sum runs in buffer order (lane-strided, then a wave reduction).
+//
+// The filter weight w = alpha (1 - (1 - e^{-beta d^2 / (2 r^2)}) / (1 - e^{-beta}))
+// is evaluated as A + B e^{x} with x = d^2 (-beta / (2 r^2)) from the squared
+// distance itself (no square root), per-query reciprocals and the hardware
+// exponential; against the reference's FP32 sequence of square root,
+// divisions and expf this moves each weight by a few 1e-7 relative, inside
+// the estimate's 1e-5 bound. The cos > 0 test keeps the reference's rounding.
 template <int CAP>
 __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLds<CAP>& L, uint32_t lane, uint32_t q,
                                                 uint32_t m) {
     const float alpha = 1.818f, beta = 1.953f;
     const float pi_f = 3.141592654f;
+    // 1 / (1 - e^{-beta}) and log2(e), rounded to FP32
+    const float inv_denom = 1.16529458f, log2e = 1.44269504f;
     float dmax = 0.f;
     for (uint32_t e = lane; e < m; e += 64) dmax = fmaxf(dmax, __uint_as_float((uint32_t)(L.buf[e] >> 32)));
 #pragma unroll
@@ -354,7 +381,9 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
     float res0 = 0.f, res1 = 0.f, res2 = 0.f;
     if (m > 0) {
         const float filter_r = __fmul_rn(rmax, rmax);
-        const float denom = 1.f - expf(-beta);
+        // x log2(e) = d^2 * s
+        const float s = __fdiv_rn(-beta, 2.f * filter_r) * log2e;
+        const float wa = alpha * (1.f - inv_denom), wb = alpha * inv_denom;
         const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
         for (uint32_t e = lane; e < m; e += 64) {
             const uint64_t v = L.buf[e];
@@ -362,9 +391,7 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
             const float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
             const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
             if (cosNL > 0.f) {
-                const float d = sqrtf(__uint_as_float((uint32_t)(v >> 32)));
-                const float x = __fdiv_rn(__fmul_rn(__fmul_rn(-beta, d), d), __fmul_rn(2.f, filter_r));
-                const float w = __fmul_rn(alpha, 1.f - __fdiv_rn(1.f - expf(x), denom));
+                const float w = wa + wb * __builtin_amdgcn_exp2f(__uint_as_float((uint32_t)(v >> 32)) * s);
                 res0 += P.power[idx] * w;
                 res1 += P.power[P.n + idx] * w;
                 res2 += P.power[2 * (size_t)P.n + idx] * w;
@@ -412,7 +439,8 @@ __device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint
 }
 
 // The candidate box of query q: its cell +- rc, clamped (PhotonGrid.cu:262-272).
-__device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, float& qx, float& qy, float& qz) {
+__device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, float& qx, float& qy, float& qz,
+                                       int* cell = nullptr) {
     // the query is the wave's: its coordinates (and so the box, the row
     // count and the cursor of walk) are wave-uniform, held in SGPRs
     qx = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qx[q])));
@@ -424,6 +452,7 @@ __device__ __forceinline__ void box_of(const KnnParams& P, uint32_t q, Box& B, f
         int c = pmd::cell_coord(qv[a], P.G.mn[a], P.G.cell, P.G.g[a]);
         B.lo[a] = max(0, c - P.rc);
         B.hi[a] = min(P.G.g[a] - 1, c + P.rc);
+        if (cell) cell[a] = c;
     }
 }
 
@@ -448,45 +477,96 @@ __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
     return true;
 }
 
+// One step of the streaming selection over two candidates per lane (index
+// sa / sb, squared distance d2a / d2b, valid flags va / vb): counts the
+// candidates within the radius, appends the in-radius keys below the
+// acceptance bound khi to the LDS buffer, and narrows the buffer first when
+// the step would overflow it.
+template <int CAP>
+__device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k, uint32_t bT, float T, bool va,
+                                      uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b, uint32_t& cnt,
+                                      uint32_t& found, uint64_t& khi) {
+    // each ballot is taken of a plain compare (no bool round trip): an
+    // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
+    const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
+    const uint64_t ba = wballot(ina), bb = wballot(inb);
+    found += (uint32_t)(__popcll(ba) + __popcll(bb));
+    const uint64_t ka = make_key(d2a, sa), kb = make_key(d2b, sb);
+    bool la = ka < khi, lb = kb < khi;
+    uint64_t bala = ba & wballot(la), balb = bb & wballot(lb);
+    uint32_t na = (uint32_t)__popcll(bala), nb = (uint32_t)__popcll(balb);
+    if (cnt + na + nb > (uint32_t)CAP) {
+        // leaves at most CAP / 2 keys, room for this step's 128
+        wave_sync();
+        cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, bT, khi);
+        la = ka < khi;
+        lb = kb < khi;
+        bala = ba & wballot(la);
+        balb = bb & wballot(lb);
+        na = (uint32_t)__popcll(bala);
+        nb = (uint32_t)__popcll(balb);
+    }
+    if (ina && la) L.buf[cnt + mbcnt(bala)] = ka;
+    if (inb && lb) L.buf[cnt + na + mbcnt(balb)] = kb;
+    cnt += na + nb;
+}
+
 // Single pass: in-radius keys stream into an LDS buffer of CAP keys; when it
 // would overflow, buf_select keeps only the smallest bin range holding >= k
 // keys and tightens the acceptance bound. Exact: a dropped key has >= k kept
 // keys below it.
+//
+// With P.h > 0 the walk first covers only the inner box (the query's cell
+// +- h). If that finds at least k keys within the radius and the k-th
+// smallest of them is closer than every photon outside the inner box can
+// be, those k keys are the answer (m = k); otherwise the query walks the
+// whole box again from an empty buffer. Both outcomes are exact.
 template <int CAP>
 __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
     Box B;
     float qx, qy, qz;
-    box_of(P, q, B, qx, qy, qz);
+    int qc[3];
+    box_of(P, q, B, qx, qy, qz, qc);
     const float T = P.T;
     const uint32_t k = (uint32_t)P.k;
-    uint32_t cnt = 0, found = 0;
-    uint64_t khi = ~0ull;
-    walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
-        // each ballot is taken of a plain compare (no bool round trip): an
-        // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
-        const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
-        const uint64_t ba = wballot(ina), bb = wballot(inb);
-        found += (uint32_t)(__popcll(ba) + __popcll(bb));
-        const uint64_t ka = make_key(d2a, sa), kb = make_key(d2b, sb);
-        bool la = ka < khi, lb = kb < khi;
-        uint64_t bala = ba & wballot(la), balb = bb & wballot(lb);
-        uint32_t na = (uint32_t)__popcll(bala), nb = (uint32_t)__popcll(balb);
-        if (cnt + na + nb > (uint32_t)CAP) {
-            // leaves at most CAP / 2 keys, room for this step's 128
-            wave_sync();
-            cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, P.bT, khi);
-            la = ka < khi;
-            lb = kb < khi;
-            bala = ba & wballot(la);
-            balb = bb & wballot(lb);
-            na = (uint32_t)__popcll(bala);
-            nb = (uint32_t)__popcll(balb);
+    // the inner box first when asked for; gbits: the d2 bits below which a
+    // key is closer than every photon outside it (less a few ulps of
+    // rounding: every such photon has d2 >= gap^2)
+    Box W = B;
+    uint32_t gbits = 0;
+    bool inner = P.h > 0;
+    if (inner) {
+        const float qv[3] = {qx, qy, qz};
+        float gap = __builtin_inff();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            W.lo[a] = max(0, qc[a] - P.h);
+            W.hi[a] = min(P.G.g[a] - 1, qc[a] + P.h);
+            gap = fminf(gap, inner_gap(qv[a], W.lo[a], W.hi[a], P.G.g[a], P.G.mn[a], P.G.cell));
         }
-        if (ina && la) L.buf[cnt + mbcnt(bala)] = ka;
-        if (inb && lb) L.buf[cnt + na + mbcnt(balb)] = kb;
-        cnt += na + nb;
-    });
-    wave_sync();
+        gbits = __builtin_amdgcn_readfirstlane(min(__float_as_uint(gap * gap * 0.999f), __float_as_uint(T) + 1u));
+    }
+    uint32_t cnt, found;
+    uint64_t khi;
+    for (;;) {
+        // in the inner box only keys closer than every photon outside it are
+        // buffered: the acceptance bound starts at gbits
+        cnt = 0;
+        found = 0;
+        khi = inner ? (uint64_t)gbits << 32 : ~0ull;
+        walk2(P, lane, W, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
+            push2<CAP>(L, lane, k, P.bT, T, va, sa, d2a, vb, sb, d2b, cnt, found, khi);
+        });
+        wave_sync();
+        // with at least k such keys, the k smallest keys overall are among
+        // them (the buffer holds every in-radius key below its bound, or at
+        // least k of them once narrowed)
+        if (!inner || cnt >= k) break;
+        // otherwise the whole box from an empty buffer
+        W = B;
+        inner = false;
+        wave_sync();
+    }
     finish<CAP>(P, L, lane, q, cnt, found, khi);
 }
 
@@ -1141,6 +1221,7 @@ pm_status setup(pm_ctx* ctx, const pm_grid* grid, const float* d_q, uint32_t nq,
     P.rc = (int)rcf;
     P.k = k;
     P.remap = (flags & PM_KNN_NO_XCD_REMAP) ? 0 : (flags & PM_KNN_XCD_CONTIGUOUS) ? 1 : 2;
+    P.h = (flags & PM_KNN_INNER_BOX) ? P.rc / 2 : 0;
     return PM_OK;
 }
 

@@ -33,11 +33,12 @@ KNN_TWO_PASS = 1 << 4
 KNN_LARGE_BUFFER = 1 << 5
 KNN_BATCH_QUEUE = 1 << 6
 KNN_XCD_CONTIGUOUS = 1 << 7
+KNN_INNER_BOX = 1 << 8
 BOX_REFERENCE = 1
 # gather variants (tuning flags): the default is the 512-key buffer with
 # XCD-interleaved chunks of workgroups
 KNN_VARIANTS = {"stream512_chunked": 0, "stream512": KNN_XCD_CONTIGUOUS, "stream1024": KNN_LARGE_BUFFER,
-                "two_pass": KNN_TWO_PASS}
+                "two_pass": KNN_TWO_PASS, "inner": KNN_INNER_BOX}
 
 _P = C.c_void_p
 _U32 = C.c_uint32

@@ -51,8 +51,12 @@ enum {
     PM_KNN_LARGE_BUFFER = 1u << 5,   /* tuning: 1024-key LDS buffer per query instead of 512 (fewer
                                         narrowing passes, lower occupancy) */
     PM_KNN_BATCH_QUEUE = 1u << 6,    /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
-    PM_KNN_XCD_CONTIGUOUS = 1u << 7  /* tuning: each XCD takes one contiguous eighth of the workgroups
+    PM_KNN_XCD_CONTIGUOUS = 1u << 7, /* tuning: each XCD takes one contiguous eighth of the workgroups
                                         instead of interleaved chunks of 256 */
+    PM_KNN_INNER_BOX = 1u << 8       /* k <= 256: each query first walks only its cell +- ceil(r/cell)/2 and
+                                        stops there when the k-th nearest found is closer than anything
+                                        outside; otherwise it walks the whole box (pays off when cells are
+                                        small against r and k is well below the photons within r) */
 };
 
 typedef struct pm_ctx pm_ctx;

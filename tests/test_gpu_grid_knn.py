@@ -335,35 +335,80 @@ def test_knn_parity_across_clamped_boundary(ctx, layout, cell):
     assert og.last_stats["found"] <= cands <= og.last_stats["cands"]
 
 
-def test_headline_config_radiance_parity(ctx):
+@pytest.mark.parametrize("cell", [0.005, 0.01])
+def test_headline_config_radiance_parity(ctx, cell):
     """The benchmark's step exactly as timed: 16M photons / 4M queries,
-    k = 64, r = 0.01, grid cell = r/2, gather variant stream512_chunked with
-    queries processed in cell order, fused radiance. Kept counts are
-    bit-exact and the radiance within 1e-5 relative of the oracle's FP64
-    estimate on 4000 sampled queries (oracle grid at the same cell size, so
-    the (d2, grid slot) tie order is the same); every query's output obeys
-    the size-independent bounds."""
+    k = 64, r = 0.01, grid cell = r/2 (and the reference's cell = r), every
+    gather variant the benchmark's tuner may pick with queries processed in
+    cell order, fused radiance. Kept counts are bit-exact and the radiance
+    within 1e-5 relative of the oracle's FP64 estimate on 4000 sampled
+    queries (oracle grid at the same cell size, so the (d2, grid slot) tie
+    order is the same); every query's output obeys the size-independent
+    bounds; and the inner-box walk's neighbour lists over all 4M queries
+    equal the whole-box walk's."""
     import torch
 
     ph, q, qn, aabb = gather_microbench(seed=1234)
-    r, k, cell = 0.01, 64, 0.005
-    flags = pm355.KNN_SORT_QUERIES | pm355.KNN_VARIANTS["stream512_chunked"]
+    r, k = 0.01, 64
     g = ctx.grid_build(torch.from_numpy(ph).cuda(), aabb, cell_size=cell)
-    rgb, cnt = ctx.gather_radiance(g, torch.from_numpy(q).cuda(), torch.from_numpy(qn).cuda(), r, k, flags=flags,
-                                   want_count=True)
-    ctx.sync()
-    rgb_h, cnt_h = _np(rgb), _np(cnt, np.uint32)
-    assert np.all(cnt_h <= k) and np.all(np.isfinite(rgb_h)) and np.all(rgb_h >= 0)
-    assert np.all(rgb_h[:, cnt_h == 0] == 0)
+    dq, dn = torch.from_numpy(q).cuda(), torch.from_numpy(qn).cuda()
     og = O.OracleGrid(ph, aabb, cell_size=cell)
     sel = np.random.default_rng(5).choice(q.shape[1], 4000, replace=False)
     oi, od, oc = og.knn(q[:, sel], r, k)
     _, o64 = O.radiance(oi, od, oc, og.photons, qn[:, sel], k)
-    assert np.array_equal(cnt_h[sel], oc)
-    got = rgb_h[:, sel].astype(np.float64)
-    nz = o64 != 0
-    assert np.all(got[~nz] == 0)
-    rel = np.abs(got[nz] - o64[nz]) / np.abs(o64[nz])
-    assert rel.max() <= RADIANCE_RTOL, rel.max()
     assert (oc == k).mean() > 0.5  # most sampled queries fill k
+    nz = o64 != 0
+    for name, variant in pm355.KNN_VARIANTS.items():
+        flags = pm355.KNN_SORT_QUERIES | variant
+        rgb, cnt = ctx.gather_radiance(g, dq, dn, r, k, flags=flags, want_count=True)
+        ctx.sync()
+        rgb_h, cnt_h = _np(rgb), _np(cnt, np.uint32)
+        assert np.all(cnt_h <= k) and np.all(np.isfinite(rgb_h)) and np.all(rgb_h >= 0), name
+        assert np.all(rgb_h[:, cnt_h == 0] == 0), name
+        assert np.array_equal(cnt_h[sel], oc), name
+        got = rgb_h[:, sel].astype(np.float64)
+        assert np.all(got[~nz] == 0), name
+        rel = np.abs(got[nz] - o64[nz]) / np.abs(o64[nz])
+        assert rel.max() <= RADIANCE_RTOL, (name, rel.max())
+        del rgb, cnt
+    S = pm355.KNN_SORT_QUERIES
+    ai, ad, ac = ctx.knn(g, dq, r, k, flags=S | pm355.KNN_VARIANTS["stream512_chunked"])
+    bi, bd, bc = ctx.knn(g, dq, r, k, flags=S | pm355.KNN_VARIANTS["inner"])
+    ctx.sync()
+    assert torch.equal(ac, bc) and torch.equal(ai, bi) and torch.equal(ad.view(torch.int32), bd.view(torch.int32))
+    assert np.array_equal(_np(bi, np.uint32)[sel], oi)
+    g.close()
+
+
+@pytest.mark.parametrize("cell,r", [(0.005, 0.01), (0.0025, 0.01), (0.004, 0.012)])
+def test_inner_box_parity(ctx, cell, r):
+    """The inner-box walk (PM_KNN_INNER_BOX): a query stops after its cell
+    +- ceil(r/cell)/2 when the k-th nearest found there is closer than any
+    photon outside, else walks the whole box. Photon density varies across
+    the set (sparse, uniform and clustered regions, an empty half-space) so
+    both outcomes occur, with queries at the grid's edges and outside it;
+    neighbour lists, counts and radiance equal the oracle's for k from 1 to
+    256."""
+    rng = np.random.default_rng(41)
+    ph, q, qn, aabb = gather_microbench(seed=41, n_photons=500_000, n_queries=3_000, n_clusters=8)
+    keep = ~((ph[0] > 0.5) & (ph[2] > 0.5) & (rng.random(ph.shape[1]) < 0.97))  # a sparse quadrant
+    ph = np.ascontiguousarray(ph[:, keep])
+    q[:, :4] = np.array([[1.3, -0.2, 0.999, 0.0], [0.5, 0.5, 0.0, 1.0], [0.5, 0.5, 0.999, 0.0]], np.float32)
+    g = ctx.grid_build(ph, aabb, cell_size=cell)
+    og = O.OracleGrid(ph, aabb, cell_size=cell)
+    flags = pm355.KNN_VARIANTS["inner"]
+    for k in (1, 64, 100, 256):
+        oi, od, oc = og.knn(q, r, k)
+        _, o64 = O.radiance(oi, od, oc, og.photons, qn, k)
+        assert (oc == k).any() and (oc < k).any()
+        for f in (flags, flags | pm355.KNN_SORT_QUERIES):
+            gi, gd, gc = ctx.knn(g, q, r, k, flags=f)
+            rgb, cnt = ctx.gather_radiance(g, q, qn, r, k, flags=f, want_count=True)
+            ctx.sync()
+            _compare_knn(gi, gd, gc, oi, od, oc)
+            assert np.array_equal(_np(cnt, np.uint32), oc)
+            got = _np(rgb).astype(np.float64)
+            rel = np.abs(got - o64) / np.maximum(np.abs(o64), 1e-30)
+            rel[o64 == 0] = np.abs(got[o64 == 0])
+            assert rel.max() <= RADIANCE_RTOL, (k, f, rel.max())
     g.close()
