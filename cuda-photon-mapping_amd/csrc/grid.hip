@@ -1,4 +1,6 @@
-// Photon hash grid build: count -> wave64 scan -> stable scatter.
+// Photon hash grid build: cell keys -> stable radix sort of (key, index) ->
+// cell sizes from the sorted runs -> exclusive scan into starts -> one
+// gather of each photon's record into the cell-ordered SoA planes.
 // Replaces PhotonGrid::build (PhotonGrid.cuh:51-66; PhotonGrid.cu:41-218).
 #include <cmath>
 
@@ -28,42 +30,32 @@ __global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9
     rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
 }
 
-// starts[c] = first sorted position with key >= c (the exclusive scan of the
-// counts, PhotonGrid.cu:195-196), sizes[c] = starts[c+1] - starts[c].
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi,
-                                                    uint32_t v) {
+// Cell sizes from the sorted keys (the counts of PhotonGrid.cu:152-174): the
+// last position of each run of equal keys writes the run's length into its
+// cell; every other cell keeps the 0 it was filled with. The exclusive scan
+// of the sizes gives the starts (PhotonGrid.cu:195-196).
+__global__ __launch_bounds__(256) void k_cell_sizes(const uint32_t* __restrict__ skeys, uint32_t n,
+                                                   uint32_t* __restrict__ sizes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = skeys[i];
+    if (i + 1 < n && skeys[i + 1] == key) return;
+    // the run's first position: gallop back over equal keys, then a binary
+    // search inside the last step (loads grow with log(run length))
+    uint32_t hi = i, step = 1;
+    while (step <= hi && skeys[hi - step] == key) {
+        hi -= step;
+        step <<= 1;
+    }
+    uint32_t lo = step <= hi ? hi - step + 1 : 0;
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
-        if (a[mid] < v)
+        if (skeys[mid] < key)
             lo = mid + 1;
         else
             hi = mid;
     }
-    return lo;
-}
-
-// starts[c] = first sorted position with key >= c (lower bound), sizes[c] =
-// starts[c + 1] - starts[c]. A block of 256 cells first brackets its keys
-// with two full searches, then each cell searches only inside the bracket.
-__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ skeys, uint32_t n,
-                                                    uint32_t n_cells, uint32_t* __restrict__ starts,
-                                                    uint32_t* __restrict__ sizes) {
-    __shared__ uint32_t s_lb[257];
-    const uint32_t t = threadIdx.x, c0 = blockIdx.x * 256;
-    if (t < 2) {
-        const uint32_t v = c0 + 256 * t;
-        s_lb[256 * t] = v >= n_cells ? n : lower_bound_u32(skeys, 0, n, v);
-    }
-    __syncthreads();
-    const uint32_t lo = s_lb[0], hi = s_lb[256];
-    const uint32_t c = c0 + t;
-    const uint32_t lb = c < n_cells ? lower_bound_u32(skeys, lo, hi, c) : n;
-    if (t > 0) s_lb[t] = lb;
-    __syncthreads();
-    if (c < n_cells) {
-        starts[c] = lb;
-        sizes[c] = s_lb[t + 1] - lb;
-    }
+    sizes[key] = i + 1 - lo;
 }
 
 // analyze_density_kernel's max (PhotonGrid.cu:70-90), as a reduction.
@@ -196,7 +188,83 @@ pm_status build_empty(pm_ctx* ctx, const float aabb[6], float cell_size, int lay
     return PM_OK;
 }
 
+size_t grid_bytes(uint32_t n_cells, uint32_t n_cap) {
+    return pm::carve_size({(size_t)n_cells * 4, (size_t)n_cells * 4, (size_t)n_cap * 36, (size_t)n_cap * 4});
+}
+
+// scratch of one build: the density counter, keys, sorted keys, sorted
+// order, radix work, records
+size_t build_scratch_bytes(uint32_t n) {
+    return pm::carve_size({256, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, pm::radix_sort_scratch_bytes(n),
+                           (size_t)n * 48});
+}
+
+// Enqueue the build of n photons at dims dd into g's memory (capacity
+// g->n_cap >= n): keys + records, stable key sort, cell sizes from the sorted
+// runs, starts by exclusive scan, record gather. No host synchronisation.
+pm_status enqueue_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6], const HostDims& dd,
+                        int layout, pm_grid* g) {
+    pm::Carve gc{static_cast<char*>(g->mem)};
+    uint32_t* starts = gc.take<uint32_t>(dd.n_cells);
+    uint32_t* sizes = gc.take<uint32_t>(dd.n_cells);
+    float* photons = gc.take<float>((size_t)g->n_cap * 9);
+    uint32_t* perm = gc.take<uint32_t>(g->n_cap);
+    pm_grid_desc& D = g->desc;
+    for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
+    D.cell_size = dd.cell;
+    for (int a = 0; a < 3; ++a) D.dims[a] = dd.g[a];
+    D.layout = layout;
+    D.n_cells = dd.n_cells;
+    D.n_photons = n;
+    D.d_cell_starts = starts;
+    D.d_cell_sizes = sizes;
+    D.d_photons = photons;
+    D.d_perm = perm;
+    pm_status st;
+    if (n == 0) {
+        if ((st = pm::fill_u32(ctx, starts, 0u, dd.n_cells)) != PM_OK) return st;
+        return pm::fill_u32(ctx, sizes, 0u, dd.n_cells);
+    }
+    void* work = nullptr;
+    if ((st = pm::scratch_reserve(ctx, build_scratch_bytes(n), &work)) != PM_OK) return st;
+    pm::Carve cv{static_cast<char*>(work)};
+    cv.take<uint32_t>(64);
+    uint32_t* keys = cv.take<uint32_t>(n);
+    uint32_t* skeys = cv.take<uint32_t>(n);
+    uint32_t* order = cv.take<uint32_t>(n);
+    void* rwork = cv.take<char>(pm::radix_sort_scratch_bytes(n));
+    float4* rec = cv.take<float4>((size_t)n * 3);
+    const uint32_t blocks = pm::div_up(n, 256);
+    const GridGeom G = geom_of(aabb, dd, layout);
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
+    PM_CHECK_LAUNCH(ctx, "k_grid_keys");
+    if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(dd.n_cells), rwork)) != PM_OK)
+        return st;
+    if ((st = pm::fill_u32(ctx, sizes, 0u, dd.n_cells)) != PM_OK) return st;
+    hipLaunchKernelGGL(k_cell_sizes, dim3(blocks), dim3(256), 0, ctx->stream, skeys, n, sizes);
+    PM_CHECK_LAUNCH(ctx, "k_cell_sizes");
+    if ((st = pm::scan_u32(ctx, sizes, starts, dd.n_cells, false, nullptr)) != PM_OK) return st;
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, rec, n, order, photons, perm);
+    PM_CHECK_LAUNCH(ctx, "k_grid_scatter");
+    return PM_OK;
+}
+
 }  // namespace
+
+extern "C" pm_status pm_grid_rebuild(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n) {
+    if (!ctx || !grid) return PM_ERR_INVALID;
+    if (n > 0 && !d_photons9) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_rebuild: null photons");
+    if (n > grid->n_cap)
+        return pm::fail(ctx, PM_ERR_CAPACITY, "pm_grid_rebuild: %u photons exceed the grid's capacity of %u", n,
+                        grid->n_cap);
+    HostDims dd;
+    dd.cell = grid->desc.cell_size;
+    for (int a = 0; a < 3; ++a) dd.g[a] = grid->desc.dims[a];
+    dd.n_cells = grid->desc.n_cells;
+    float aabb[6];
+    for (int a = 0; a < 6; ++a) aabb[a] = grid->desc.aabb[a];
+    return enqueue_build(ctx, d_photons9, n, aabb, dd, grid->desc.layout, grid);
+}
 
 extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
                                    float cell_size, int32_t desired, int32_t layout, pm_grid** out) {
@@ -210,71 +278,36 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: need cell_size > 0 or desired_per_cell > 0");
 
     if (n == 0) return build_empty(ctx, aabb, cell_size, layout, out);
-    const uint32_t blocks = pm::div_up(n, 256);
 
     HostDims d;
     float cell = cell_size > 0.f ? cell_size : reference_cell(n, aabb, desired);
     if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
         return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_grid_build: grid too large for cell %g", (double)cell);
 
-    // scratch: keys, sorted keys, sorted order, radix work, max counter, records
-    size_t rs = pm::radix_sort_scratch_bytes(n);
-    void* work = nullptr;
     pm_status st;
-    size_t total = pm::carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, 256, (size_t)n * 48});
-    if ((st = pm::scratch_reserve(ctx, total, &work)) != PM_OK) return st;
-    pm::Carve cv{static_cast<char*>(work)};
-    uint32_t* keys = cv.take<uint32_t>(n);
-    uint32_t* skeys = cv.take<uint32_t>(n);
-    uint32_t* order = cv.take<uint32_t>(n);
-    void* rwork = cv.take<char>(rs);
-    uint32_t* dmax = cv.take<uint32_t>(64);
-    float4* rec = cv.take<float4>((size_t)n * 3);
+    uint32_t* dmax = nullptr;
+    {
+        void* w = nullptr;
+        if ((st = pm::scratch_reserve(ctx, build_scratch_bytes(n), &w)) != PM_OK) return st;
+        pm::Carve cv{static_cast<char*>(w)};
+        dmax = cv.take<uint32_t>(64);
+    }
 
-    // One build at dims d: keys + records, key sort, cell bounds, scatter.
+    // One build at dims d into a fresh allocation of capacity n.
     auto build_once = [&](const HostDims& dd, pm_grid*& out_g) -> pm_status {
         pm_grid* g = new pm_grid();
-        size_t bytes = pm::carve_size({(size_t)dd.n_cells * 4, (size_t)dd.n_cells * 4, (size_t)n * 36, (size_t)n * 4});
-        hipError_t e = hipMalloc(&g->mem, bytes);
+        hipError_t e = hipMalloc(&g->mem, grid_bytes(dd.n_cells, n));
         if (e != hipSuccess) {
             delete g;
             return pm::hip_fail(ctx, e, "hipMalloc(grid)");
         }
-        pm::Carve gc{static_cast<char*>(g->mem)};
-        uint32_t* starts = gc.take<uint32_t>(dd.n_cells);
-        uint32_t* sizes = gc.take<uint32_t>(dd.n_cells);
-        float* photons = gc.take<float>((size_t)n * 9);
-        uint32_t* perm = gc.take<uint32_t>(n);
-
-        GridGeom G = geom_of(aabb, dd, layout);
-        auto bail = [&](pm_status s) {
+        g->n_cap = n;
+        pm_status bs = enqueue_build(ctx, d_photons9, n, aabb, dd, layout, g);
+        if (bs != PM_OK) {
             (void)hipFree(g->mem);
             delete g;
-            return s;
-        };
-        pm_status bs;
-        hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
-        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_keys launch"));
-        if ((bs = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(dd.n_cells), rwork)) !=
-            PM_OK)
-            return bail(bs);
-        hipLaunchKernelGGL(k_cell_bounds, dim3(pm::div_up(dd.n_cells, 256)), dim3(256), 0, ctx->stream, skeys, n,
-                           dd.n_cells, starts, sizes);
-        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_cell_bounds launch"));
-        hipLaunchKernelGGL(k_grid_scatter, dim3(blocks), dim3(256), 0, ctx->stream, rec, n, order, photons, perm);
-        if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "k_grid_scatter launch"));
-
-        pm_grid_desc& D = g->desc;
-        for (int a = 0; a < 6; ++a) D.aabb[a] = aabb[a];
-        D.cell_size = dd.cell;
-        for (int a = 0; a < 3; ++a) D.dims[a] = dd.g[a];
-        D.layout = layout;
-        D.n_cells = dd.n_cells;
-        D.n_photons = n;
-        D.d_cell_starts = starts;
-        D.d_cell_sizes = sizes;
-        D.d_photons = photons;
-        D.d_perm = perm;
+            return bs;
+        }
         out_g = g;
         return PM_OK;
     };

@@ -44,6 +44,7 @@ struct pm_ctx {
 struct pm_grid {
     pm_grid_desc desc{};
     void* mem = nullptr;  // one allocation backing starts/sizes/photons/perm
+    uint32_t n_cap = 0;   // photons the allocation holds (pm_grid_rebuild)
 };
 
 struct pm_kdtree {

@@ -76,6 +76,7 @@ _SIGS = {
     "pm_last_error": (C.c_char_p, [_P]),
     "pm_scan_u32": (_I32, [_P, _P, _P, _U32, _I32, _P]),
     "pm_grid_build": (_I32, [_P, _P, _U32, C.POINTER(_F32), _F32, _I32, _I32, C.POINTER(_P)]),
+    "pm_grid_rebuild": (_I32, [_P, _P, _P, _U32]),
     "pm_grid_describe": (_I32, [_P, C.POINTER(GridDesc)]),
     "pm_grid_copy_to_host": (_I32, [_P, _P, _P, _P, _P, _P]),
     "pm_grid_destroy": (_I32, [_P]),
@@ -520,6 +521,20 @@ class Grid:
         ctx.check(ctx.lib.pm_grid_describe(h, C.byref(d)))
         self.desc = d
         self.info = GridInfo(tuple(d.aabb), d.cell_size, tuple(d.dims), d.layout, d.n_cells, d.n_photons)
+
+    def rebuild(self, photons9, n: int | None = None):
+        """pm_grid_rebuild: the same cells over a new photon block (at most
+        the photon count the grid was built with)."""
+        ph = _f32_dev(photons9, self.ctx.device)
+        assert ph.dim() == 2 and ph.shape[0] == 9, "photons9 must be [9, n]"
+        n = ph.shape[1] if n is None else int(n)
+        self.ctx.check(self.ctx.lib.pm_grid_rebuild(self.ctx.h, self.h, _ptr(ph), n))
+        self._keep = ph
+        d = GridDesc()
+        self.ctx.check(self.ctx.lib.pm_grid_describe(self.h, C.byref(d)))
+        self.desc = d
+        self.info = GridInfo(tuple(d.aabb), d.cell_size, tuple(d.dims), d.layout, d.n_cells, d.n_photons)
+        return self
 
     def to_host(self):
         d = self.desc

@@ -95,10 +95,12 @@ pm_status pm_scan_u32(pm_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32
 /* ---- hash grid: PhotonGrid(int, AABB) + PhotonGrid::build
  * (PhotonGrid.cuh:44-66, PhotonGrid.cu:41-218). cell_size > 0 is used as
  * given; cell_size <= 0 applies the reference rule cbrt(V / ceil(n/desired))
- * with its density re-check (PhotonGrid.cu:92-141). The build is count ->
- * exclusive scan -> stable scatter: in-cell order is ascending input index,
- * so the grid is deterministic (the reference's atomicAdd slot is not,
- * PhotonGrid.cu:167). */
+ * with its density re-check (PhotonGrid.cu:92-141). The build sorts the
+ * photons' cell keys stably (radix sort of (key, index)), takes each cell's
+ * size from the sorted runs and its start from their exclusive scan, and
+ * gathers the photons into cell order: in-cell order is ascending input
+ * index, so the grid is deterministic (the reference's atomicAdd slot is
+ * not, PhotonGrid.cu:167). */
 typedef struct {
     float aabb[6];       /* min xyz, max xyz */
     float cell_size;
@@ -114,6 +116,14 @@ typedef struct {
 
 pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6], float cell_size,
                         int32_t desired_per_cell, int32_t layout, pm_grid** out);
+
+/* Rebuild a grid in place from n photons (n <= the photon count it was built
+ * with), keeping its cell size, dims and layout: the per-frame rebuild of a
+ * resident grid (the reference builds once per photon pass,
+ * Initializer.cu:157-160). Allocates nothing and never synchronises with the
+ * host once the context has served a build of n photons, so it can be
+ * captured into a HIP graph. PM_ERR_CAPACITY if n is too large. */
+pm_status pm_grid_rebuild(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n);
 pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out);
 /* PhotonGrid::copy_to_cpu (PhotonGrid.cuh:75-98). Any output may be NULL. */
 pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint32_t* h_starts, uint32_t* h_sizes,

@@ -412,3 +412,30 @@ def test_inner_box_parity(ctx, cell, r):
             rel[o64 == 0] = np.abs(got[o64 == 0])
             assert rel.max() <= RADIANCE_RTOL, (k, f, rel.max())
     g.close()
+
+
+def test_grid_rebuild_in_place(ctx):
+    """pm_grid_rebuild: a grid rebuilt in place from another photon block (as
+    many photons, then fewer) equals a grid built fresh from that block, and
+    its gathers match; more photons than its capacity are refused."""
+    import torch
+
+    ph1, q, qn, aabb = gather_microbench(seed=61, n_photons=300_000, n_queries=3_000, n_clusters=8)
+    ph2, _, _, _ = gather_microbench(seed=62, n_photons=300_000, n_queries=10, n_clusters=8)
+    g = ctx.grid_build(ph1, aabb, cell_size=0.004)
+    for ph in (ph2, np.ascontiguousarray(ph2[:, :123_457]), ph1):
+        g.rebuild(ph)
+        f = ctx.grid_build(ph, aabb, cell_size=0.004)
+        a, b = g.to_host(), f.to_host()
+        for x, y in zip(a, b):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+        for flags in (0, pm355.KNN_SORT_QUERIES | pm355.KNN_INNER_BOX):
+            ra, ca = ctx.gather_radiance(g, q, qn, 0.01, 64, flags=flags, want_count=True)
+            rb, cb = ctx.gather_radiance(f, q, qn, 0.01, 64, flags=flags, want_count=True)
+            ctx.sync()
+            assert torch.equal(ca, cb) and torch.equal(ra.view(torch.int32), rb.view(torch.int32))
+        f.close()
+    with pytest.raises(pm355.PMError) as e:
+        g.rebuild(np.zeros((9, 300_001), np.float32))
+    assert e.value.code == pm355.PM_ERR_CAPACITY
+    g.close()
