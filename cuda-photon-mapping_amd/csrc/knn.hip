@@ -62,6 +62,9 @@ struct KnnParams {
     float* rgb;  // 3 planes of nq (radiance only)
     int remap;
     int h;  // inner box half-width in cells (0: walk the whole box at once)
+    // (nullable) the query count on the device, at most nq: positions past
+    // it are skipped; nq stays the planes' stride
+    const uint32_t* nq_dev;
 };
 
 struct Box {
@@ -472,7 +475,7 @@ __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
                          : P.remap   ? pmd::xcd_remap(blockIdx.x, gridDim.x)
                                      : blockIdx.x;
     const uint32_t ql = blk * kWaves + wave;
-    if (ql >= P.nq) return false;
+    if (ql >= P.nq || (P.nq_dev && ql >= *P.nq_dev)) return false;
     q = __builtin_amdgcn_readfirstlane(P.order ? P.order[ql] : ql);
     return true;
 }
@@ -1113,9 +1116,10 @@ __global__ __launch_bounds__(256) void k_batch_query_keys(const KnnParams* __res
 // Cell key of each query for the optional cell-ordered processing.
 __global__ __launch_bounds__(256) void k_query_keys(GridGeom G, const float* __restrict__ qx,
                                                    const float* __restrict__ qy, const float* __restrict__ qz,
-                                                   uint32_t nq, uint32_t* __restrict__ keys) {
+                                                   uint32_t nq, uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ nq_dev) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
+    if (i >= nq || (nq_dev && i >= *nq_dev)) return;
     int x = pmd::cell_coord(qx[i], G.mn[0], G.cell, G.g[0]);
     int y = pmd::cell_coord(qy[i], G.mn[1], G.cell, G.g[1]);
     int z = pmd::cell_coord(qz[i], G.mn[2], G.cell, G.g[2]);
@@ -1238,11 +1242,11 @@ pm_status query_order(pm_ctx* ctx, const pm_grid* grid, const KnnParams& P, uint
     uint32_t* order = order_out ? order_out : own;
     void* rw = cv.take<char>(rs);
     hipLaunchKernelGGL(k_query_keys, dim3(pm::div_up(P.nq, 256)), dim3(256), 0, ctx->stream, P.G, P.qx, P.qy, P.qz,
-                       P.nq, keys);
+                       P.nq, keys, P.nq_dev);
     PM_CHECK_LAUNCH(ctx, "k_query_keys");
     int bits = 1;
     while (bits < 32 && (1ull << bits) < (uint64_t)grid->desc.n_cells) ++bits;
-    return pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, P.nq, bits, rw);
+    return pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, P.nq, bits, rw, P.nq_dev);
 }
 
 pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags) {
@@ -1352,6 +1356,21 @@ extern "C" pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, c
     P.out_count = d_count;
     return launch(ctx, grid, P, flags);
 }
+
+namespace pm {
+pm_status gather_radiance_counted(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, const float* d_normals3,
+                                  uint32_t cap, const uint32_t* d_nq, float radius, int32_t k, uint32_t flags,
+                                  float* d_rgb3, uint32_t* d_count) {
+    KnnParams P;
+    pm_status st = setup(ctx, grid, d_queries3, cap, radius, k, flags, P, "gather (device count)");
+    if (st != PM_OK) return st;
+    P.nq_dev = d_nq;
+    P.nrm = d_normals3;
+    P.rgb = d_rgb3;
+    P.out_count = d_count;
+    return launch(ctx, grid, P, flags);
+}
+}  // namespace pm
 
 extern "C" pm_status pm_radiance(pm_ctx* ctx, const pm_grid* grid, const uint32_t* d_idx, const float* d_dist,
                                  const uint32_t* d_count, const float* d_normals3, uint32_t nq, int32_t k,

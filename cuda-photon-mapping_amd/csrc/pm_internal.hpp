@@ -121,14 +121,44 @@ pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, b
 
 // Stable LSD radix sort of (key, value) pairs over key bits [0, key_bits).
 // Sorted output ends in keys_out / vals_out. vals_in may be null (identity).
+// n_dev (nullable): the element count on the device, at most n; only the
+// first *n_dev elements are sorted (and written).
 pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                            uint32_t* vals_out, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                           uint32_t* hist, uint32_t* offs);
+                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev = nullptr);
 size_t radix_sort_scratch_bytes(uint32_t n);
 pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                                uint32_t* vals_out, uint32_t n, int key_bits, void* work);
+                                uint32_t* vals_out, uint32_t n, int key_bits, void* work,
+                                const uint32_t* n_dev = nullptr);
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
+
+// Planned photon traces (trace.hip): a fixed schedule of `paths` paths in
+// batches, enqueued without host synchronisation; the device fill counts
+// (u64 global, caustic) tell afterwards whether the maps filled.
+struct TracePlan;
+pm_status trace_plan_create(pm_ctx* ctx, const pm_scene* scene, uint32_t n_global, uint32_t n_caustic,
+                            int32_t max_bounces, uint64_t paths, TracePlan** out);
+pm_status trace_plan_enqueue(pm_ctx* ctx, TracePlan* T, uint64_t seed, float* d_global9, float* d_caustic9);
+const uint64_t* trace_plan_counts(const TracePlan* T);
+void trace_plan_destroy(TracePlan* T);
+
+// Planned frames (render.hip): pm_render_frame's phases with the hit count
+// kept on the device (query block sized for every pixel's full ray tree).
+struct FramePlan;
+pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera* camera,
+                            const pm_render_params* params, FramePlan** out);
+pm_status frame_plan_enqueue(pm_ctx* ctx, FramePlan* F, const pm_grid* global_grid, const pm_grid* caustic_grid,
+                             float* d_rgb3, uint8_t* d_rgb8);
+const uint64_t* frame_plan_hits(const FramePlan* F);
+void frame_plan_destroy(FramePlan* F);
+
+// Fused gather + radiance over a query block of capacity `cap` (the planes'
+// stride) whose count lives on the device (*d_nq <= cap): nothing is read
+// back to the host, so a caller can enqueue it before the count is known.
+pm_status gather_radiance_counted(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, const float* d_normals3,
+                                  uint32_t cap, const uint32_t* d_nq, float radius, int32_t k, uint32_t flags,
+                                  float* d_rgb3, uint32_t* d_count);
 
 // Device address of the allocation-free gather's fixed scratch on the
 // context's device (taken once per device; called by pm_ctx_create).

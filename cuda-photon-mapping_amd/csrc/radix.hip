@@ -17,10 +17,14 @@ constexpr int kWaveKeys = 64 * kItems;     // 1024 keys per wave
 constexpr int kTile = kThreads * kItems;   // 4096 keys per tile
 constexpr int kBins = 256;
 
+// n_dev (nullable): the element count on the device, at most n (the launch
+// covers n; tiles past the device count contribute nothing).
 __global__ __launch_bounds__(kThreads) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
-                                                     uint32_t mask, uint32_t* __restrict__ hist, uint32_t tiles) {
+                                                     uint32_t mask, uint32_t* __restrict__ hist, uint32_t tiles,
+                                                     const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t s_hist[kBins];
     const uint32_t t = threadIdx.x, tile = blockIdx.x;
+    if (n_dev) n = min(n, *n_dev);
     s_hist[t] = 0;
     __syncthreads();
     const size_t base = (size_t)tile * kTile;
@@ -37,7 +41,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restr
                                                         const uint32_t* __restrict__ vals_in,
                                                         uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                         uint32_t n, int shift, uint32_t mask,
-                                                        const uint32_t* __restrict__ offsets, uint32_t tiles) {
+                                                        const uint32_t* __restrict__ offsets, uint32_t tiles,
+                                                        const uint32_t* __restrict__ n_dev) {
+    if (n_dev) n = min(n, *n_dev);
+    if ((size_t)blockIdx.x * kTile >= n) return;  // (block-uniform, before any barrier)
     __shared__ uint32_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
     __shared__ uint32_t s_whist[kWaves][kBins];
@@ -136,7 +143,7 @@ size_t radix_sort_scratch_bytes(uint32_t n) {
 
 pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                            uint32_t* vals_out, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                           uint32_t* hist, uint32_t* offs) {
+                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev) {
     if (n == 0) return PM_OK;
     if (key_bits < 1) key_bits = 1;
     if (key_bits > 32) key_bits = 32;
@@ -152,12 +159,13 @@ pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t*
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* dk = to_out ? keys_out : keys_tmp;
         uint32_t* dv = to_out ? vals_out : vals_tmp;
-        hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kThreads), 0, ctx->stream, src_k, n, shift, mask, hist, tiles);
+        hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kThreads), 0, ctx->stream, src_k, n, shift, mask, hist, tiles,
+                           n_dev);
         PM_CHECK_LAUNCH(ctx, "k_rs_hist");
         pm_status s = scan_u32(ctx, hist, offs, kBins * tiles, false, nullptr);
         if (s != PM_OK) return s;
         hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(kThreads), 0, ctx->stream, src_k, src_v, dk, dv, n, shift,
-                           mask, offs, tiles);
+                           mask, offs, tiles, n_dev);
         PM_CHECK_LAUNCH(ctx, "k_rs_scatter");
         src_k = dk;
         src_v = dv;
@@ -167,14 +175,14 @@ pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t*
 
 // Sort using a caller-provided work block of radix_sort_scratch_bytes(n).
 pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                                uint32_t* vals_out, uint32_t n, int key_bits, void* work) {
+                                uint32_t* vals_out, uint32_t n, int key_bits, void* work, const uint32_t* n_dev) {
     uint32_t tiles = div_up(n ? n : 1, kTile);
     Carve cv{static_cast<char*>(work)};
     uint32_t* kt = cv.take<uint32_t>(n);
     uint32_t* vt = cv.take<uint32_t>(n);
     uint32_t* hist = cv.take<uint32_t>((size_t)kBins * tiles);
     uint32_t* offs = cv.take<uint32_t>((size_t)kBins * tiles);
-    return radix_sort_pairs(ctx, keys_in, vals_in, keys_out, vals_out, kt, vt, n, key_bits, hist, offs);
+    return radix_sort_pairs(ctx, keys_in, vals_in, keys_out, vals_out, kt, vt, n, key_bits, hist, offs, n_dev);
 }
 
 }  // namespace pm

@@ -529,3 +529,134 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
     }
     return pm_frame_finish(ctx, F, d_rgb3, d_rgb8, stats);
 }
+
+// ---------------------------------------------------------------------------
+// Planned frames: pm_render_frame's phases enqueued without reading the hit
+// count back. The query block is sized for every pixel's full ray tree
+// (npix * (2^max_depth - 1) hits); the hit count stays on the device and the
+// gathers skip positions past it. Same kernels, same frame.
+namespace pm {
+
+struct FramePlan {
+    void* mem = nullptr;
+    uint32_t W = 0, H = 0, npix = 0, cap = 0;
+    int hmax = 0, nm = 0;
+    pm_render_params R{};
+    RaysParams P{};
+    DevModel* d_models = nullptr;
+    float *hc = nullptr, *q3 = nullptr, *n3 = nullptr, *lg = nullptr, *lc = nullptr;
+    int* hm = nullptr;
+    uint32_t *count = nullptr, *off = nullptr, *ccount = nullptr;
+    uint64_t* total = nullptr;
+};
+
+pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera* camera,
+                            const pm_render_params* params, FramePlan** out) {
+    *out = nullptr;
+    if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
+        return fail(ctx, PM_ERR_INVALID, "frame plan: empty scene");
+    const pm_render_params& R = *params;
+    if (R.k_global < 1 || R.k_global > 2048 || R.k_caustic < 1 || R.k_caustic > 2048)
+        return fail(ctx, PM_ERR_UNSUPPORTED, "frame plan: k outside [1, 2048]");
+    if (R.max_depth < 1 || R.max_depth > kMaxDepth)
+        return fail(ctx, PM_ERR_UNSUPPORTED, "frame plan: max_depth outside [1, %d]", kMaxDepth);
+    if (quirk_mode(R)) return fail(ctx, PM_ERR_UNSUPPORTED, "frame plan: the reference-caustic mode is eager only");
+    if (camera->width <= 0 || camera->height <= 0) return fail(ctx, PM_ERR_INVALID, "frame plan: empty canvas");
+    const uint64_t npix64 = (uint64_t)camera->width * (uint64_t)camera->height;
+    const uint64_t cap64 = npix64 * (uint64_t)((1 << R.max_depth) - 1);
+    if (npix64 > (1ull << 26) || cap64 > (1ull << 30)) return fail(ctx, PM_ERR_UNSUPPORTED, "frame plan: canvas too large");
+    FramePlan* F = new FramePlan();
+    F->W = (uint32_t)camera->width;
+    F->H = (uint32_t)camera->height;
+    F->npix = (uint32_t)npix64;
+    F->hmax = (1 << R.max_depth) - 1;
+    F->cap = (uint32_t)cap64;
+    F->nm = scene->n_models;
+    F->R = R;
+    const size_t cap = F->cap, npix = F->npix;
+    const size_t bytes = carve_size({(size_t)F->nm * sizeof(DevModel), (size_t)F->nm * 32, cap * 12, cap * 12,
+                                     cap * 12, cap * 4, npix * 4, npix * 4, 64, cap * 12, cap * 12, cap * 12, cap * 12,
+                                     cap * 4});
+    hipError_t e = hipMalloc(&F->mem, bytes);
+    if (e != hipSuccess) {
+        delete F;
+        return hip_fail(ctx, e, "hipMalloc(frame plan)");
+    }
+    Carve cv{static_cast<char*>(F->mem)};
+    F->d_models = cv.take<DevModel>(F->nm);
+    void* tmp = cv.take<char>((size_t)F->nm * 32);
+    float* hp = cv.take<float>(cap * 3);
+    float* hn = cv.take<float>(cap * 3);
+    F->hc = cv.take<float>(cap * 3);
+    F->hm = cv.take<int>(cap);
+    F->count = cv.take<uint32_t>(npix);
+    F->off = cv.take<uint32_t>(npix);
+    F->total = cv.take<uint64_t>(1);
+    F->q3 = cv.take<float>(cap * 3);
+    F->n3 = cv.take<float>(cap * 3);
+    F->lg = cv.take<float>(cap * 3);
+    F->lc = cv.take<float>(cap * 3);
+    F->ccount = cv.take<uint32_t>(cap);
+    std::vector<DevModel> host_models;
+    pm_status st = upload_models(ctx, scene, F->d_models, tmp, host_models);
+    if (st == PM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(ctx, PM_ERR_HIP, "frame plan: upload");
+    if (st != PM_OK) {
+        frame_plan_destroy(F);
+        return st;
+    }
+    RaysParams& P = F->P;
+    P.tris = scene->d_tris9;
+    P.nrms = scene->d_normals9;
+    P.models = F->d_models;
+    P.n_models = F->nm;
+    P.bvh = bvh_nodes(scene);
+    P.def_ior = scene->default_refr_index;
+    std::memcpy(&P.cam, camera, sizeof(Cam));
+    P.max_depth = R.max_depth;
+    P.hmax = F->hmax;
+    P.hp = hp;
+    P.hn = hn;
+    P.hc = F->hc;
+    P.hm = F->hm;
+    P.count = F->count;
+    P.npix = F->npix;
+    *out = F;
+    return PM_OK;
+}
+
+void frame_plan_destroy(FramePlan* F) {
+    if (!F) return;
+    if (F->mem) (void)hipFree(F->mem);
+    delete F;
+}
+
+const uint64_t* frame_plan_hits(const FramePlan* F) { return F->total; }
+
+pm_status frame_plan_enqueue(pm_ctx* ctx, FramePlan* F, const pm_grid* global_grid, const pm_grid* caustic_grid,
+                             float* d_rgb3, uint8_t* d_rgb8) {
+    const pm_render_params& R = F->R;
+    const uint32_t tiles = ((F->W + 7) / 8) * ((F->H + 7) / 8);
+    hipLaunchKernelGGL(k_render_rays, dim3(div_up(tiles, 4)), dim3(256), 0, ctx->stream, F->P);
+    PM_CHECK_LAUNCH(ctx, "k_render_rays (plan)");
+    pm_status st = scan_u32(ctx, F->count, F->off, F->npix, false, F->total);
+    if (st != PM_OK) return st;
+    hipLaunchKernelGGL(k_render_compact, dim3(div_up(F->npix, 256)), dim3(256), 0, ctx->stream, F->P.hp, F->P.hn,
+                       F->count, F->off, F->npix, F->hmax, F->cap, F->q3, F->n3);
+    PM_CHECK_LAUNCH(ctx, "k_render_compact (plan)");
+    const uint32_t* nq = reinterpret_cast<const uint32_t*>(F->total);  // (the low word: hits < 2^32)
+    const uint32_t gflags = R.gather_flags | PM_KNN_SORT_QUERIES;
+    if ((st = gather_radiance_counted(ctx, global_grid, F->q3, F->n3, F->cap, nq, R.r_global, R.k_global, gflags, F->lg,
+                                      nullptr)) != PM_OK)
+        return st;
+    if (caustic_grid &&
+        (st = gather_radiance_counted(ctx, caustic_grid, F->q3, F->n3, F->cap, nq, R.r_caustic, R.k_caustic, gflags,
+                                      F->lc, F->ccount)) != PM_OK)
+        return st;
+    hipLaunchKernelGGL(k_render_combine, dim3(div_up(F->npix, 256)), dim3(256), 0, ctx->stream, F->d_models, F->hc,
+                       F->hm, F->count, F->off, F->npix, F->hmax, F->cap, F->lg, F->lc, F->ccount, R.k_caustic,
+                       caustic_grid ? 1 : 0, d_rgb3, d_rgb8);
+    PM_CHECK_LAUNCH(ctx, "k_render_combine (plan)");
+    return PM_OK;
+}
+
+}  // namespace pm

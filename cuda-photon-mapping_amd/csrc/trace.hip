@@ -56,7 +56,23 @@ struct TraceParams {
     uint64_t spec_mask;
     int spec_mask_valid;
     uint32_t n_segs;  // caustic phase: emission blocks (queue segments) of this batch
+    // Planned traces (a fixed batch schedule enqueued without host
+    // synchronisation): the maps' fill counts so far, on the device, decide
+    // each batch's phase; null for the host-driven loop.
+    const uint64_t* got;
+    uint32_t need_g, need_c;
 };
+
+// Phase of a batch: 0 global (both maps still filling), 1 caustic (global
+// map full), 2 done (both full). Host-driven batches carry theirs in
+// caustic_only; planned batches read the device fill counts written by the
+// previous batches.
+__device__ __forceinline__ int batch_phase(const TraceParams& P) {
+    if (!P.got) return P.caustic_only;
+    const uint64_t g = P.got[0], c = P.got[1];
+    if (g >= P.need_g && c >= P.need_c) return 2;
+    return g >= P.need_g ? 1 : 0;
+}
 
 // photon_trace_stack (PhotonTraceStack.cuh:37-116) as a register shift
 // stack of 16-bit model indices, the top entry in the low bits: an entry's
@@ -313,6 +329,11 @@ __global__ __launch_bounds__(kEmitBlock) void k_trace_emit(TraceParams P, PathSt
     __shared__ uint32_t s_wave[kEmitBlock / 64];
     const uint32_t i = blockIdx.x * kEmitBlock + threadIdx.x;
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    if (P.got && batch_phase(P) != 1) {
+        // a planned batch outside the caustic phase: no survivors here
+        if (threadIdx.x == 0) seg_len[blockIdx.x] = 0;
+        return;
+    }
     bool keep = false;
     Ray ray;
     Stream rng;
@@ -382,6 +403,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                    const uint64_t* __restrict__ n_queued) {
     __shared__ uint2 s_stack[kLdsStack * kStackLanes];
     __shared__ uint32_t s_next;
+    if (P.got) {
+        // planned batch: the queued kernels trace the caustic phase when the
+        // scene allows it, this one everything else; nothing once both maps
+        // are full
+        const int ph = batch_phase(P);
+        if (ph == 2 || (!kQueued && ph == 1 && P.spec_mask_valid) || (kQueued && ph != 1)) return;
+        P.caustic_only = ph;
+    }
     const BvhStack S{s_stack + threadIdx.x};
     const uint32_t total = kQueued ? (uint32_t)*n_queued : P.n_paths;
     const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
@@ -434,9 +463,18 @@ __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__
                                                       const uint32_t* __restrict__ off_g,
                                                       const uint32_t* __restrict__ off_c, uint32_t n_paths, int depth,
                                                       uint64_t base_g, uint64_t base_c, float* __restrict__ out_g,
-                                                      uint32_t cap_g, float* __restrict__ out_c, uint32_t cap_c) {
+                                                      uint32_t cap_g, float* __restrict__ out_c, uint32_t cap_c,
+                                                      const uint64_t* __restrict__ got) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_paths) return;
+    if (got) {
+        // planned batch: the bases are the device fill counts; a caustic-phase
+        // batch has no global counts
+        base_g = got[0];
+        base_c = got[1];
+        if (base_g >= cap_g && base_c >= cap_c) return;
+        if (base_g >= cap_g) cnt_g = nullptr;
+    }
     // cnt_g == nullptr: caustic phase, every staged photon is caustic
     const uint32_t ne = (cnt_g ? cnt_g[i] : 0u) + cnt_c[i];
     if (ne == 0) return;
@@ -453,38 +491,46 @@ __global__ __launch_bounds__(256) void k_trace_compact(const float* __restrict__
     }
 }
 
+// Planned batches: add the batch's totals to the device fill counts (a
+// caustic-phase batch stores no global photon).
+__global__ void k_trace_advance(uint64_t* __restrict__ got, const uint64_t* __restrict__ tot, uint32_t need_g) {
+    if (threadIdx.x != 0) return;
+    const bool caustic_phase = got[0] >= need_g;
+    got[0] += caustic_phase ? 0ull : tot[0];
+    got[1] += tot[1];
+}
+
 }  // namespace
 
-extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, uint32_t n_global,
-                                      uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,
-                                      float* d_global9, float* d_caustic9, pm_trace_stats* stats) {
-    if (!ctx || !scene) return PM_ERR_INVALID;
+namespace {
+
+// Checks of a trace request and the host tables it needs: each model's
+// triangle range and ior, and compute_critical_angles (PhotonTracing.cu:71-125):
+// cos(asin(n2/n1)) for every ordered pair with n2/n1 < 1, against each other
+// and the default.
+pm_status trace_tables(pm_ctx* ctx, const pm_scene* scene, uint32_t n_global, int32_t max_bounces, const char* who,
+                       std::vector<DevModel>& hm, std::vector<CaEntry>& ca) {
     if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0)
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: empty scene");
+        return pm::fail(ctx, PM_ERR_INVALID, "%s: empty scene", who);
     if (scene->light_model < 0 || scene->light_model >= scene->n_models)
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: bad light model");
+        return pm::fail(ctx, PM_ERR_INVALID, "%s: bad light model", who);
     if (max_bounces < 1 || max_bounces > kMaxBounces)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_trace_photons: max_bounces outside [1, %d]", kMaxBounces);
-    if (scene->n_models > 65536)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_trace_photons: more than 65536 models");
-    if (n_global == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: n_global must be >= 1");
-    if ((n_global && !d_global9) || (n_caustic && !d_caustic9))
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: null output map");
+        return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: max_bounces outside [1, %d]", who, kMaxBounces);
+    if (scene->n_models > 65536) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "%s: more than 65536 models", who);
+    if (n_global == 0) return pm::fail(ctx, PM_ERR_INVALID, "%s: n_global must be >= 1", who);
     const int nm = scene->n_models;
-    std::vector<DevModel> hm(nm);
+    hm.assign(nm, DevModel{});
     for (int i = 0; i < nm; ++i) {
         const pm_model& s = scene->h_models[i];
         if (s.tri_end > scene->n_tris || s.tri_begin > s.tri_end)
-            return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: model %d triangle range", i);
+            return pm::fail(ctx, PM_ERR_INVALID, "%s: model %d triangle range", who, i);
         hm[i].t0 = s.tri_begin;
         hm[i].t1 = s.tri_end;
         hm[i].ior = s.refr_index;
     }
     if (hm[scene->light_model].t1 == hm[scene->light_model].t0)
-        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: light model has no triangles");
-    // compute_critical_angles (PhotonTracing.cu:71-125): cos(asin(n2/n1)) for
-    // every ordered pair with n2/n1 < 1, against each other and the default.
-    std::vector<CaEntry> ca;
+        return pm::fail(ctx, PM_ERR_INVALID, "%s: light model has no triangles", who);
+    ca.clear();
     auto add_pair = [&](float e1, float e2) {
         float eta = e2 / e1;
         if (!(eta < 1.f)) return;
@@ -501,6 +547,46 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         add_pair(scene->default_refr_index, hm[i].ior);
         add_pair(hm[i].ior, scene->default_refr_index);
     }
+    return PM_OK;
+}
+
+// The scene part of a trace's kernel parameters.
+void trace_params_base(const pm_scene* scene, const DevModel* d_models, const CaEntry* d_ca, int n_ca,
+                       uint32_t n_global, int B, TraceParams& P) {
+    const int nm = scene->n_models;
+    P = TraceParams{};
+    P.tris = scene->d_tris9;
+    P.nrms = scene->d_normals9;
+    P.models = d_models;
+    P.n_models = nm;
+    P.bvh = bvh_nodes(scene);
+    P.spec_mask = 0;
+    P.spec_mask_valid = nm <= 64 ? 1 : 0;
+    for (int i = 0; i < nm && i < 64; ++i) {
+        const pm_model& m = scene->h_models[i];
+        if (m.opaque != 1.f || m.ks[0] != 0.f || m.ks[1] != 0.f || m.ks[2] != 0.f) P.spec_mask |= 1ull << i;
+    }
+    P.light = scene->light_model;
+    for (int c = 0; c < 3; ++c) P.pp[c] = scene->light_intensity[c] / (float)n_global;
+    P.def_ior = scene->default_refr_index;
+    P.ca = d_ca;
+    P.n_ca = n_ca;
+    P.max_b = B;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64_t seed, uint32_t n_global,
+                                      uint32_t n_caustic, int32_t max_bounces, uint64_t max_paths,
+                                      float* d_global9, float* d_caustic9, pm_trace_stats* stats) {
+    if (!ctx || !scene) return PM_ERR_INVALID;
+    std::vector<DevModel> hm;
+    std::vector<CaEntry> ca;
+    pm_status st0 = trace_tables(ctx, scene, n_global, max_bounces, "pm_trace_photons", hm, ca);
+    if (st0 != PM_OK) return st0;
+    if ((n_global && !d_global9) || (n_caustic && !d_caustic9))
+        return pm::fail(ctx, PM_ERR_INVALID, "pm_trace_photons: null output map");
+    const int nm = scene->n_models;
 
     // Global phase: batches of up to 4M paths, each staging up to
     // max_bounces photons. Caustic phase (global map full): a path stores at
@@ -545,25 +631,9 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     if (!ca.empty())
         PM_HIP(ctx, hipMemcpyAsync(d_ca, ca.data(), ca.size() * sizeof(CaEntry), hipMemcpyHostToDevice, ctx->stream));
 
-    TraceParams P{};
-    P.tris = scene->d_tris9;
-    P.nrms = scene->d_normals9;
-    P.models = d_models;
-    P.n_models = nm;
-    P.bvh = bvh_nodes(scene);
-    P.spec_mask = 0;
-    P.spec_mask_valid = nm <= 64 ? 1 : 0;
-    for (int i = 0; i < nm && i < 64; ++i) {
-        const pm_model& m = scene->h_models[i];
-        if (m.opaque != 1.f || m.ks[0] != 0.f || m.ks[1] != 0.f || m.ks[2] != 0.f) P.spec_mask |= 1ull << i;
-    }
-    P.light = scene->light_model;
-    for (int c = 0; c < 3; ++c) P.pp[c] = scene->light_intensity[c] / (float)n_global;
-    P.def_ior = scene->default_refr_index;
-    P.ca = d_ca;
-    P.n_ca = (int)ca.size();
+    TraceParams P;
+    trace_params_base(scene, d_models, d_ca, (int)ca.size(), n_global, B, P);
     P.seed = seed;
-    P.max_b = B;
     P.stage = stage;
     P.kind = kind;
     P.cnt_g = cnt_g;
@@ -607,7 +677,7 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
         if ((st = pm::scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
         hipLaunchKernelGGL(k_trace_compact, dim3(pm::div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind,
                            P.caustic_only ? nullptr : cnt_g, cnt_c, off_g, off_c, nb, P.stage_depth, got_g, got_c,
-                           d_global9, n_global, d_caustic9, n_caustic);
+                           d_global9, n_global, d_caustic9, n_caustic, nullptr);
         PM_CHECK_LAUNCH(ctx, "k_trace_compact");
         uint64_t h[2];
         PM_HIP(ctx, hipMemcpyAsync(h, tot, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -632,3 +702,147 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     }
     return PM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Planned traces: the same photon maps as pm_trace_photons, from a fixed
+// schedule of batches that is enqueued without reading anything back. Every
+// batch launches both phases' kernels; each reads the maps' fill counts on
+// the device and works only in its phase (a batch after both maps are full
+// does nothing). The maps are the first n photons of each kind in (path,
+// event) order whatever the batch sizes, so they equal pm_trace_photons'
+// maps whenever the schedule's paths suffice to fill them.
+namespace pm {
+
+struct TracePlan {
+    void* mem = nullptr;  // model table, critical-angle table, AABB scratch, fill counts
+    TraceParams P{};
+    uint64_t* got = nullptr;
+    uint64_t paths = 0;
+    uint32_t batch = 0, n_batches = 0, pool_blocks = 0;
+    uint32_t n_global = 0, n_caustic = 0;
+    int B = 0;
+};
+
+size_t trace_plan_scratch_bytes(const TracePlan* T) {
+    const size_t slots = (size_t)T->batch * T->B;
+    const uint32_t nseg = div_up(T->batch, kEmitBlock);
+    return carve_size({slots * 36, slots, (size_t)T->batch * 4 * 4, 64, (size_t)nseg * kEmitBlock * sizeof(PathState),
+                       (size_t)nseg * 4, (size_t)nseg * 4});
+}
+
+pm_status trace_plan_create(pm_ctx* ctx, const pm_scene* scene, uint32_t n_global, uint32_t n_caustic,
+                            int32_t max_bounces, uint64_t paths, TracePlan** out) {
+    *out = nullptr;
+    std::vector<DevModel> hm;
+    std::vector<CaEntry> ca;
+    pm_status st = trace_tables(ctx, scene, n_global, max_bounces, "trace plan", hm, ca);
+    if (st != PM_OK) return st;
+    if (paths == 0) return fail(ctx, PM_ERR_INVALID, "trace plan: no paths");
+    const int nm = scene->n_models;
+    TracePlan* T = new TracePlan();
+    T->n_global = n_global;
+    T->n_caustic = n_caustic;
+    T->B = max_bounces;
+    T->paths = paths;
+    T->batch = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>((uint64_t)n_global + n_caustic, 65536),
+                                                               4u << 20),
+                                           paths);
+    T->n_batches = (uint32_t)((paths + T->batch - 1) / T->batch);
+    const size_t bytes = carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16, (size_t)nm * 32, 16});
+    hipError_t e = hipMalloc(&T->mem, bytes);
+    if (e != hipSuccess) {
+        delete T;
+        return hip_fail(ctx, e, "hipMalloc(trace plan)");
+    }
+    Carve cv{static_cast<char*>(T->mem)};
+    DevModel* d_models = cv.take<DevModel>(nm);
+    CaEntry* d_ca = cv.take<CaEntry>(ca.size() + 1);
+    void* d_tmp = cv.take<char>(32 * (size_t)nm);
+    T->got = cv.take<uint64_t>(2);
+    if ((st = upload_models(ctx, scene, d_models, d_tmp, hm)) != PM_OK ||
+        (!ca.empty() && (st = hipMemcpyAsync(d_ca, ca.data(), ca.size() * sizeof(CaEntry), hipMemcpyHostToDevice,
+                                             ctx->stream) == hipSuccess
+                                 ? PM_OK
+                                 : fail(ctx, PM_ERR_HIP, "trace plan: critical angles")) != PM_OK)) {
+        trace_plan_destroy(T);
+        return st;
+    }
+    // (the tables are read by every later frame: the host copies complete here)
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        trace_plan_destroy(T);
+        return fail(ctx, PM_ERR_HIP, "trace plan: upload");
+    }
+    trace_params_base(scene, d_models, d_ca, (int)ca.size(), n_global, max_bounces, T->P);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_pool<false>, 256, 0) != hipSuccess) per_cu = 1;
+    T->pool_blocks = (uint32_t)std::max(1, per_cu) * (uint32_t)ctx->cu_count;
+    *out = T;
+    return PM_OK;
+}
+
+void trace_plan_destroy(TracePlan* T) {
+    if (!T) return;
+    if (T->mem) (void)hipFree(T->mem);
+    delete T;
+}
+
+const uint64_t* trace_plan_counts(const TracePlan* T) { return T->got; }
+
+pm_status trace_plan_enqueue(pm_ctx* ctx, TracePlan* T, uint64_t seed, float* d_global9, float* d_caustic9) {
+    void* work = nullptr;
+    pm_status st = scratch_reserve(ctx, trace_plan_scratch_bytes(T), &work);
+    if (st != PM_OK) return st;
+    const size_t slots = (size_t)T->batch * T->B;
+    const uint32_t nseg_max = div_up(T->batch, kEmitBlock);
+    Carve cv{static_cast<char*>(work)};
+    float* stage = cv.take<float>(slots * 9);
+    uint8_t* kind = cv.take<uint8_t>(slots);
+    uint32_t* cnt_g = cv.take<uint32_t>(T->batch);
+    uint32_t* cnt_c = cv.take<uint32_t>(T->batch);
+    uint32_t* off_g = cv.take<uint32_t>(T->batch);
+    uint32_t* off_c = cv.take<uint32_t>(T->batch);
+    uint64_t* tot = cv.take<uint64_t>(3);
+    PathState* queue = cv.take<PathState>((size_t)nseg_max * kEmitBlock);
+    uint32_t* seg_len = cv.take<uint32_t>(nseg_max);
+    uint32_t* seg_off = cv.take<uint32_t>(nseg_max);
+    if ((st = fill_u32(ctx, reinterpret_cast<uint32_t*>(T->got), 0u, 4)) != PM_OK) return st;
+    TraceParams P = T->P;
+    P.seed = seed;
+    P.stage = stage;
+    P.kind = kind;
+    P.cnt_g = cnt_g;
+    P.cnt_c = cnt_c;
+    P.stage_depth = T->B;
+    P.caustic_only = 0;
+    P.got = T->got;
+    P.need_g = T->n_global;
+    P.need_c = T->n_caustic;
+    for (uint32_t b = 0; b < T->n_batches; ++b) {
+        const uint64_t base = (uint64_t)b * T->batch;
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(T->batch, T->paths - base);
+        P.path_base = base;
+        P.n_paths = nb;
+        P.n_segs = div_up(nb, kEmitBlock);
+        hipLaunchKernelGGL(k_trace_pool<false>, dim3(std::min(div_up(nb, 256), T->pool_blocks)), dim3(256), 0,
+                           ctx->stream, P, nullptr, nullptr, nullptr);
+        PM_CHECK_LAUNCH(ctx, "k_trace_pool (plan)");
+        if (P.spec_mask_valid) {
+            hipLaunchKernelGGL(k_trace_emit, dim3(P.n_segs), dim3(kEmitBlock), 0, ctx->stream, P, queue, seg_len);
+            PM_CHECK_LAUNCH(ctx, "k_trace_emit (plan)");
+            if ((st = scan_u32(ctx, seg_len, seg_off, P.n_segs, false, tot + 2)) != PM_OK) return st;
+            hipLaunchKernelGGL(k_trace_pool<true>, dim3(std::min(div_up(nb, 256), T->pool_blocks)), dim3(256), 0,
+                               ctx->stream, P, queue, seg_off, tot + 2);
+            PM_CHECK_LAUNCH(ctx, "k_trace_pool<queued> (plan)");
+        }
+        if ((st = scan_u32(ctx, cnt_g, off_g, nb, false, tot)) != PM_OK) return st;
+        if ((st = scan_u32(ctx, cnt_c, off_c, nb, false, tot + 1)) != PM_OK) return st;
+        hipLaunchKernelGGL(k_trace_compact, dim3(div_up(nb, 256)), dim3(256), 0, ctx->stream, stage, kind, cnt_g, cnt_c,
+                           off_g, off_c, nb, T->B, 0ull, 0ull, d_global9, T->n_global, d_caustic9, T->n_caustic, T->got);
+        PM_CHECK_LAUNCH(ctx, "k_trace_compact (plan)");
+        hipLaunchKernelGGL(k_trace_advance, dim3(1), dim3(64), 0, ctx->stream, T->got, tot, T->n_global);
+        PM_CHECK_LAUNCH(ctx, "k_trace_advance");
+    }
+    return PM_OK;
+}
+
+}  // namespace pm

@@ -108,6 +108,11 @@ _SIGS = {
     "pm_model_aabb": (_I32, [_P, _P, _P, _I32, _P]),
     "pm_render_frame": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pm_trace_photons": (_I32, [_P, _P, C.c_uint64, _U32, _U32, _I32, C.c_uint64, _P, _P, _P]),
+    "pm_pipeline_create": (_I32, [_P, _P, C.POINTER(_P)]),
+    "pm_pipeline_frame": (_I32, [_P, C.c_uint64, _P, _P]),
+    "pm_pipeline_status": (_I32, [_P, _P]),
+    "pm_pipeline_grids": (_I32, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "pm_pipeline_destroy": (_I32, [_P]),
     "pm_photon_file_count": (_I32, [_P, C.c_char_p, C.POINTER(C.c_uint64)]),
     "pm_photon_file_read": (_I32, [_P, C.c_char_p, _P, _U32, C.POINTER(_U32)]),
     "pm_photon_file_write": (_I32, [_P, C.c_char_p, _P, _U32]),
@@ -143,6 +148,18 @@ class JobDesc(C.Structure):
                 ("n_caustic", C.c_uint32), ("max_bounces", C.c_int32), ("max_paths", C.c_uint64),
                 ("seed", C.c_uint64), ("camera", CameraRec), ("params", C.c_int32 * 7), ("frames", C.c_uint32),
                 ("d_rgb8", C.c_void_p)]
+
+
+class PipelineDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("aabb", C.c_float * 6), ("cell_global", C.c_float),
+                ("cell_caustic", C.c_float), ("n_global", C.c_uint32), ("n_caustic", C.c_uint32),
+                ("max_bounces", C.c_int32), ("max_paths", C.c_uint64), ("plan_seed", C.c_uint64),
+                ("camera", CameraRec), ("params", C.c_int32 * 7)]
+
+
+class PipelineInfo(C.Structure):
+    _fields_ = [("scheduled_paths", C.c_uint64), ("n_global", C.c_uint32), ("n_caustic", C.c_uint32),
+                ("complete", C.c_uint32), ("hits", C.c_uint64)]
 
 
 class ServerInfo(C.Structure):
@@ -447,6 +464,27 @@ class Context:
                                             cam.ctypes.data, C.byref(p), _ptr(rgb), _ptr(rgb8), C.byref(st)))
         return rgb, rgb8, int(st.hits)
 
+    def pipeline(self, dscene: "DeviceScene", aabb, cell_global: float, cell_caustic: float, n_global: int,
+                 n_caustic: int, camera, max_bounces: int = 10, max_paths: int = 1 << 40, plan_seed: int = 1,
+                 max_depth: int = 4, k_global: int = 50, r_global: float = 0.05, k_caustic: int = 50,
+                 r_caustic: float = 0.05, gather_flags: int = 0) -> "Pipeline":
+        """pm_pipeline_create: a per-frame pipeline (trace, grid rebuilds,
+        render) that pm_pipeline_frame enqueues without host synchronisation."""
+        d = PipelineDesc()
+        d.scene = C.addressof(dscene.rec)
+        d.aabb = (C.c_float * 6)(*[float(v) for v in aabb])
+        d.cell_global, d.cell_caustic = float(cell_global), float(cell_caustic)
+        d.n_global, d.n_caustic = int(n_global), int(n_caustic)
+        d.max_bounces, d.max_paths, d.plan_seed = int(max_bounces), int(max_paths), int(plan_seed)
+        C.memmove(C.byref(d.camera), np.ascontiguousarray(camera).ctypes.data, C.sizeof(CameraRec))
+        p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic), 0,
+                         int(gather_flags))
+        C.memmove(C.byref(d.params), C.byref(p), C.sizeof(RenderParams))
+        h = _P()
+        self.check(self.lib.pm_pipeline_create(self.h, C.byref(d), C.byref(h)))
+        cam = np.ascontiguousarray(camera)
+        return Pipeline(self, h, dscene, int(cam["width"].item()) * int(cam["height"].item()))
+
     def render_frames_batched(self, frames, flags: int = 0, want_rgb8: bool = True):
         """Render many frames with their gathers in one pm_gather_batch:
         frames = [(dscene, grid_g, grid_c or None, camera, max_depth, k_global,
@@ -549,6 +587,33 @@ class Grid:
     def close(self):
         if self.h:
             self.ctx.lib.pm_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pipeline:
+    def __init__(self, ctx: Context, h, dscene, npix: int):
+        self.ctx, self.h, self.dscene, self.npix = ctx, h, dscene, npix
+
+    def frame(self, seed: int, rgb8=None, rgb3=None):
+        """Enqueue one frame (pm_pipeline_frame) into the given device
+        buffers (uint8 [H*W*3], float32 [3, H*W]; either may be None)."""
+        self.ctx.check(self.ctx.lib.pm_pipeline_frame(self.h, int(seed), _ptr(rgb3), _ptr(rgb8)))
+
+    def status(self) -> dict:
+        info = PipelineInfo()
+        self.ctx.check(self.ctx.lib.pm_pipeline_status(self.h, C.byref(info)))
+        return {"scheduled_paths": int(info.scheduled_paths), "n_global": int(info.n_global),
+                "n_caustic": int(info.n_caustic), "complete": bool(info.complete), "hits": int(info.hits)}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.pm_pipeline_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -365,6 +365,49 @@ pm_status pm_frame_gather_jobs(pm_frame* frame, const pm_grid* global_grid, cons
                                pm_gather_job* jobs /* [2] */, uint32_t* n_jobs);
 pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
 
+/* ---- per-frame pipeline for graph capture: the whole frame of the reference
+ * (trace_photons, PhotonGrid::build of both maps, Raytracer::render_gpu:
+ * Initializer.cu:142-177, Drawer.cu:18-57) as one sequence of launches on the
+ * context's stream with every count kept on the device, so it can be
+ * captured into a HIP graph and replayed each frame. pm_pipeline_create
+ * renders one planning frame eagerly (plan_seed) to size the photon maps,
+ * the grids (built at the given cell sizes), the trace's path schedule (the
+ * planning frame's paths plus a quarter, at most max_paths) and the query
+ * block (every pixel's full ray tree), and grows the context's scratch; it
+ * fails with PM_ERR_CAPACITY if max_paths cannot fill the maps.
+ * pm_pipeline_frame enqueues one frame for `seed` (uchar3 and/or HDR floats
+ * into caller buffers, either nullable): no allocation, no host
+ * synchronisation. Its maps, grids and image equal the eager calls' for the
+ * same seed whenever the scheduled paths fill both maps, which
+ * pm_pipeline_status reports (it synchronises the stream). One pipeline per
+ * context at a time: frames share the context's scratch. The
+ * reference-caustic mode is not supported. */
+typedef struct {
+    const pm_scene* scene;
+    float aabb[6];                     /* grid bounds */
+    float cell_global, cell_caustic;   /* grid cell sizes (> 0) */
+    uint32_t n_global, n_caustic;
+    int32_t max_bounces;
+    uint64_t max_paths;
+    uint64_t plan_seed;
+    pm_camera camera;
+    pm_render_params params;
+} pm_pipeline_desc;
+
+typedef struct {
+    uint64_t scheduled_paths;   /* paths each frame traces */
+    uint32_t n_global, n_caustic;  /* photons the last frame stored (capped at the map sizes) */
+    uint32_t complete;          /* 1 when both maps filled */
+    uint64_t hits;              /* the last frame's eye-ray hits (queries per map) */
+} pm_pipeline_info;
+
+typedef struct pm_pipeline pm_pipeline;
+pm_status pm_pipeline_create(pm_ctx* ctx, const pm_pipeline_desc* desc, pm_pipeline** out);
+pm_status pm_pipeline_frame(pm_pipeline* pipeline, uint64_t seed, float* d_rgb3, uint8_t* d_rgb8);
+pm_status pm_pipeline_status(pm_pipeline* pipeline, pm_pipeline_info* out);
+pm_status pm_pipeline_grids(const pm_pipeline* pipeline, const pm_grid** global_grid, const pm_grid** caustic_grid);
+pm_status pm_pipeline_destroy(pm_pipeline* pipeline);
+
 /* ---- serving: continuous batching of render jobs (the reference renders one
  * scene, one frame at a time, Drawer.cu:18-101). A job is a resident scene
  * (with its pm_accel, if any), photon budgets, grid cell size, camera,
