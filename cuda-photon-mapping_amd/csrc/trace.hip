@@ -605,7 +605,8 @@ extern "C" pm_status pm_trace_photons(pm_ctx* ctx, const pm_scene* scene, uint64
     const size_t slots = std::max((size_t)batch_g * B, (size_t)batch_c);
     const uint32_t nseg = pm::div_up(batch_c, kEmitBlock);
     size_t rs_bytes = pm::carve_size({(size_t)nm * sizeof(DevModel), ca.size() * sizeof(CaEntry) + 16,
-                                      (size_t)nm * 32, slots * 36, slots, (size_t)npath * 4 * 4, 64,
+                                      (size_t)nm * 32, slots * 36, slots, (size_t)npath * 4, (size_t)npath * 4,
+                                      (size_t)npath * 4, (size_t)npath * 4, 3 * sizeof(uint64_t),
                                       (size_t)nseg * kEmitBlock * sizeof(PathState), (size_t)nseg * 4,
                                       (size_t)nseg * 4});
     void* work = nullptr;
@@ -726,8 +727,9 @@ struct TracePlan {
 size_t trace_plan_scratch_bytes(const TracePlan* T) {
     const size_t slots = (size_t)T->batch * T->B;
     const uint32_t nseg = div_up(T->batch, kEmitBlock);
-    return carve_size({slots * 36, slots, (size_t)T->batch * 4 * 4, 64, (size_t)nseg * kEmitBlock * sizeof(PathState),
-                       (size_t)nseg * 4, (size_t)nseg * 4});
+    const size_t per_path = (size_t)T->batch * 4;
+    return carve_size({slots * 36, slots, per_path, per_path, per_path, per_path, 3 * sizeof(uint64_t),
+                       (size_t)nseg * kEmitBlock * sizeof(PathState), (size_t)nseg * 4, (size_t)nseg * 4});
 }
 
 pm_status trace_plan_create(pm_ctx* ctx, const pm_scene* scene, uint32_t n_global, uint32_t n_caustic,

@@ -381,7 +381,10 @@ pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* 
  * same seed whenever the scheduled paths fill both maps, which
  * pm_pipeline_status reports (it synchronises the stream). One pipeline per
  * context at a time: frames share the context's scratch. The
- * reference-caustic mode is not supported. */
+ * reference-caustic mode is not supported.
+ * Currently switched off: pm_pipeline_create returns PM_ERR_UNSUPPORTED
+ * (a creation has been seen not to finish on the GPU; the cause is not yet
+ * known). */
 typedef struct {
     const pm_scene* scene;
     float aabb[6];                     /* grid bounds */
