@@ -5,13 +5,15 @@
 // tests; this is a standard kd search with the same answer as the grid
 // gather: the k smallest (d2, photon index) among photons within r.
 //
-// One wave64 per query. A breadth-first descent from the root gathers up to
-// 64 subtree roots whose half-spaces meet the query ball; then every lane
-// walks one subtree depth-first with a private stack, visiting one node per
-// step. Each step's visited photons feed the wave's LDS key buffer exactly as
-// in the grid gather (pm_select.hpp). A subtree is skipped only when the
-// squared distance from the query to its splitting plane exceeds the radius
-// threshold by a relative 1e-5, so no photon within r is ever missed.
+// One wave64 per query, walking the tree with one LIFO stack of node ids in
+// LDS shared by the wave: each step the 64 lanes pop the top 64 entries,
+// visit one node each and push the children whose half-spaces meet the
+// query ball, so the lanes stay busy however unevenly the ball splits the
+// tree. The visited photons feed the wave's LDS key buffer exactly as in the
+// grid gather (pm_select.hpp); the k smallest keys do not depend on the
+// visiting order. A subtree is skipped only when the squared distance from
+// the query to its splitting plane exceeds the radius threshold by a
+// relative 1e-5, so no photon within r is ever missed.
 #include <cmath>
 
 #include <cstring>
@@ -23,15 +25,12 @@ namespace {
 using namespace pms;
 
 constexpr int kWaves = 4;
-constexpr int kCap = 512;
-constexpr int kStack = 40;
+constexpr uint32_t kMaxLevels = 40;
 
 struct KdParams {
-    const float* px;
-    const float* py;
-    const float* pz;
-    const uint32_t* node_idx;
-    const int8_t* node_plane;
+    // per node {x, y, z, photon index | split plane << 30}; w = 0xFFFFFFFF
+    // for an empty slot
+    const float4* nodes;
     uint32_t cap;
     const float* qx;
     const float* qy;
@@ -46,12 +45,20 @@ struct KdParams {
     uint32_t* out_count;
 };
 
-constexpr int kLdsStack = 12;  // top of each lane's depth-first stack, lane-interleaved in LDS
+// The wave's node stack. A step pops at most 64 entries and pushes at most
+// two children per popped node, so it grows the stack by at most 64; when
+// fewer than 128 slots are free the wave pops one node per step instead (a
+// depth-first walk from any state grows the stack by at most the tree's
+// depth, at most kMaxLevels = 40 < 64 by the host's check), so the stack
+// never overflows.
+constexpr uint32_t kStackCap = 1024;
 
+// CAP: the key buffer (256 keys for k <= 64, which narrow to at most 128;
+// 512 otherwise): the smaller buffer leaves LDS for more waves per CU.
+template <int CAP>
 struct KdLds {
-    WaveLds<kCap> sel;
-    uint32_t frontier[2][64];
-    uint32_t stk[kLdsStack][64];
+    WaveLds<CAP> sel;
+    uint32_t stk[kStackCap];
 };
 
 __device__ __forceinline__ float kd_dist2(float px, float py, float pz, float qx, float qy, float qz) {
@@ -66,12 +73,14 @@ __device__ __forceinline__ void visit(const KdParams& P, uint32_t node, float q0
     valid = false;
     go_l = go_r = false;
     if (node >= P.cap) return;
-    pid = P.node_idx[node];
-    if (pid == 0xFFFFFFFFu) return;
-    const float x = P.px[node], y = P.py[node], z = P.pz[node];
+    const float4 rec = P.nodes[node];
+    const uint32_t w = __float_as_uint(rec.w);
+    if (w == 0xFFFFFFFFu) return;
+    pid = w & 0x3FFFFFFFu;
+    const float x = rec.x, y = rec.y, z = rec.z;
     d2 = kd_dist2(x, y, z, q0, q1, q2);
     valid = true;
-    const int p = P.node_plane[node];
+    const int p = (int)(w >> 30);
     const float diff = (p == 0 ? q0 : p == 1 ? q1 : q2) - (p == 0 ? x : p == 1 ? y : z);
     const float dd = diff * diff;
     // left subtree: coordinate <= median; right: >= median
@@ -81,12 +90,13 @@ __device__ __forceinline__ void visit(const KdParams& P, uint32_t node, float q0
     go_r = go_r && 2 * node + 2 < P.cap;
 }
 
+template <int kCap>
 __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
-    __shared__ KdLds lds[kWaves];
+    __shared__ KdLds<kCap> lds[kWaves];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t q = blockIdx.x * kWaves + wave;
     if (q >= P.nq) return;  // wave-uniform
-    KdLds& L = lds[wave];
+    KdLds<kCap>& L = lds[wave];
     const float q0 = P.qx[q], q1 = P.qy[q], q2 = P.qz[q];
     const float T = P.T;
     const uint32_t k = (uint32_t)P.k;
@@ -111,56 +121,27 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
         cnt += na;
     };
 
-    // Breadth-first until the frontier would exceed 64 subtree roots.
-    uint32_t fsize = 1;
-    int cur = 0;
-    if (lane == 0) L.frontier[0][0] = 0;
+    uint32_t top = 1;  // (wave-uniform)
+    if (lane == 0) L.stk[0] = 0;
     wave_sync();
-    while (fsize > 0 && fsize <= 32) {
-        uint32_t node = lane < fsize ? L.frontier[cur][lane] : P.cap;
-        bool valid, gl, gr;
-        uint32_t pid = 0;
-        float d2 = 0.f;
-        visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
-        feed(valid, pid, d2);
-        uint64_t bl = wballot(gl), br = wballot(gr);
-        uint32_t nl = (uint32_t)__popcll(bl);
-        wave_sync();
-        if (gl) L.frontier[cur ^ 1][mbcnt(bl)] = 2 * node + 1;
-        if (gr) L.frontier[cur ^ 1][nl + mbcnt(br)] = 2 * node + 2;
-        fsize = nl + (uint32_t)__popcll(br);
-        cur ^= 1;
-        wave_sync();
-    }
-    // Depth-first: lane l walks the subtree rooted at frontier[l]. The top
-    // kLdsStack entries of its stack live in LDS (one column per lane, no
-    // bank conflicts), deeper ones in a per-lane array.
-    uint32_t deep[kStack - kLdsStack];
-    int sp = 0;
-    auto push = [&](uint32_t v) {
-        if (sp < kLdsStack)
-            L.stk[sp][lane] = v;
-        else
-            deep[sp - kLdsStack] = v;
-        ++sp;
-    };
-    if (lane < fsize) push(L.frontier[cur][lane]);
-    for (;;) {
-        if (!__builtin_amdgcn_ballot_w64(sp > 0)) break;
+    while (top > 0) {
+        const uint32_t n = kStackCap - top >= 128 ? (top < 64 ? top : 64u) : 1u;
         uint32_t node = P.cap;
-        if (sp > 0) {
-            --sp;
-            node = sp < kLdsStack ? L.stk[sp][lane] : deep[sp - kLdsStack];
-        }
+        if (lane < n) node = L.stk[top - n + lane];
+        wave_sync();  // every pop is read before the pushes reuse the slots
+        top -= n;
         bool valid, gl, gr;
         uint32_t pid = 0;
         float d2 = 0.f;
         visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
         feed(valid, pid, d2);
-        // right child below the left one, so the left subtree is walked
-        // first; the order does not change the result (pruning uses r only)
-        if (gr && sp < kStack) push(2 * node + 2);
-        if (gl && sp < kStack) push(2 * node + 1);
+        // right children below the left ones: left subtrees are popped first
+        const uint64_t br = wballot(gr), bl = wballot(gl);
+        const uint32_t nr = (uint32_t)__popcll(br);
+        if (gr) L.stk[top + mbcnt(br)] = 2 * node + 2;
+        if (gl) L.stk[top + nr + mbcnt(bl)] = 2 * node + 1;
+        top = __builtin_amdgcn_readfirstlane(top + nr + (uint32_t)__popcll(bl));
+        wave_sync();
     }
     wave_sync();
 
@@ -193,15 +174,11 @@ extern "C" pm_status pm_kdtree_knn(pm_ctx* ctx, const pm_kdtree* tree, const flo
     if (!tree) return pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_knn: null tree");
     if (!(radius >= 0.f) || !std::isfinite(radius)) return pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_knn: bad radius");
     if (k < 1 || k > 256) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_kdtree_knn: k=%d outside [1, 256]", k);
-    if (tree->levels > kStack) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_kdtree_knn: tree too deep");
+    if (tree->levels > kMaxLevels) return pm::fail(ctx, PM_ERR_UNSUPPORTED, "pm_kdtree_knn: tree too deep");
     if (nq == 0) return PM_OK;
     if (!d_queries3 || !d_idx || !d_dist || !d_count) return pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_knn: null buffer");
     KdParams P{};
-    P.px = tree->photons;
-    P.py = tree->photons + tree->cap;
-    P.pz = tree->photons + 2 * (size_t)tree->cap;
-    P.node_idx = tree->node_idx;
-    P.node_plane = tree->node_plane;
+    P.nodes = tree->nodes;
     P.cap = tree->cap;
     P.qx = d_queries3;
     P.qy = d_queries3 + nq;
@@ -216,7 +193,10 @@ extern "C" pm_status pm_kdtree_knn(pm_ctx* ctx, const pm_kdtree* tree, const flo
     P.out_idx = d_idx;
     P.out_dist = d_dist;
     P.out_count = d_count;
-    hipLaunchKernelGGL(k_kd_knn, dim3(pm::div_up(nq, kWaves)), dim3(256), 0, ctx->stream, P);
+    if (k <= 64)
+        hipLaunchKernelGGL(k_kd_knn<256>, dim3(pm::div_up(nq, kWaves)), dim3(256), 0, ctx->stream, P);
+    else
+        hipLaunchKernelGGL(k_kd_knn<512>, dim3(pm::div_up(nq, kWaves)), dim3(256), 0, ctx->stream, P);
     PM_CHECK_LAUNCH(ctx, "k_kd_knn");
     return PM_OK;
 }

@@ -529,9 +529,13 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
     }
 }
 
+// The tree's photons in node order (nine planes), and each node's 16-byte
+// descent record {x, y, z, photon index | plane << 30} (w = 0xFFFFFFFF for
+// an empty slot): the kNN reads one record per visited node.
 __global__ __launch_bounds__(256) void k_tree_photons(const float4* __restrict__ prec,
-                                                     const uint32_t* __restrict__ node_idx, uint32_t cap,
-                                                     float* __restrict__ out9) {
+                                                     const uint32_t* __restrict__ node_idx,
+                                                     const int8_t* __restrict__ node_plane, uint32_t cap,
+                                                     float* __restrict__ out9, float4* __restrict__ nodes) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cap) return;
     uint32_t id = node_idx[i];
@@ -545,6 +549,8 @@ __global__ __launch_bounds__(256) void k_tree_photons(const float4* __restrict__
     }
 #pragma unroll
     for (int p = 0; p < 9; ++p) out9[(size_t)p * cap + i] = v[p];
+    const uint32_t w = id != kFinal ? id | ((uint32_t)node_plane[i] << 30) : kFinal;
+    nodes[i] = make_float4(v[0], v[1], v[2], __uint_as_float(w));
 }
 
 // ---- model AABB (the live effect of create_kd_tree, KdTree.cu:395-464) ----
@@ -601,7 +607,7 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     T->n = n;
     T->cap = cap;
     T->levels = L;
-    size_t bytes = pm::carve_size({(size_t)cap * 4, (size_t)cap, (size_t)cap * 36});
+    size_t bytes = pm::carve_size({(size_t)cap * 4, (size_t)cap, (size_t)cap * 36, (size_t)cap * 16});
     hipError_t e = hipMalloc(&T->mem, bytes);
     if (e != hipSuccess) {
         delete T;
@@ -611,6 +617,7 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     T->node_idx = tc.take<uint32_t>(cap);
     T->node_plane = tc.take<int8_t>(cap);
     T->photons = tc.take<float>((size_t)cap * 9);
+    T->nodes = tc.take<float4>(cap);
     auto bail = [&](pm_status s) {
         (void)hipFree(T->mem);
         delete T;
@@ -686,8 +693,8 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
                            plane, lcounter);
     }
-    hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx, cap,
-                       T->photons);
+    hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx,
+                       T->node_plane, cap, T->photons, T->nodes);
     if (hipGetLastError() != hipSuccess) return bail(pm::fail(ctx, PM_ERR_HIP, "pm_kdtree_build: launch failed"));
     *out = T;
     return PM_OK;

@@ -53,6 +53,9 @@ struct pm_kdtree {
     uint32_t* node_idx = nullptr;  // [cap]
     int8_t* node_plane = nullptr;  // [cap]
     float* photons = nullptr;      // photons9 in node order [9][cap] (empty slots hold +inf positions)
+    // one 16-byte record per node for the kNN descent: position and
+    // (photon index | split plane << 30); 0xFFFFFFFF in w marks an empty slot
+    float4* nodes = nullptr;       // [cap]
 };
 
 namespace pm {
