@@ -1332,6 +1332,7 @@ pm_status launch(pm_ctx* ctx, const pm_grid* grid, KnnParams& P, uint32_t flags)
 extern "C" pm_status pm_grid_knn(pm_ctx* ctx, const pm_grid* grid, const float* d_queries3, uint32_t nq, float radius,
                                  int32_t k, uint32_t flags, uint32_t* d_idx, float* d_dist, uint32_t* d_count) {
     if (!ctx) return PM_ERR_INVALID;
+    flags = pm::tuned_flags(ctx, grid, radius, k, flags);
     KnnParams P;
     pm_status st = setup(ctx, grid, d_queries3, nq, radius, k, flags, P, "pm_grid_knn");
     if (st != PM_OK) return st;
@@ -1346,6 +1347,7 @@ extern "C" pm_status pm_grid_gather_radiance(pm_ctx* ctx, const pm_grid* grid, c
                                              const float* d_normals3, uint32_t nq, float radius, int32_t k,
                                              uint32_t flags, float* d_rgb3, uint32_t* d_count) {
     if (!ctx) return PM_ERR_INVALID;
+    flags = pm::tuned_flags(ctx, grid, radius, k, flags);
     KnnParams P;
     pm_status st = setup(ctx, grid, d_queries3, nq, radius, k, flags, P, "pm_grid_gather_radiance");
     if (st != PM_OK) return st;
@@ -1445,8 +1447,10 @@ extern "C" pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* jobs, uin
     uint64_t total = 0, cells = 0;
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const pm_gather_job& J = jobs[j];
+        // this job's flags: the call's, its own, and its tuning
+        const uint32_t jf = pm::tuned_flags(ctx, J.grid, J.radius, J.k, flags | J.flags);
         KnnParams P;
-        pm_status st = setup(ctx, J.grid, J.d_queries3, J.nq, J.radius, J.k, flags, P, "pm_gather_batch");
+        pm_status st = setup(ctx, J.grid, J.d_queries3, J.nq, J.radius, J.k, jf, P, "pm_gather_batch");
         if (st != PM_OK) return st;
         if (J.nq > 0) {
             if ((J.d_idx == nullptr) != (J.d_dist == nullptr))

@@ -11,6 +11,15 @@
 
 #define PM_WAVE 64
 
+namespace pm {
+// One remembered gather tuning: k, photon density class, choice.
+struct TuneEntry {
+    int k;
+    int cls;
+    pm_gather_tuning t;
+};
+}  // namespace pm
+
 struct pm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -39,6 +48,8 @@ struct pm_ctx {
     // Frame state of pm_render_frame (hit slots and query block survive the
     // gathers, which use `scratch`).
     pm_frame* frame_state = nullptr;
+    // Gather tunings measured by pm_tune_gather on this context.
+    std::vector<pm::TuneEntry> tunes;
 };
 
 struct pm_grid {
@@ -118,6 +129,13 @@ __device__ __forceinline__ void lookback_advance(uint32_t* counter, uint32_t epo
     __hip_atomic_store(counter + kEpochWord, epoch + 1 < (1u << 30) ? epoch + 1 : 1u, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// PM_KNN_TUNED: the flags with the tuning bits of the context's remembered
+// choice for (k, the grid's photon density at this radius); without the
+// flag, or without a remembered choice, the flags as given (less the flag).
+uint32_t tuned_flags(const pm_ctx* ctx, const pm_grid* grid, float radius, int32_t k, uint32_t flags);
+// The remembered cell size for (k, density), else `fallback`.
+float tuned_cell(const pm_ctx* ctx, uint32_t n, const float aabb[6], float radius, int32_t k, float fallback);
 
 // Scan over u32 (exclusive or inclusive), optional u64 device total.
 pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total);

@@ -455,6 +455,7 @@ extern "C" pm_status pm_frame_gather_jobs(pm_frame* F, const pm_grid* global_gri
     g.k = F->R.k_global;
     g.d_normals3 = F->n3;
     g.d_rgb3 = F->lg;
+    g.flags = F->R.gather_flags;
     jobs[0] = g;
     *n_jobs = 1;
     F->caustic = caustic_grid != nullptr;

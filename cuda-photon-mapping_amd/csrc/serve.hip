@@ -1,12 +1,16 @@
-// Serving loop: continuous batching of render jobs on one context.
+// Serving loop: continuous batching of render jobs.
 //
 // The reference renders one scene, one frame at a time (Drawer.cu:18-101).
 // Here a server holds many jobs (a resident scene, photon budgets, a camera,
 // a number of frames). Each step produces the next frame of every active
 // job: its photon pass is traced (re-seeded per frame), both grids are
-// rebuilt, its eye rays are traced into its own frame state, and then the
-// gathers of ALL active frames run as one pm_gather_batch before each frame
-// is finished. Jobs waiting in the queue are admitted at the start of a step
+// rebuilt in place, its eye rays are traced into its own frame state, and
+// then the gathers of ALL active frames run as one pm_gather_batch before
+// each frame is finished. With worker contexts (pm_server_set_workers) the
+// jobs' trace / grid / eye-ray stages run concurrently, one host thread and
+// one HIP stream per worker, so the host round trips of one job's trace
+// overlap other jobs' kernels; the shared gather launch waits on every
+// worker's stream. Jobs waiting in the queue are admitted at the start of a step
 // as soon as their device footprint fits the memory budget, so arrivals join
 // work already in flight (continuous batching); a finished job releases its
 // memory at once. A job's frames are bit-identical alone and in any mix: every
@@ -16,6 +20,7 @@
 #include <deque>
 #include <memory>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "pm_internal.hpp"
@@ -58,6 +63,10 @@ struct pm_server {
     std::vector<uint64_t> finished;   // ids
     std::vector<double> latencies;    // seconds, one per completed frame
     uint64_t queries = 0;
+    // worker contexts for the per-job stages, each on a stream of its own
+    std::vector<pm_ctx*> workers;
+    std::vector<hipStream_t> wstreams;
+    std::vector<hipEvent_t> wevents;
 };
 
 namespace {
@@ -109,9 +118,54 @@ extern "C" pm_status pm_server_create(pm_ctx* ctx, size_t budget_bytes, pm_serve
     return PM_OK;
 }
 
+namespace {
+
+void drop_workers(pm_server* S) {
+    for (hipStream_t w : S->wstreams) (void)hipStreamSynchronize(w);
+    for (pm_ctx* c : S->workers) pm_ctx_destroy(c);
+    for (hipEvent_t e : S->wevents) (void)hipEventDestroy(e);
+    for (hipStream_t w : S->wstreams) (void)hipStreamDestroy(w);
+    S->workers.clear();
+    S->wevents.clear();
+    S->wstreams.clear();
+}
+
+}  // namespace
+
+extern "C" pm_status pm_server_set_workers(pm_server* S, int32_t n_workers) {
+    if (!S) return PM_ERR_INVALID;
+    pm_ctx* ctx = S->ctx;
+    if (n_workers < 0 || n_workers > 64) return pm::fail(ctx, PM_ERR_INVALID, "pm_server_set_workers: 0..64 workers");
+    PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_workers(S);
+    for (int32_t w = 0; w < n_workers; ++w) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        pm_ctx* wc = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            drop_workers(S);
+            return pm::fail(ctx, PM_ERR_HIP, "pm_server_set_workers: stream");
+        }
+        S->wstreams.push_back(st);
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            drop_workers(S);
+            return pm::fail(ctx, PM_ERR_HIP, "pm_server_set_workers: event");
+        }
+        S->wevents.push_back(ev);
+        pm_status rc = pm_ctx_create(ctx->device, st, &wc);
+        if (rc != PM_OK) {
+            drop_workers(S);
+            return pm::fail(ctx, rc, "pm_server_set_workers: worker context");
+        }
+        S->workers.push_back(wc);
+    }
+    return PM_OK;
+}
+
 extern "C" pm_status pm_server_destroy(pm_server* S) {
     if (!S) return PM_OK;
     (void)hipStreamSynchronize(S->ctx->stream);
+    drop_workers(S);
     for (auto& j : S->active) release(S, *j);
     delete S;
     return PM_OK;
@@ -136,6 +190,48 @@ extern "C" pm_status pm_server_submit(pm_server* S, const pm_job_desc* desc, uin
     return PM_OK;
 }
 
+namespace {
+
+// Grid of this frame's n photons: rebuilt in place when the job's grid holds
+// them, else built afresh (cell: the job's, or its tuned size).
+pm_status frame_grid(pm_ctx* ctx, pm_grid*& g, const float* ph9, uint32_t n, const pm_job_desc& d, float r, int32_t k) {
+    const float cell =
+        (d.params.gather_flags & PM_KNN_TUNED) ? pm::tuned_cell(ctx, n, d.aabb, r, k, d.cell_size) : d.cell_size;
+    if (g && g->desc.cell_size == cell && n <= g->n_cap) return pm_grid_rebuild(ctx, g, ph9, n);
+    if (g) {
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        pm_grid_destroy(g);
+        g = nullptr;
+    }
+    return pm_grid_build(ctx, ph9, n, d.aabb, cell, 1000, PM_LAYOUT_LINEAR, &g);
+}
+
+// One job's stages before the gathers, on context `ctx`: its next frame's
+// photon pass, both grids, its eye rays; its gather jobs into out[0..n).
+pm_status prepare(pm_ctx* ctx, Job& j, pm_gather_job* out, uint32_t& n, uint32_t& nq) {
+    const pm_job_desc& d = j.d;
+    const uint64_t seed = d.seed + 0x9E3779B97F4A7C15ull * (j.done + 1);
+    pm_trace_stats ts{};
+    pm_status st;
+    if ((st = pm_trace_photons(ctx, d.scene, seed, d.n_global, d.n_caustic, d.max_bounces, d.max_paths, j.g9,
+                               d.n_caustic ? j.c9 : nullptr, &ts)) != PM_OK)
+        return st;
+    if ((st = frame_grid(ctx, j.gg, j.g9, ts.n_global, d, d.params.r_global, d.params.k_global)) != PM_OK) return st;
+    if (ts.n_caustic > 0) {
+        if ((st = frame_grid(ctx, j.gc, j.c9, ts.n_caustic, d, d.params.r_caustic, d.params.k_caustic)) != PM_OK)
+            return st;
+    } else if (j.gc) {
+        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        pm_grid_destroy(j.gc);
+        j.gc = nullptr;
+    }
+    nq = 0;
+    if ((st = pm_frame_rays(ctx, j.frame, d.scene, &d.camera, &d.params, &nq)) != PM_OK) return st;
+    return pm_frame_gather_jobs(j.frame, j.gg, j.gc, out, &n);
+}
+
+}  // namespace
+
 // One step: admit what fits, then the next frame of every active job with
 // all gathers in one batch. *frames_done = frames completed by this step.
 extern "C" pm_status pm_server_step(pm_server* S, uint32_t* frames_done) {
@@ -150,32 +246,47 @@ extern "C" pm_status pm_server_step(pm_server* S, uint32_t* frames_done) {
         S->active.push_back(std::move(j));
     }
     if (S->active.empty()) return PM_OK;
+    // per job: trace, grids (rebuilt in place), eye rays -> its gather jobs
+    const size_t na = S->active.size();
+    std::vector<pm_gather_job> fj(2 * na);
+    std::vector<uint32_t> nj(na, 0), nqs(na, 0);
+    const size_t W = S->workers.size();
+    if (W == 0) {
+        for (size_t i = 0; i < na; ++i)
+            if ((st = prepare(ctx, *S->active[i], &fj[2 * i], nj[i], nqs[i])) != PM_OK) return st;
+    } else {
+        // the workers' streams start after the main stream's earlier work
+        // (this step's queued frames read nothing of it, but a caller's
+        // work on the main stream may have produced the scenes)
+        PM_HIP(ctx, hipEventRecord(S->wevents[0], ctx->stream));
+        for (size_t w = 0; w < W; ++w) {
+            PM_HIP(ctx, hipStreamWaitEvent(S->wstreams[w], S->wevents[0], 0));
+            S->workers[w]->tunes = ctx->tunes;  // the tuned grid sizes
+        }
+        std::vector<pm_status> res(W, PM_OK);
+        std::vector<std::thread> th;
+        th.reserve(W);
+        for (size_t w = 0; w < W; ++w)
+            th.emplace_back([&, w] {
+                for (size_t i = w; i < na && res[w] == PM_OK; i += W)
+                    res[w] = prepare(S->workers[w], *S->active[i], &fj[2 * i], nj[i], nqs[i]);
+            });
+        for (auto& t : th) t.join();
+        for (size_t w = 0; w < W; ++w)
+            if (res[w] != PM_OK) {
+                ctx->err = S->workers[w]->err;
+                return res[w];
+            }
+        for (size_t w = 0; w < W; ++w) {
+            PM_HIP(ctx, hipEventRecord(S->wevents[w], S->wstreams[w]));
+            PM_HIP(ctx, hipStreamWaitEvent(ctx->stream, S->wevents[w], 0));
+        }
+    }
     std::vector<pm_gather_job> jobs;
-    jobs.reserve(2 * S->active.size());
-    for (auto& jp : S->active) {
-        Job& j = *jp;
-        const pm_job_desc& d = j.d;
-        const uint64_t seed = d.seed + 0x9E3779B97F4A7C15ull * (j.done + 1);
-        pm_trace_stats ts{};
-        if ((st = pm_trace_photons(ctx, d.scene, seed, d.n_global, d.n_caustic, d.max_bounces, d.max_paths, j.g9,
-                                   d.n_caustic ? j.c9 : nullptr, &ts)) != PM_OK)
-            return st;
-        if (j.gg) pm_grid_destroy(j.gg);
-        if (j.gc) pm_grid_destroy(j.gc);
-        j.gg = j.gc = nullptr;
-        if ((st = pm_grid_build(ctx, j.g9, ts.n_global, d.aabb, d.cell_size, 1000, PM_LAYOUT_LINEAR,
-                                &j.gg)) != PM_OK)
-            return st;
-        if (ts.n_caustic > 0 &&
-            (st = pm_grid_build(ctx, j.c9, ts.n_caustic, d.aabb, d.cell_size, 1000, PM_LAYOUT_LINEAR, &j.gc)) != PM_OK)
-            return st;
-        uint32_t nq = 0;
-        if ((st = pm_frame_rays(ctx, j.frame, d.scene, &d.camera, &d.params, &nq)) != PM_OK) return st;
-        pm_gather_job fj[2];
-        uint32_t nj = 0;
-        if ((st = pm_frame_gather_jobs(j.frame, j.gg, j.gc, fj, &nj)) != PM_OK) return st;
-        for (uint32_t i = 0; i < nj; ++i) jobs.push_back(fj[i]);
-        S->queries += (uint64_t)nq * nj;
+    jobs.reserve(2 * na);
+    for (size_t i = 0; i < na; ++i) {
+        for (uint32_t t = 0; t < nj[i]; ++t) jobs.push_back(fj[2 * i + t]);
+        S->queries += (uint64_t)nqs[i] * nj[i];
     }
     if ((st = pm_gather_batch(ctx, jobs.data(), (uint32_t)jobs.size(), PM_KNN_SORT_QUERIES)) != PM_OK) return st;
     for (auto& jp : S->active) {

@@ -34,6 +34,7 @@ KNN_LARGE_BUFFER = 1 << 5
 KNN_BATCH_QUEUE = 1 << 6
 KNN_XCD_CONTIGUOUS = 1 << 7
 KNN_INNER_BOX = 1 << 8
+KNN_TUNED = 1 << 9
 BOX_REFERENCE = 1
 # gather variants (tuning flags): the default is the 512-key buffer with
 # XCD-interleaved chunks of workgroups
@@ -102,6 +103,9 @@ _SIGS = {
     "pm_server_destroy": (_I32, [_P]),
     "pm_server_submit": (_I32, [_P, _P, C.POINTER(C.c_uint64)]),
     "pm_server_step": (_I32, [_P, C.POINTER(_U32)]),
+    "pm_server_set_workers": (_I32, [_P, _I32]),
+    "pm_tune_gather": (_I32, [_P, _P, _U32, _P, _P, _P, _U32, _F32, _I32, _P]),
+    "pm_tuning_lookup": (_I32, [_P, _U32, _P, _F32, _I32, _P]),
     "pm_server_stats": (_I32, [_P, _P]),
     "pm_server_latencies": (_I32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pm_accel_destroy": (_I32, [_P]),
@@ -134,7 +138,12 @@ class SceneRec(C.Structure):
 class GatherJob(C.Structure):
     _fields_ = [("grid", C.c_void_p), ("d_queries3", C.c_void_p), ("nq", C.c_uint32), ("radius", C.c_float),
                 ("k", C.c_int32), ("d_idx", C.c_void_p), ("d_dist", C.c_void_p), ("d_count", C.c_void_p),
-                ("d_normals3", C.c_void_p), ("d_rgb3", C.c_void_p)]
+                ("d_normals3", C.c_void_p), ("d_rgb3", C.c_void_p), ("flags", C.c_uint32)]
+
+
+class GatherTuning(C.Structure):
+    _fields_ = [("cell_over_r", C.c_float), ("flags", C.c_uint32), ("ms", C.c_float), ("ms_default", C.c_float),
+                ("found", C.c_int32)]
 
 
 class CameraRec(C.Structure):
@@ -343,6 +352,24 @@ class Context:
                                                             int(k), int(flags), _ptr(rgb), _ptr(count)))
         return rgb, count
 
+    def tune_gather(self, photons9, aabb, queries3, normals3, radius: float, k: int) -> dict:
+        """pm_tune_gather: time the cell sizes and gather variants on these
+        photons and queries; the choice is remembered for PM_KNN_TUNED."""
+        ph = _f32_dev(photons9, self.device)
+        q = _f32_dev(queries3, self.device)
+        nrm = _f32_dev(normals3, self.device)
+        ab = (C.c_float * 6)(*[float(v) for v in aabb])
+        t = GatherTuning()
+        self.check(self.lib.pm_tune_gather(self.h, _ptr(ph), ph.shape[1], ab, _ptr(q), _ptr(nrm), q.shape[1],
+                                           float(radius), int(k), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in GatherTuning._fields_}
+
+    def tuning_lookup(self, n: int, aabb, radius: float, k: int) -> dict:
+        ab = (C.c_float * 6)(*[float(v) for v in aabb])
+        t = GatherTuning()
+        self.check(self.lib.pm_tuning_lookup(self.h, int(n), ab, float(radius), int(k), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in GatherTuning._fields_}
+
     def gather_batch(self, jobs, flags: int = 0, want_knn: bool = False, want_count: bool = True):
         """Batched gather over many (grid, queries3, normals3, radius, k) jobs in
         one persistent launch. Returns per job a dict with "rgb" (when normals
@@ -351,12 +378,15 @@ class Context:
 
         recs = (GatherJob * max(len(jobs), 1))()
         outs, keep = [], []
-        for i, (grid, q3, n3, radius, k) in enumerate(jobs):
+        for i, job in enumerate(jobs):
+            # (grid, queries3, normals3, radius, k[, flags of this job])
+            grid, q3, n3, radius, k = job[:5]
             q = _f32_dev(q3, self.device)
             nq = q.shape[1]
             o = {}
             r = recs[i]
             r.grid, r.d_queries3, r.nq, r.radius, r.k = grid.h, _ptr(q), nq, float(radius), int(k)
+            r.flags = int(job[5]) if len(job) > 5 else 0
             keep.append(q)
             if n3 is not None:
                 nrm = _f32_dev(n3, self.device)
@@ -707,9 +737,13 @@ class Server:
         ctx.check(ctx.lib.pm_server_create(ctx.h, int(budget_bytes), C.byref(self.h)))
         self._keep = []
 
+    def set_workers(self, n: int):
+        """pm_server_set_workers: per-job stages on n worker contexts / streams."""
+        self.ctx.check(self.ctx.lib.pm_server_set_workers(self.h, int(n)))
+
     def submit(self, dscene: "DeviceScene", aabb, cell_size, n_global, n_caustic, camera, frames, seed,
                max_depth=4, k_global=64, r_global=0.05, k_caustic=64, r_caustic=0.05, max_bounces=10,
-               max_paths=1 << 40):
+               max_paths=1 << 40, gather_flags=0):
         """Queue a job; returns (job_id, rgb8 device tensor [frames, H*W*3])."""
         import torch
 
@@ -724,7 +758,8 @@ class Server:
         d.n_global, d.n_caustic, d.max_bounces = int(n_global), int(n_caustic), int(max_bounces)
         d.max_paths, d.seed = int(max_paths), int(seed)
         C.memmove(C.byref(d.camera), cam.ctypes.data, C.sizeof(CameraRec))
-        p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic), 0, 0)
+        p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic), 0,
+                         int(gather_flags))
         C.memmove(C.byref(d.params), C.byref(p), C.sizeof(RenderParams))
         d.frames = int(frames)
         d.d_rgb8 = out.data_ptr()

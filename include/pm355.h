@@ -53,10 +53,13 @@ enum {
     PM_KNN_BATCH_QUEUE = 1u << 6,    /* tuning (pm_gather_batch): persistent workgroups on a device work queue */
     PM_KNN_XCD_CONTIGUOUS = 1u << 7, /* tuning: each XCD takes one contiguous eighth of the workgroups
                                         instead of interleaved chunks of 256 */
-    PM_KNN_INNER_BOX = 1u << 8       /* k <= 256: each query first walks only its cell +- ceil(r/cell)/2 and
+    PM_KNN_INNER_BOX = 1u << 8,      /* k <= 256: each query first walks only its cell +- ceil(r/cell)/2 and
                                         stops there when the k-th nearest found is closer than anything
                                         outside; otherwise it walks the whole box (pays off when cells are
                                         small against r and k is well below the photons within r) */
+    PM_KNN_TUNED = 1u << 9           /* use the context's autotuned variant (pm_tune_gather) for this
+                                        gather's k and photon density in place of the tuning bits given;
+                                        the bits given stay when nothing was tuned */
 };
 
 typedef struct pm_ctx pm_ctx;
@@ -199,9 +202,41 @@ typedef struct {
     uint32_t* d_count;        /* optional [nq] */
     const float* d_normals3;  /* [3][nq], needed with d_rgb3 */
     float* d_rgb3;            /* optional [3][nq] radiance */
+    uint32_t flags;           /* this job's PM_KNN_INNER_BOX / PM_KNN_TUNED, OR-ed with the call's flags */
 } pm_gather_job;
 
 pm_status pm_gather_batch(pm_ctx* ctx, const pm_gather_job* h_jobs, uint32_t n_jobs, uint32_t flags);
+
+/* ---- autotuner: the launch configuration of the fused gather, chosen by
+ * measurement (the reference sizes its launches by fixed rules,
+ * CudaUtils.cu:42-68, Defines.cuh:5-72). pm_tune_gather times, on the
+ * caller's photons and queries, a grid build (rebuilt in place) at each
+ * candidate cell size (cell / r in {1/4, 1/3, 0.4, 1/2, 2/3, 1}) and, on the
+ * cell-ordered queries, the fused gather + radiance of each variant
+ * (512-key stream with interleaved or contiguous XCD chunks, 1024-key
+ * stream, two-pass, inner box), with HIP events on the context's stream.
+ * The fastest build + gather is returned next to the default (cell = r,
+ * no tuning bits) and remembered in the context under (k, photon density
+ * class: round(2 log2(n r^3 / V)), V the AABB volume). pm_tuning_lookup
+ * returns the remembered choice for a (k, density) (the nearest class with
+ * the same k, else the default with found = 0). Gathers flagged
+ * PM_KNN_TUNED take the remembered variant; pm_server jobs whose
+ * params.gather_flags hold PM_KNN_TUNED also build their grids at the
+ * remembered cell size. Results do not depend on the choice: every variant
+ * and cell size returns the same neighbours and radiance. */
+typedef struct {
+    float cell_over_r;   /* grid cell size / radius */
+    uint32_t flags;      /* PM_KNN_* tuning bits of the gather */
+    float ms;            /* grid rebuild + gather of the choice */
+    float ms_default;    /* the same at cell = r with no tuning bits */
+    int32_t found;       /* pm_tuning_lookup: 1 if a tuning was remembered for this k */
+} pm_gather_tuning;
+
+pm_status pm_tune_gather(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
+                         const float* d_queries3, const float* d_normals3, uint32_t nq, float radius, int32_t k,
+                         pm_gather_tuning* out);
+pm_status pm_tuning_lookup(const pm_ctx* ctx, uint32_t n, const float aabb[6], float radius, int32_t k,
+                           pm_gather_tuning* out);
 
 /* Per-query candidate statistics, summed: cells C_q and photons P_q.
  * flags = PM_BOX_REFERENCE: the reference's candidate box (cell +- ceil(r/cell),
@@ -446,6 +481,13 @@ pm_status pm_server_create(pm_ctx* ctx, size_t budget_bytes, pm_server** out);
 pm_status pm_server_destroy(pm_server* server);
 pm_status pm_server_submit(pm_server* server, const pm_job_desc* job, uint64_t* job_id);
 pm_status pm_server_step(pm_server* server, uint32_t* frames_done);
+/* Worker contexts for the per-job stages (trace, grids, eye rays): with
+ * n_workers > 0 each step runs those stages of its jobs on n_workers host
+ * threads, each driving a context on a stream of its own (created by the
+ * library at the default priority), and the shared gather launch on the
+ * server's context waits on all of them. 0 (the default) runs them on the
+ * server's context. Frames are identical either way. */
+pm_status pm_server_set_workers(pm_server* server, int32_t n_workers);
 pm_status pm_server_stats(const pm_server* server, pm_server_info* out);
 /* per-frame latency in seconds (eligible -> completed), oldest first */
 pm_status pm_server_latencies(const pm_server* server, double* out, uint64_t cap, uint64_t* n);

@@ -121,3 +121,63 @@ def test_empty_photon_map(ctx):
         ctx.sync()
         assert np.array_equal(got[f], rgb8.cpu().numpy())
     assert got.any()  # the light is in view
+
+
+def _serve_staggered(ctx, specs, arrivals, workers=0):
+    """Submit job i at step arrivals[i]; step until every job is done."""
+    srv = ctx.server(0)
+    if workers:
+        srv.set_workers(workers)
+    outs = [None] * len(specs)
+    steps = 0
+    while True:
+        for i, a in enumerate(arrivals):
+            if a == steps:
+                outs[i] = srv.submit(**specs[i])[1]
+        if steps > max(arrivals) and srv.info()["finished"] == len(specs):
+            break
+        srv.step()
+        steps += 1
+        assert steps < 100
+    ctx.sync()
+    res = [o.cpu().numpy() for o in outs]
+    srv.close()
+    return res
+
+
+def test_staggered_arrivals_with_workers_equal_alone(ctx):
+    """Jobs arriving at different steps, served with three worker contexts
+    (their trace / grid / eye-ray stages concurrent on three streams),
+    render every frame exactly as the job alone on a server without
+    workers."""
+    specs = _jobs(ctx)
+    mixed = _serve_staggered(ctx, specs, arrivals=[0, 1, 3], workers=3)
+    for s, m in zip(specs, mixed):
+        print("staggered job", s["seed"], flush=True)
+        alone = _serve(ctx, [s])[0][0]
+        assert np.array_equal(m, alone)
+        assert m.any()
+
+
+def test_tuned_jobs_equal_alone_and_mixed():
+    """Jobs flagged PM_KNN_TUNED build their grids at the tuned cell size and
+    gather with the tuned variant; they render the same frames alone and in a
+    staggered mix with workers."""
+    import pm355
+    from pm355.workloads import gather_microbench
+
+    c = pm355.Context(0)
+    try:
+        ph, q, qn, aabb = gather_microbench(seed=5, n_photons=200_000, n_queries=20_000)
+        t = c.tune_gather(ph, aabb, q, qn, 0.05, 50)
+        assert t["found"] == 1
+        specs = _jobs(c)
+        for s in specs:
+            s["gather_flags"] = pm355.KNN_TUNED
+        mixed = _serve_staggered(c, specs, arrivals=[0, 0, 2], workers=2)
+        for s, m in zip(specs, mixed):
+            alone = _serve(c, [s])[0][0]
+            assert np.array_equal(m, alone)
+            assert m.any()
+    finally:
+        c.close()
