@@ -3,11 +3,12 @@
 Workload: the gather microbenchmark -- 16M FP32 photons (70% uniform on the
 unit-cube faces, 30% in 64 Gaussian clusters on the floor), 4M queries on the
 faces, k = 64 nearest photons within r = 0.01. The hash-grid cell size
-(r/4, r/3, r/2, 2r/3 or r) and the gather variant are autotuned jointly; the result is
-the same for every choice. One step = build the photon hash grid from the
-photon block (keys -> radix sort -> cell bounds -> record gather) + the fused
-k-nearest gather and radiance estimate for all 4M queries. Inputs are
-resident in HBM before the timed region.
+(r/4, r/3, 0.4r, r/2, 2r/3 or r) and the gather variant are chosen jointly by
+the library's autotuner (pm_tune_gather); the neighbours are the same for
+every choice. One step = rebuild the resident photon hash grid in place from
+the photon block (keys -> radix sort -> cell sizes from the sorted runs ->
+scan -> record gather) + the fused k-nearest gather and radiance estimate for
+all 4M queries. Inputs are resident in HBM before the timed region.
 
 Prints one JSON line: metric, value (queries/s), step time, the roofline of
 the gather kernel and the CPU baseline, plus the per-frame, batched and
@@ -191,7 +192,7 @@ def frame_bench(ctx, n_frames):
     with torch.cuda.stream(kd_stream):
         ctx_kd = pm355.Context(ctx.device.index)
 
-    def frame(seed, overlap):
+    def frame(seed, overlap, cell_g=0.02, cell_c=0.02, gflags=0):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         kev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record(stream)
@@ -203,15 +204,15 @@ def frame_bench(ctx, n_frames):
                 kev[0].record(kd_stream)
                 kg, kc = ctx_kd.kdtree_build(g), ctx_kd.kdtree_build(c)
                 kev[1].record(kd_stream)
-        gg = ctx.grid_build(g, aabb, cell_size=0.02)
-        gc = ctx.grid_build(c, aabb, cell_size=0.02)
+        gg = ctx.grid_build(g, aabb, cell_size=cell_g)
+        gc = ctx.grid_build(c, aabb, cell_size=cell_c)
         ev[2].record(stream)
         if not overlap:
             kev[0].record(stream)
             kg, kc = ctx.kdtree_build(g), ctx.kdtree_build(c)
             kev[1].record(stream)
         ev[3].record(stream)
-        rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, 4, 100, 0.02, 100, 0.02)
+        rgb, rgb8, hits = ctx.render_frame(ds, gg, gc, cam, 4, 100, 0.02, 100, 0.02, gather_flags=gflags)
         ev[4].record(stream)
         if overlap:
             stream.wait_stream(kd_stream)
@@ -233,6 +234,24 @@ def frame_bench(ctx, n_frames):
             stored = st[1] + st[2]
         stages = np.median(np.array(stages), 0)
         out[overlap] = (lat, stages, stored)
+    # Tuned: the library's autotuner on one frame's global map with query
+    # points on its surfaces (every fourth stored photon's position, the
+    # normal facing its incoming light); both grids at their densities' tuned
+    # cell sizes and the render's gathers flagged PM_KNN_TUNED.
+    g, c, _ = ctx.trace_photons(ds, 7, 2_000_000, 500_000, 10, max_paths=200_000_000)
+    tq, tn = g[0:3, ::4].contiguous(), (-g[6:9, ::4]).contiguous()
+    tuning = ctx.tune_gather(g, aabb, tq, tn, 0.02, 100)
+    del g, c, tq, tn
+    cg = 0.02 * ctx.tuning_lookup(2_000_000, aabb, 0.02, 100)["cell_over_r"]
+    cc = 0.02 * ctx.tuning_lookup(500_000, aabb, 0.02, 100)["cell_over_r"]
+    frame(1, True, cg, cc, pm355.KNN_TUNED)
+    tlat, tstages = [], []
+    for i in range(n_frames):
+        t0 = time.perf_counter()
+        _, sm = frame(100 + i, True, cg, cc, pm355.KNN_TUNED)
+        tlat.append((time.perf_counter() - t0) * 1e3)
+        tstages.append(sm)
+    tstages = np.median(np.array(tstages), 0)
     ctx_kd.close()
     lat, stages, stored = out[True]
     slat = out[False][0]
@@ -266,6 +285,12 @@ def frame_bench(ctx, n_frames):
         "one_stream_stage_ms": {"trace": out[False][1][0], "grid_builds": out[False][1][1],
                                 "kdtree_builds": out[False][1][2], "render": out[False][1][3]},
         "trace_valu": valu,
+        "tuned": {"p50_ms": float(np.percentile(tlat, 50)), "p99_ms": float(np.percentile(tlat, 99)),
+                  "stage_ms": {"trace": tstages[0], "grid_builds": tstages[1], "kdtree_builds": tstages[2],
+                               "render": tstages[3]},
+                  "cell_over_r": {"global": cg / 0.02, "caustic": cc / 0.02},
+                  "variant": {v: n_ for n_, v in pm355.KNN_VARIANTS.items()}.get(tuning["flags"]),
+                  "p50_speedup_vs_default": float(np.percentile(lat, 50)) / float(np.percentile(tlat, 50))},
     }
 
 
@@ -426,6 +451,49 @@ def batch_bench(ctx, n_jobs, reps=2):
         res[name] = (time.perf_counter() - t0) / reps
     for g in grids:
         g.close()
+    grids.clear()
+
+    # Tuned: the library's autotuner on four jobs spread over the mix's photon
+    # densities (n r^3), every job's grid at its density's tuned cell size and
+    # its gathers flagged PM_KNN_TUNED (the nearest tuned density's variant).
+    dens = sorted(range(len(mix)), key=lambda i: mix[i]["photons"] * mix[i]["radius"] ** 3)
+    picks = sorted({dens[int(f * (len(dens) - 1))] for f in (0.2, 0.5, 0.8, 1.0)})
+    tunings = []
+    for i in picks:
+        j = mix[i]
+        ph, q, qn, aabb = gather_inputs_device(j["photons"], j["queries"], j["seed"], ctx.device)
+        t = ctx.tune_gather(ph, aabb, q, qn, j["radius"], j["k"])
+        tunings.append({"photons": j["photons"], "queries": j["queries"], "radius": j["radius"],
+                        "cell_over_r": t["cell_over_r"], "flags": t["flags"], "ms": t["ms"],
+                        "ms_default": t["ms_default"]})
+        del ph, q, qn
+    tjobs = []
+    for j, (_, q, qn, r, k) in zip(mix, jobs):
+        ph, _, _, aabb = gather_inputs_device(j["photons"], 1, j["seed"], ctx.device)
+        cell = r * ctx.tuning_lookup(j["photons"], aabb, r, k)["cell_over_r"]
+        g = ctx.grid_build(ph, aabb, cell_size=cell)
+        del ph
+        grids.append(g)
+        tjobs.append((g, q, qn, r, k, pm355.KNN_TUNED))
+    torch.cuda.synchronize()
+
+    def plain_tuned():
+        for g, q, qn, r, k, f in tjobs:
+            ctx.gather_radiance(g, q, qn, r, k, flags=plain_flags | f)
+
+    def batched_tuned():
+        ctx.gather_batch(tjobs, flags=pm355.KNN_SORT_QUERIES, want_count=False)
+
+    for name, fn in (("plain_tuned", plain_tuned), ("batched_tuned", batched_tuned)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps
+    for g in grids:
+        g.close()
     return {
         "workload": f"{n_jobs}-job gather mix: photons log-uniform 10k-4M per job ({n_ph} total), one query per "
                     f"pixel of 128^2/256^2/640x480/1280x720 frames ({nq_total} total), k=64, r U[0.01,0.05], "
@@ -439,27 +507,39 @@ def batch_bench(ctx, n_jobs, reps=2):
         "small_jobs": {"jobs": len(small), "queries": nq_small, "plain_ms": res["plain_small"] * 1e3,
                        "batched_ms": res["batched_small"] * 1e3,
                        "speedup": res["plain_small"] / res["batched_small"]},
+        "tuned": {"plain_ms": res["plain_tuned"] * 1e3, "batched_ms": res["batched_tuned"] * 1e3,
+                  "plain_speedup_vs_default": res["plain"] / res["plain_tuned"],
+                  "batched_speedup_vs_default": res["batched"] / res["batched_tuned"],
+                  "tunings": tunings,
+                  "note": "grids at each job's tuned cell size, gathers flagged PM_KNN_TUNED; the batched launch "
+                          "takes each job's inner-box choice, its kernel is the 512-key stream for all jobs"},
     }
 
 
-def serve_bench(ctx, rates=(10, 50, 200), n_jobs=12, seed=11):
+def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
+                configs=(("default", 0, 0), ("tuned", 0, 1))):
     """The served form: Poisson arrivals of render jobs (scenes of the batched
     distribution, resident with their BVHs; 50k-8M photons, 5% caustic;
     frames of 128x128 up to 1920x1080; 1-30 frames each; k = 64,
     r in [0.01, 0.05], cell = r) into the continuous-batching server.
-    Reports per arrival rate the p50 / p99 frame latency (eligible ->
-    completed) and gather queries/s over the run."""
+    Reports per arrival rate and server configuration the p50 / p99 frame
+    latency (eligible -> completed) and gather queries/s over the run, with
+    the jobs as given (cell = r) and flagged PM_KNN_TUNED (grid cell and
+    gather variant from the context's tunings, the nearest tuned photon
+    density), for the same arrivals."""
     import numpy as np
     import torch
 
     from pm355.scenes import camera, random_scene
 
-    rng = np.random.default_rng(seed)
     scenes = [random_scene(100 + i) for i in range(6)]
     dss = [ctx.upload_scene(sc, accel=True) for sc in scenes]
     res_set = [(128, 128), (256, 256), (640, 480), (1280, 720), (1920, 1080)]
+    import pm355
+
     out = {}
-    for rate in rates:
+    for rate, (cname, workers, tuned) in [(r_, c_) for r_ in rates for c_ in configs]:
+        rng = np.random.default_rng(seed + rate)
         jobs = []
         t = 0.0
         for j in range(n_jobs):
@@ -471,6 +551,9 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=12, seed=11):
             jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 20, frames=int(rng.integers(1, 31)), r=r,
                              seed=int(rng.integers(1, 1 << 62))))
         srv = ctx.server()
+        if workers:
+            srv.set_workers(workers)
+        log(f"served: {rate} jobs/s, {cname}")
         t0 = time.perf_counter()
         nxt = 0
         while nxt < len(jobs) or srv.info()["finished"] < len(jobs):
@@ -480,7 +563,8 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=12, seed=11):
                 sc = scenes[jb["scene"]]
                 cam = camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, jb["w"], jb["h"])
                 srv.submit(dss[jb["scene"]], sc.aabb(), jb["r"], jb["ng"], jb["nc"], cam, jb["frames"], jb["seed"],
-                           k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=3 * jb["ng"])
+                           k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=3 * jb["ng"],
+                           gather_flags=pm355.KNN_TUNED if tuned else 0)
                 nxt += 1
             if srv.step() == 0 and nxt < len(jobs):
                 time.sleep(max(0.0, min(0.01, jobs[nxt]["at"] - (time.perf_counter() - t0))))
@@ -488,9 +572,10 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=12, seed=11):
         wall = time.perf_counter() - t0
         lat = srv.latencies() * 1e3
         info = srv.info()
-        out[f"{rate}_jobs_per_s"] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                                     "frames": int(info["frames"]), "queries_per_s": info["queries"] / wall,
-                                     "wall_s": wall}
+        out.setdefault(f"{rate}_jobs_per_s", {})[cname] = {
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "frames": int(info["frames"]), "jobs": len(jobs), "queries_per_s": info["queries"] / wall,
+            "wall_s": wall}
         srv.close()
     return out
 
@@ -521,7 +606,7 @@ def main():
                     help="only trace the Cornell-box photon maps twice (for rocprofv3 counter runs) and exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after tuning, launch the tuned gather once on the cell-ordered block and exit")
-    ap.add_argument("--serve-jobs", type=int, default=12, help="render jobs per arrival rate for the served form (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=50, help="render jobs per arrival rate for the served form (0 = skip)")
     args = ap.parse_args()
 
     build = None
@@ -589,38 +674,29 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    # Autotune: the grid's cell size relative to r and the gather variant,
-    # jointly, for the step's time: each cell size's grid build and, on the
-    # cell-ordered query block, each variant's gather, timed with HIP events
-    # on the launch stream.
-    fracs = [args.cell_over_r] if args.cell_over_r else [0.25, 1.0 / 3.0, 0.4, 0.5, 2.0 / 3.0, 1.0]
-    variants = [args.variant] if args.variant else list(pm355.KNN_VARIANTS)
-    tune, best, best_ms = {}, None, None
-    for frac in fracs:
-        gt = ctx.grid_build(d_ph, aabb, cell_size=r * frac)
-        ot = ctx.query_order(gt, d_q).long()
-        qs, ns = d_q[:, ot].contiguous(), d_n[:, ot].contiguous()
-        torch.cuda.synchronize()
-        row = {"grid_build": timed(lambda: gt.rebuild(d_ph), 2)}
-        for name in variants:
-            vf = pm355.KNN_VARIANTS[name]
-            row[name] = timed(lambda: ctx.gather_radiance(gt, qs, ns, r, k, flags=vf), 2)
-            if best_ms is None or row[name] + row["grid_build"] < best_ms:
-                best, best_ms = (frac, name), row[name] + row["grid_build"]
-        tune[f"{frac:.3f}"] = row
-        gt.close()
-        del qs, ns
-    frac, vname = best
-    # Timings within 2% are noise: then keep the configuration whose
-    # counters were profiled (profiles/), so the bench line can report them.
-    tpath0 = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_counters.json")
-    if os.path.exists(tpath0) and not (args.cell_over_r or args.variant):
-        t0 = json.load(open(tpath0))
-        pf, pv = t0.get("cell_over_r"), t0.get("variant")
-        prow = tune.get(f"{pf:.3f}", {}) if pf is not None else {}
-        pms = prow.get(pv)
-        if pms is not None and pms + prow["grid_build"] <= best_ms * 1.02:
-            frac, vname = pf, pv
+    # Autotune (the library's pm_tune_gather): the grid's cell size relative
+    # to r and the gather variant, jointly, for the step's time: each cell
+    # size's grid rebuild and each variant's fused gather (queries sorted
+    # inside the call), timed with HIP events on the launch stream.
+    if args.cell_over_r or args.variant:
+        frac = args.cell_over_r or 1.0
+        vname = args.variant or "stream512_chunked"
+        tune, tuning = {}, None
+    else:
+        tuning = ctx.tune_gather(d_ph, aabb, d_q, d_n, r, k)
+        tune = ctx.tuning_table()
+        frac = tuning["cell_over_r"]
+        vname = {v: n_ for n_, v in pm355.KNN_VARIANTS.items()}[tuning["flags"]]
+        best_ms = tuning["ms"]
+        # Timings within 2% are noise: then keep the configuration whose
+        # counters were profiled (profiles/), so the bench line can report them.
+        tpath0 = os.path.join(ROOT, "profiles", "gather_microbench_16M_4M_k64_counters.json")
+        if os.path.exists(tpath0):
+            t0 = json.load(open(tpath0))
+            pf, pv = t0.get("cell_over_r"), t0.get("variant")
+            pms = tune.get(f"{pf:.3f}", {}).get(pv) if pf is not None else None
+            if pms is not None and pms <= best_ms * 1.02:
+                frac, vname = pf, pv
     cell = r * frac
     vflags = pm355.KNN_VARIANTS[vname]
     flags = (pm355.KNN_SORT_QUERIES if args.sort_queries else 0) | vflags
@@ -774,6 +850,11 @@ def main():
             "gather_radiance": gather_ms,
         },
         "autotune_ms": tune,
+        "autotune": None if tuning is None else {
+            "source": "pm_tune_gather (library): grid rebuild + sorted fused gather per cell size and variant",
+            "chosen": {"cell_over_r": tuning["cell_over_r"], "variant": vname, "ms": tuning["ms"]},
+            "default": {"cell_over_r": 1.0, "variant": "stream512_chunked", "ms": tuning["ms_default"]},
+            "speedup_vs_default": tuning["ms_default"] / tuning["ms"]},
         "gather_only_queries_per_s": nq / (gather_ms * 1e-3),
         "box_stats": {"reference_box_cell_r": {"cells": cells_ref, "candidates": cands_ref},
                       "walked_tuned_cell": {"cells": cells, "candidates": cands,
@@ -850,7 +931,9 @@ def main():
         log(f"served form: {args.serve_jobs} jobs per arrival rate")
         res["served"] = {"workload": "Poisson arrivals of render jobs: 6 resident scenes of the batched distribution "
                                      "(unit box + 1-8 spheres of 200-20k triangles, BVHs), 50k-8M global photons + 5% "
-                                     "caustic re-traced per frame (at most 3 paths per global photon), 128^2..1920x1080, 1-30 frames, k=64, r U[0.01,0.05]",
+                                     "caustic re-traced per frame (at most 3 paths per global photon), 128^2..1920x1080, "
+                                     "1-30 frames, k=64, r U[0.01,0.05]; per rate: jobs as given (cell = r) and tuned "
+                                     "(tunings of the batched mix and the headline)",
                          **serve_bench(ctx, n_jobs=args.serve_jobs)}
     if not args.no_cpu_baseline:
         log("cpu baseline")

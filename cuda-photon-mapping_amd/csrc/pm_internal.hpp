@@ -48,8 +48,10 @@ struct pm_ctx {
     // Frame state of pm_render_frame (hit slots and query block survive the
     // gathers, which use `scratch`).
     pm_frame* frame_state = nullptr;
-    // Gather tunings measured by pm_tune_gather on this context.
+    // Gather tunings measured by pm_tune_gather on this context, and the
+    // last tuning's table of measurements (cell sizes x variants, ms).
     std::vector<pm::TuneEntry> tunes;
+    std::vector<float> tune_table;
 };
 
 struct pm_grid {

@@ -111,6 +111,7 @@ extern "C" pm_status pm_tune_gather(pm_ctx* ctx, const float* d_photons9, uint32
         st = pm::fail(ctx, PM_ERR_HIP, "pm_tune_gather: events");
     pm_gather_tuning best{}, def{};
     best.ms = def.ms = -1.f;
+    ctx->tune_table.assign(6 * 5, -1.f);
     for (int fi = 0; st == PM_OK && fi < 6; ++fi) {
         const float frac = kFracs[fi];
         pm_grid* g = nullptr;
@@ -136,6 +137,7 @@ extern "C" pm_status pm_tune_gather(pm_ctx* ctx, const float* d_photons9, uint32
             });
             if (st != PM_OK) break;
             const float tot = build_ms + ms;
+            ctx->tune_table[(size_t)fi * 5 + vi] = tot;
             if (best.ms < 0.f || tot < best.ms) {
                 best.cell_over_r = frac;
                 best.flags = kVariants[vi];
@@ -162,5 +164,11 @@ extern "C" pm_status pm_tune_gather(pm_ctx* ctx, const float* d_photons9, uint32
     else
         ctx->tunes.push_back(pm::TuneEntry{k, cls, best});
     *out = best;
+    return PM_OK;
+}
+
+extern "C" pm_status pm_tuning_table(const pm_ctx* ctx, float* h_ms, uint32_t cap) {
+    if (!ctx || (!h_ms && cap)) return PM_ERR_INVALID;
+    for (uint32_t i = 0; i < cap; ++i) h_ms[i] = i < ctx->tune_table.size() ? ctx->tune_table[i] : -1.f;
     return PM_OK;
 }

@@ -106,6 +106,7 @@ _SIGS = {
     "pm_server_set_workers": (_I32, [_P, _I32]),
     "pm_tune_gather": (_I32, [_P, _P, _U32, _P, _P, _P, _U32, _F32, _I32, _P]),
     "pm_tuning_lookup": (_I32, [_P, _U32, _P, _F32, _I32, _P]),
+    "pm_tuning_table": (_I32, [_P, _P, _U32]),
     "pm_server_stats": (_I32, [_P, _P]),
     "pm_server_latencies": (_I32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pm_accel_destroy": (_I32, [_P]),
@@ -363,6 +364,21 @@ class Context:
         self.check(self.lib.pm_tune_gather(self.h, _ptr(ph), ph.shape[1], ab, _ptr(q), _ptr(nrm), q.shape[1],
                                            float(radius), int(k), C.byref(t)))
         return {f: getattr(t, f) for f, _ in GatherTuning._fields_}
+
+    TUNE_FRACS = (0.25, 1.0 / 3.0, 0.4, 0.5, 2.0 / 3.0, 1.0)
+    TUNE_VARIANTS = ("stream512_chunked", "stream512", "stream1024", "two_pass", "inner")
+
+    def tuning_table(self) -> dict:
+        """The last tune_gather's measurements: {cell/r: {variant: ms}}
+        (grid rebuild + gather; configurations not measured are left out)."""
+        buf = (C.c_float * 30)()
+        self.check(self.lib.pm_tuning_table(self.h, buf, 30))
+        out = {}
+        for fi, f in enumerate(self.TUNE_FRACS):
+            row = {v: buf[fi * 5 + vi] for vi, v in enumerate(self.TUNE_VARIANTS) if buf[fi * 5 + vi] >= 0}
+            if row:
+                out[f"{f:.3f}"] = row
+        return out
 
     def tuning_lookup(self, n: int, aabb, radius: float, k: int) -> dict:
         ab = (C.c_float * 6)(*[float(v) for v in aabb])

@@ -237,6 +237,11 @@ pm_status pm_tune_gather(pm_ctx* ctx, const float* d_photons9, uint32_t n, const
                          pm_gather_tuning* out);
 pm_status pm_tuning_lookup(const pm_ctx* ctx, uint32_t n, const float aabb[6], float radius, int32_t k,
                            pm_gather_tuning* out);
+/* The last pm_tune_gather's measurements on this context: h_ms[6 * 5]
+ * (cell sizes in the order above x variants 0, XCD_CONTIGUOUS,
+ * LARGE_BUFFER, TWO_PASS, INNER_BOX) of grid rebuild + gather in ms, -1 for
+ * a configuration not measured. */
+pm_status pm_tuning_table(const pm_ctx* ctx, float* h_ms, uint32_t cap);
 
 /* Per-query candidate statistics, summed: cells C_q and photons P_q.
  * flags = PM_BOX_REFERENCE: the reference's candidate box (cell +- ceil(r/cell),
@@ -486,7 +491,10 @@ pm_status pm_server_step(pm_server* server, uint32_t* frames_done);
  * threads, each driving a context on a stream of its own (created by the
  * library at the default priority), and the shared gather launch on the
  * server's context waits on all of them. 0 (the default) runs them on the
- * server's context. Frames are identical either way. */
+ * server's context. Frames are identical either way.
+ * Currently switched off: n_workers > 0 returns PM_ERR_UNSUPPORTED (a
+ * served run with workers was seen not to finish on the GPU; the cause is
+ * not yet known). */
 pm_status pm_server_set_workers(pm_server* server, int32_t n_workers);
 pm_status pm_server_stats(const pm_server* server, pm_server_info* out);
 /* per-frame latency in seconds (eligible -> completed), oldest first */

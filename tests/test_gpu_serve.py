@@ -145,6 +145,30 @@ def _serve_staggered(ctx, specs, arrivals, workers=0):
     return res
 
 
+def test_staggered_arrivals_equal_alone(ctx):
+    """Jobs arriving at different steps render every frame exactly as the
+    job alone."""
+    specs = _jobs(ctx)
+    mixed = _serve_staggered(ctx, specs, arrivals=[0, 1, 3])
+    for s, m in zip(specs, mixed):
+        print("staggered job", s["seed"], flush=True)
+        alone = _serve(ctx, [s])[0][0]
+        assert np.array_equal(m, alone)
+        assert m.any()
+
+
+def test_worker_contexts_are_switched_off(ctx):
+    import pm355
+
+    srv = ctx.server(0)
+    with pytest.raises(pm355.PMError):
+        srv.set_workers(2)
+    srv.set_workers(0)
+    srv.close()
+
+
+@pytest.mark.skip(reason="pm_server_set_workers is switched off: a served run with four worker contexts was seen "
+                         "not to finish on the GPU and the cause has not been found")
 def test_staggered_arrivals_with_workers_equal_alone(ctx):
     """Jobs arriving at different steps, served with three worker contexts
     (their trace / grid / eye-ray stages concurrent on three streams),
@@ -162,7 +186,7 @@ def test_staggered_arrivals_with_workers_equal_alone(ctx):
 def test_tuned_jobs_equal_alone_and_mixed():
     """Jobs flagged PM_KNN_TUNED build their grids at the tuned cell size and
     gather with the tuned variant; they render the same frames alone and in a
-    staggered mix with workers."""
+    staggered mix."""
     import pm355
     from pm355.workloads import gather_microbench
 
@@ -174,7 +198,7 @@ def test_tuned_jobs_equal_alone_and_mixed():
         specs = _jobs(c)
         for s in specs:
             s["gather_flags"] = pm355.KNN_TUNED
-        mixed = _serve_staggered(c, specs, arrivals=[0, 0, 2], workers=2)
+        mixed = _serve_staggered(c, specs, arrivals=[0, 0, 2])
         for s, m in zip(specs, mixed):
             alone = _serve(c, [s])[0][0]
             assert np.array_equal(m, alone)
