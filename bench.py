@@ -234,13 +234,14 @@ def frame_bench(ctx, n_frames):
             stored = st[1] + st[2]
         stages = np.median(np.array(stages), 0)
         out[overlap] = (lat, stages, stored)
-    # Tuned: the library's autotuner on one frame's global map with query
-    # points on its surfaces (every fourth stored photon's position, the
-    # normal facing its incoming light); both grids at their densities' tuned
-    # cell sizes and the render's gathers flagged PM_KNN_TUNED.
+    # Tuned: the library's autotuner on one frame's maps and the frame's own
+    # gather queries (its eye-ray hits), the global and the caustic map each;
+    # both grids at their tuned cell sizes and the render's gathers flagged
+    # PM_KNN_TUNED.
     g, c, _ = ctx.trace_photons(ds, 7, 2_000_000, 500_000, 10, max_paths=200_000_000)
-    tq, tn = g[0:3, ::4].contiguous(), (-g[6:9, ::4]).contiguous()
+    tq, tn = ctx.frame_queries(ds, cam, 4, 100, 0.02, 100, 0.02)
     tuning = ctx.tune_gather(g, aabb, tq, tn, 0.02, 100)
+    tuning_c = ctx.tune_gather(c, aabb, tq, tn, 0.02, 100)
     del g, c, tq, tn
     cg = 0.02 * ctx.tuning_lookup(2_000_000, aabb, 0.02, 100)["cell_over_r"]
     cc = 0.02 * ctx.tuning_lookup(500_000, aabb, 0.02, 100)["cell_over_r"]
@@ -267,7 +268,7 @@ def frame_bench(ctx, n_frames):
         tflops = v["fp32_flop_per_trace"] / (stages[0] * 1e-3) / 1e12
         valu = {"fp32_tflops": tflops, "peak_tflops": 157.3, "frac": tflops / 157.3,
                 "source": "profiles/cornell_c2_trace_valu.json",
-                "stale": v.get("source_hash") != profiling.source_hash()}
+                "stale": v.get("source_hash") != profiling.source_hash(v.get("source_files"))}
     return {
         "workload": "Cornell box (10 wall + 2x960 sphere + 2 light triangles, per-model BVHs), 2M global + 500k "
                     "caustic photons (<=10 bounces), grids cell=r and kd-trees built per frame, 1280x720, "
@@ -289,7 +290,8 @@ def frame_bench(ctx, n_frames):
                   "stage_ms": {"trace": tstages[0], "grid_builds": tstages[1], "kdtree_builds": tstages[2],
                                "render": tstages[3]},
                   "cell_over_r": {"global": cg / 0.02, "caustic": cc / 0.02},
-                  "variant": {v: n_ for n_, v in pm355.KNN_VARIANTS.items()}.get(tuning["flags"]),
+                  "variant": {m: {v: n_ for n_, v in pm355.KNN_VARIANTS.items()}.get(t_["flags"])
+                              for m, t_ in (("global", tuning), ("caustic", tuning_c))},
                   "p50_speedup_vs_default": float(np.percentile(lat, 50)) / float(np.percentile(tlat, 50))},
     }
 

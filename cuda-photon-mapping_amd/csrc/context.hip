@@ -105,6 +105,7 @@ pm_status pm_ctx_destroy(pm_ctx* ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     pm_frame_destroy(ctx->frame_state);
     if (ctx->scan_state) (void)hipFree(ctx->scan_state);
+    if (ctx->tile_sums) (void)hipFree(ctx->tile_sums);
     if (ctx->scan_counter) (void)hipFree(ctx->scan_counter);
     delete ctx;
     return PM_OK;

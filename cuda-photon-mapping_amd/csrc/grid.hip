@@ -1,5 +1,5 @@
 // Photon hash grid build: cell keys -> stable radix sort of (key, index) ->
-// cell sizes from the sorted runs -> exclusive scan into starts -> one
+// cell sizes from the sorted runs' ends -> exclusive scan into starts -> one
 // gather of each photon's record into the cell-ordered SoA planes.
 // Replaces PhotonGrid::build (PhotonGrid.cuh:51-66; PhotonGrid.cu:41-218).
 #include <cmath>
@@ -30,32 +30,24 @@ __global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9
     rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
 }
 
-// Cell sizes from the sorted keys (the counts of PhotonGrid.cu:152-174): the
-// last position of each run of equal keys writes the run's length into its
-// cell; every other cell keeps the 0 it was filled with. The exclusive scan
-// of the sizes gives the starts (PhotonGrid.cu:195-196).
+// Cell sizes from the sorted keys (the counts of PhotonGrid.cu:152-174): a
+// run of equal keys [s, e] adds e + 1 at its last position and subtracts s
+// at its first (mod 2^32), so each occupied cell ends at e + 1 - s and every
+// other cell keeps the 0 it was filled with. The exclusive scan of the
+// sizes gives the starts (PhotonGrid.cu:195-196).
 __global__ __launch_bounds__(256) void k_cell_sizes(const uint32_t* __restrict__ skeys, uint32_t n,
                                                    uint32_t* __restrict__ sizes) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t key = skeys[i];
-    if (i + 1 < n && skeys[i + 1] == key) return;
-    // the run's first position: gallop back over equal keys, then a binary
-    // search inside the last step (loads grow with log(run length))
-    uint32_t hi = i, step = 1;
-    while (step <= hi && skeys[hi - step] == key) {
-        hi -= step;
-        step <<= 1;
-    }
-    uint32_t lo = step <= hi ? hi - step + 1 : 0;
-    while (lo < hi) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        if (skeys[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    sizes[key] = i + 1 - lo;
+    const bool first = i == 0 || skeys[i - 1] != key;
+    const bool last = i + 1 == n || skeys[i + 1] != key;
+    if (first && last)
+        sizes[key] = 1u;
+    else if (first)
+        atomicSub(&sizes[key], i);
+    else if (last)
+        atomicAdd(&sizes[key], i + 1);
 }
 
 // analyze_density_kernel's max (PhotonGrid.cu:70-90), as a reduction.

@@ -35,6 +35,9 @@ struct pm_ctx {
     // [0]: ordered tile ids, reset by each launch's last tile; [32]: the
     // epoch of the next look-back launch (device-side, starts at 1)
     uint32_t* scan_counter = nullptr;
+    // tile sums and offsets of the two-level scan (large inputs)
+    uint32_t* tile_sums = nullptr;
+    size_t tile_sums_cap = 0;
     int cu_count = 256;
     // Fixed device memory that replaces the scratch arena and the look-back
     // state for the duration of one call (the allocation-free entry point):

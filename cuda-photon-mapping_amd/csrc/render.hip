@@ -488,6 +488,23 @@ extern "C" pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* F, float* d_rgb3, ui
     return PM_OK;
 }
 
+extern "C" pm_status pm_frame_copy_queries(pm_ctx* ctx, const pm_frame* F, float* d_queries3, float* d_normals3,
+                                           uint32_t cap, uint32_t* nq) {
+    if (!ctx || !F || !nq) return PM_ERR_INVALID;
+    if (!F->rays_done) return pm::fail(ctx, PM_ERR_INVALID, "pm_frame_copy_queries: no rays traced");
+    *nq = F->nq;
+    const uint32_t m = std::min(F->nq, cap);
+    if (m == 0) return PM_OK;
+    if (!d_queries3 || !d_normals3) return pm::fail(ctx, PM_ERR_INVALID, "pm_frame_copy_queries: null buffer");
+    for (int p = 0; p < 3; ++p) {
+        PM_HIP(ctx, hipMemcpyAsync(d_queries3 + (size_t)p * m, F->q3 + (size_t)p * F->nq, (size_t)m * 4,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        PM_HIP(ctx, hipMemcpyAsync(d_normals3 + (size_t)p * m, F->n3 + (size_t)p * F->nq, (size_t)m * 4,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return PM_OK;
+}
+
 extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* global_grid,
                                      const pm_grid* caustic_grid, const pm_camera* camera,
                                      const pm_render_params* params, float* d_rgb3, uint8_t* d_rgb8,

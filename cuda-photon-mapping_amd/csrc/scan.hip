@@ -6,7 +6,10 @@
 // the reference's 44-block Blelloch scan with its software grid barrier
 // (PrefixScanSum.cuh:12-279, CudaGridSynchronizer.cuh:14-27), which needs all
 // blocks co-resident; here a tile only waits on tiles that took a lower
-// ordered id, so any number of tiles may be in flight.
+// ordered id, so any number of tiles may be in flight. Inputs of 4M
+// elements and more take a two-level scan instead (tile sums, a scan of the
+// sums, tiles rescanned from their offsets), whose workgroups never wait on
+// each other.
 #include "pm_internal.hpp"
 
 namespace {
@@ -16,6 +19,8 @@ constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;  // 4096
 
 constexpr uint64_t kStatusAgg = 1, kStatusIncl = 2;
+// inputs of at least this many elements take the two-level scan
+constexpr uint32_t kTwoLevelMin = 1u << 22;
 
 __device__ __forceinline__ uint64_t pack_state(uint32_t epoch, uint64_t status, uint32_t value) {
     return ((uint64_t)epoch << 34) | (status << 32) | value;
@@ -164,6 +169,93 @@ __global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t* __restric
 
 __global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
+// ---- two-level scan for large inputs: tile sums -> scan of the sums ->
+// each tile rescanned from its offset. Three passes and no waiting between
+// workgroups: past a few million elements the single pass is held up by
+// its ordered tile counter, one device-scope atomic per tile.
+__global__ __launch_bounds__(kThreads) void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                       uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)t * kItems;
+    uint32_t sum = 0;
+    if (base + kItems <= n && ((reinterpret_cast<uintptr_t>(in) & 15) == 0)) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int j = 0; j < kItems / 4; ++j) {
+            const uint4 q = p[j];
+            sum += q.x + q.y + q.z + q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) sum += (base + j < n) ? in[base + j] : 0u;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) s_wave[wave] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t a = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) a += s_wave[w];
+        sums[blockIdx.x] = a;
+    }
+}
+
+template <bool kInclusive>
+__global__ __launch_bounds__(kThreads) void k_tile_rescan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                         uint32_t n, const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t s_wave[kThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)t * kItems;
+    const bool full = base + kItems <= n;
+    uint32_t v[kItems];
+    if (full && ((reinterpret_cast<uintptr_t>(in) & 15) == 0)) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int j = 0; j < kItems / 4; ++j) {
+            const uint4 q = p[j];
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) v[j] = (base + j < n) ? in[base + j] : 0u;
+    }
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) tsum += v[j];
+    const uint32_t incl = wave_incl_scan(tsum, lane);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t run = offs[blockIdx.x] + (incl - tsum);
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w)
+        if ((uint32_t)w < wave) run += s_wave[w];
+    uint32_t o[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (kInclusive) {
+            run += v[j];
+            o[j] = run;
+        } else {
+            o[j] = run;
+            run += v[j];
+        }
+    }
+    if (full && ((reinterpret_cast<uintptr_t>(out) & 15) == 0)) {
+        uint4* p = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int j = 0; j < kItems / 4; ++j) p[j] = make_uint4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (base + j < n) out[base + j] = o[j];
+    }
+}
+
 }  // namespace
 
 namespace pm {
@@ -202,6 +294,30 @@ pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, b
         return PM_OK;
     }
     uint32_t tiles = div_up(n, kTile);
+    if (n >= kTwoLevelMin && !ctx->fixed_state) {
+        // tile sums and their offsets in the context's own block (callers'
+        // scratch carves stay untouched)
+        if (2 * (size_t)tiles > ctx->tile_sums_cap) {
+            PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->tile_sums) PM_HIP(ctx, hipFree(ctx->tile_sums));
+            ctx->tile_sums = nullptr;
+            const size_t want = 2 * (size_t)tiles + tiles;
+            PM_HIP(ctx, hipMalloc(&ctx->tile_sums, want * 4));
+            ctx->tile_sums_cap = want;
+        }
+        uint32_t* sums = ctx->tile_sums;
+        uint32_t* offs = ctx->tile_sums + tiles;
+        hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, n, sums);
+        PM_CHECK_LAUNCH(ctx, "k_tile_sums");
+        pm_status st = scan_u32(ctx, sums, offs, tiles, false, total);
+        if (st != PM_OK) return st;
+        if (inclusive)
+            hipLaunchKernelGGL(k_tile_rescan<true>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, offs);
+        else
+            hipLaunchKernelGGL(k_tile_rescan<false>, dim3(tiles), dim3(kThreads), 0, ctx->stream, in, out, n, offs);
+        PM_CHECK_LAUNCH(ctx, "k_tile_rescan");
+        return PM_OK;
+    }
     uint64_t* state;
     uint32_t* counter;
     pm_status st = lookback_begin(ctx, tiles, &state, &counter);

@@ -107,6 +107,7 @@ _SIGS = {
     "pm_tune_gather": (_I32, [_P, _P, _U32, _P, _P, _P, _U32, _F32, _I32, _P]),
     "pm_tuning_lookup": (_I32, [_P, _U32, _P, _F32, _I32, _P]),
     "pm_tuning_table": (_I32, [_P, _P, _U32]),
+    "pm_frame_copy_queries": (_I32, [_P, _P, _P, _P, _U32, C.POINTER(_U32)]),
     "pm_server_stats": (_I32, [_P, _P]),
     "pm_server_latencies": (_I32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pm_accel_destroy": (_I32, [_P]),
@@ -530,6 +531,29 @@ class Context:
         self.check(self.lib.pm_pipeline_create(self.h, C.byref(d), C.byref(h)))
         cam = np.ascontiguousarray(camera)
         return Pipeline(self, h, dscene, int(cam["width"].item()) * int(cam["height"].item()))
+
+    def frame_queries(self, dscene: "DeviceScene", camera, max_depth=4, k_global=50, r_global=0.05, k_caustic=50,
+                      r_caustic=0.05):
+        """A frame's gather queries (pm_frame_rays + pm_frame_copy_queries):
+        (queries3, normals3) device tensors of the eye-ray hits."""
+        import torch
+
+        f = _P()
+        self.check(self.lib.pm_frame_create(self.h, C.byref(f)))
+        try:
+            cam = np.ascontiguousarray(camera)
+            p = RenderParams(int(max_depth), int(k_global), float(r_global), int(k_caustic), float(r_caustic), 0, 0)
+            nq = _U32()
+            self.check(self.lib.pm_frame_rays(self.h, f, C.byref(dscene.rec), cam.ctypes.data, C.byref(p),
+                                              C.byref(nq)))
+            q = torch.empty((3, max(nq.value, 1)), dtype=torch.float32, device=self.device)
+            n = torch.empty_like(q)
+            got = _U32()
+            self.check(self.lib.pm_frame_copy_queries(self.h, f, _ptr(q), _ptr(n), nq.value, C.byref(got)))
+            self.sync()
+            return q[:, :nq.value], n[:, :nq.value]
+        finally:
+            self.lib.pm_frame_destroy(f)
 
     def render_frames_batched(self, frames, flags: int = 0, want_rgb8: bool = True):
         """Render many frames with their gathers in one pm_gather_batch:

@@ -9,7 +9,7 @@ measured on; bench.py flags the numbers as stale when the sources have
 changed since.
 
     python -m pm355.profiling summarize --kernel k_knn_stream512 \
-        --pmc DIR [DIR ...] --copy DIR [DIR ...] --meta key=value ... --out FILE
+        --pmc DIR [DIR ...] --copy DIR [DIR ...] --meta sources=gather key=value ... --out FILE
     python -m pm355.profiling valu --pmc DIR --traces 2 --out FILE
 
 DIRs are rocprofv3 output directories (-d) of --pmc runs; --copy runs are
@@ -37,10 +37,21 @@ HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30
 
 
-def source_hash() -> str:
-    """sha256 over the HIP sources of libpm355.so (names and contents)."""
+# The sources each profiled kernel is compiled from: its own file and the
+# shared headers (a change anywhere else cannot change the kernel).
+KERNEL_SOURCES = {
+    "gather": ["knn.hip", "pm_select.hpp", "pm_internal.hpp"],
+    "trace": ["trace.hip", "pm_geom.hpp", "pm_internal.hpp"],
+}
+
+
+def source_hash(files=None) -> str:
+    """sha256 over HIP sources of libpm355.so (names and contents): the
+    given file names under csrc/, or every source."""
     h = hashlib.sha256()
-    for p in sorted(glob.glob(os.path.join(CSRC, "*"))):
+    paths = sorted(glob.glob(os.path.join(CSRC, "*"))) if files is None else \
+        [os.path.join(CSRC, f) for f in sorted(files)]
+    for p in paths:
         h.update(os.path.basename(p).encode())
         with open(p, "rb") as f:
             h.update(f.read())
@@ -75,8 +86,9 @@ def summarize(kernel, pmc_dirs, copy_dirs, meta, out):
     k = per_dispatch(pmc_dirs, kernel)
     c = per_dispatch(copy_dirs, "__amd_rocclr_copyBuffer")
     med = {n: statistics.median(v) for n, v in k.items()}
-    res = {"kernel": kernel, "source_hash": source_hash(), "dispatches": {n: len(v) for n, v in k.items()},
-           "per_dispatch_median": med, **meta}
+    files = KERNEL_SOURCES.get(meta.get("sources", ""))
+    res = {"kernel": kernel, "source_hash": source_hash(files), "source_files": files,
+           "dispatches": {n: len(v) for n, v in k.items()}, "per_dispatch_median": med, **meta}
     # FETCH_SIZE / WRITE_SIZE in KiB; the copy probe's 1 GiB copies give the
     # scale from counter units to bytes read / written
     if "FETCH_SIZE" in med:
@@ -110,8 +122,9 @@ def summarize_valu(pmc_dirs, traces, meta, out):
             tot[r["Counter_Name"]] = tot.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
     flop = 64.0 * (tot.get("SQ_INSTS_VALU_ADD_F32", 0) + tot.get("SQ_INSTS_VALU_MUL_F32", 0)
                    + tot.get("SQ_INSTS_VALU_TRANS_F32", 0) + 2 * tot.get("SQ_INSTS_VALU_FMA_F32", 0))
+    files = KERNEL_SOURCES["trace"]
     res = {"fp32_flop_per_trace": flop / traces, "counter_totals": tot, "traces": traces,
-           "source_hash": source_hash(), **meta}
+           "source_hash": source_hash(files), "source_files": files, **meta}
     with open(out, "w") as f:
         json.dump(res, f, indent=1, sort_keys=True)
     return res
@@ -132,7 +145,7 @@ def load(path, **match):
                 return None
         elif got != want:
             return None
-    s["stale"] = s.get("source_hash") != source_hash()
+    s["stale"] = s.get("source_hash") != source_hash(s.get("source_files"))
     return s
 
 
@@ -150,10 +163,11 @@ def main(argv=None):
     v.add_argument("--traces", type=int, required=True)
     v.add_argument("--meta", nargs="*", default=[])
     v.add_argument("--out", required=True)
-    sub.add_parser("hash")
+    hs = sub.add_parser("hash")
+    hs.add_argument("--sources", default="", help="kernel source set (gather, trace); empty = every source")
     a = ap.parse_args(argv)
     if a.cmd == "hash":
-        print(source_hash())
+        print(source_hash(KERNEL_SOURCES.get(a.sources)))
         return
     meta = {}
     for kv in a.meta:

@@ -404,6 +404,12 @@ pm_status pm_frame_rays(pm_ctx* ctx, pm_frame* frame, const pm_scene* scene, con
 pm_status pm_frame_gather_jobs(pm_frame* frame, const pm_grid* global_grid, const pm_grid* caustic_grid,
                                pm_gather_job* jobs /* [2] */, uint32_t* n_jobs);
 pm_status pm_frame_finish(pm_ctx* ctx, pm_frame* frame, float* d_rgb3, uint8_t* d_rgb8, pm_frame_stats* stats);
+/* The frame's gather queries after pm_frame_rays (hit points and normals,
+ * [3][nq] each) copied into caller device buffers of at most cap queries,
+ * on the context's stream; *nq = the frame's query count (copied:
+ * min(nq, cap)). For tuning a renderer's gathers on its own queries. */
+pm_status pm_frame_copy_queries(pm_ctx* ctx, const pm_frame* frame, float* d_queries3, float* d_normals3,
+                                uint32_t cap, uint32_t* nq);
 
 /* ---- per-frame pipeline for graph capture: the whole frame of the reference
  * (trace_photons, PhotonGrid::build of both maps, Raytracer::render_gpu:

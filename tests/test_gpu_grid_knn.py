@@ -41,7 +41,8 @@ def test_scan_kat(ctx):
     assert int(tot.item()) == ref_tot == 665_385
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 123_457, 3_000_001])
+# (4,194,304 elements and more: the two-level scan)
+@pytest.mark.parametrize("n", [1, 63, 64, 4095, 4096, 4097, 123_457, 3_000_001, 4_194_304, 9_000_011])
 def test_scan_sizes(ctx, n):
     import torch
 

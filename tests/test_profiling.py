@@ -44,6 +44,12 @@ def test_summarize_scales_and_rates(tmp_path):
     d["source_hash"] = "0" * 16
     json.dump(d, open(out, "w"))
     assert profiling.load(str(out), variant="v")["stale"] is True
+    # a summary of the gather kernel is tied to the gather's own sources only
+    s2 = profiling.summarize("k_knn_stream512", [str(tmp_path / "a")], [], {"sources": "gather"},
+                             str(tmp_path / "g.json"))
+    assert s2["source_files"] == profiling.KERNEL_SOURCES["gather"]
+    assert s2["source_hash"] == profiling.source_hash(profiling.KERNEL_SOURCES["gather"])
+    assert profiling.load(str(tmp_path / "g.json"))["stale"] is False
 
 
 def test_valu_summary(tmp_path):
