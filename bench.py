@@ -518,6 +518,107 @@ def batch_bench(ctx, n_jobs, reps=2):
     }
 
 
+def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
+    """The batched form on whole frames of the many-scene mix: n_scenes
+    scenes of the batched distribution (unit box + 1-8 spheres of 200-20k
+    triangles, BVHs; diffuse / mirror / glass), global photons log-uniform in
+    10k-4M (+5% caustic when the scene has a specular or refractive model;
+    at most 3 paths per requested photon), canvases of 128^2 / 256^2 /
+    640x480 / 1280x720, k = 64, r U[0.01, 0.05], cells of r, depth 4.
+
+    plain: one scene's frame at a time, as the reference renders -- trace,
+    both grid builds, pm_render_frame (its own gather launch).
+    batched: every scene's trace and grids, then every frame's eye rays, the
+    gathers of all frames in one pm_gather_batch, and each frame's fold.
+    Both on one context and stream; the frames are identical."""
+    import numpy as np
+    import torch
+
+    from pm355.scenes import camera, random_scene
+
+    rng = np.random.default_rng(seed)
+    res_set = [(128, 128), (256, 256), (640, 480), (1280, 720)]
+    jobs = []
+    for i in range(n_scenes):
+        sc = random_scene(1000 + i)
+        spec = any(float(m["opaque"]) != 1.0 or bool(np.any(m["ks"] != 0.0))
+                   for m, nm in zip(sc.models, sc.names) if nm.startswith("sphere"))
+        ng = int(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
+        w, h = res_set[int(rng.integers(0, len(res_set)))]
+        r = float(rng.uniform(0.01, 0.05))
+        jobs.append(dict(scene=sc, aabb=sc.aabb(), ng=ng, nc=ng // 20 if spec else 0, r=r,
+                         seed=int(rng.integers(1, 1 << 62)),
+                         cam=camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, w, h)))
+    log(f"frames mix: {n_scenes} scenes drawn")
+    for i, j in enumerate(jobs):
+        j["ds"] = ctx.upload_scene(j["scene"], accel=True)
+    torch.cuda.synchronize()
+    log("frames mix: scenes resident")
+    n_ph = [0]
+    n_q = [0]
+
+    def maps(j):
+        g, c, st = ctx.trace_photons(j["ds"], j["seed"], j["ng"], j["nc"], 10, max_paths=3 * (j["ng"] + j["nc"]))
+        gg = ctx.grid_build(g[:, :st[1]], j["aabb"], cell_size=j["r"])
+        gc = ctx.grid_build(c[:, :st[2]], j["aabb"], cell_size=j["r"]) if st[2] > 0 else None
+        n_ph[0] += st[1] + st[2]
+        return gg, gc
+
+    def plain():
+        outs = []
+        for j in jobs:
+            gg, gc = maps(j)
+            _, rgb8, hits = ctx.render_frame(j["ds"], gg, gc, j["cam"], 4, 64, j["r"], 64, j["r"])
+            n_q[0] += hits * (2 if gc else 1)
+            outs.append(rgb8)
+            gg.close()
+            if gc:
+                gc.close()
+        return outs
+
+    def batched():
+        grids = [maps(j) for j in jobs]
+        res = ctx.render_frames_batched([(j["ds"], gg, gc, j["cam"], 4, 64, j["r"], 64, j["r"])
+                                         for j, (gg, gc) in zip(jobs, grids)])
+        for gg, gc in grids:
+            gg.close()
+            if gc:
+                gc.close()
+        return [r8 for _, r8, _ in res]
+
+    out = {}
+    first = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        log(f"frames mix: {name}")
+        first[name] = fn()
+        torch.cuda.synchronize()
+        n_ph[0] = n_q[0] = 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / reps
+        out[name + "_photons"] = n_ph[0] // reps
+        if name == "plain":
+            out["queries"] = n_q[0] // reps
+    same = all(torch.equal(a, b) for a, b in zip(first["plain"], first["batched"]))
+    jobs.clear()
+    return {
+        "workload": f"{n_scenes} scenes of the batched distribution (unit box + 1-8 spheres of 200-20k triangles, "
+                    "BVHs; diffuse/mirror/glass), global photons log-uniform 10k-4M (+5% caustic, <= 3 paths per "
+                    "photon), 128^2/256^2/640x480/1280x720 canvases, k=64, r U[0.01,0.05], cell=r, depth 4; one "
+                    "frame per scene (trace, grids, eye rays, gathers, fold)",
+        "plain_ms": out["plain"] * 1e3,
+        "batched_ms": out["batched"] * 1e3,
+        "speedup": out["plain"] / out["batched"],
+        "photons_traced": out["plain_photons"],
+        "gather_queries": out["queries"],
+        "frames_identical": bool(same),
+        "note": "batched = all traces and grids, one pm_gather_batch for every frame's gathers; the per-frame "
+                "graph-captured pipeline that would also batch the traces is switched off",
+    }
+
+
 def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
                 configs=(("default", 0, 0), ("tuned", 0, 1))):
     """The served form: Poisson arrivals of render jobs (scenes of the batched
@@ -535,6 +636,7 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
     from pm355.scenes import camera, random_scene
 
     scenes = [random_scene(100 + i) for i in range(6)]
+    boxes = [sc.aabb() for sc in scenes]
     dss = [ctx.upload_scene(sc, accel=True) for sc in scenes]
     res_set = [(128, 128), (256, 256), (640, 480), (1280, 720), (1920, 1080)]
     import pm355
@@ -564,7 +666,7 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
                 jb = jobs[nxt]
                 sc = scenes[jb["scene"]]
                 cam = camera((0.5, 0.5, 1.85), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, jb["w"], jb["h"])
-                srv.submit(dss[jb["scene"]], sc.aabb(), jb["r"], jb["ng"], jb["nc"], cam, jb["frames"], jb["seed"],
+                srv.submit(dss[jb["scene"]], boxes[jb["scene"]], jb["r"], jb["ng"], jb["nc"], cam, jb["frames"], jb["seed"],
                            k_global=64, r_global=jb["r"], k_caustic=64, r_caustic=jb["r"], max_paths=3 * jb["ng"],
                            gather_flags=pm355.KNN_TUNED if tuned else 0)
                 nxt += 1
@@ -609,6 +711,7 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after tuning, launch the tuned gather once on the cell-ordered block and exit")
     ap.add_argument("--serve-jobs", type=int, default=50, help="render jobs per arrival rate for the served form (0 = skip)")
+    ap.add_argument("--frame-scenes", type=int, default=256, help="scenes of the batched frames mix (0 = skip)")
     args = ap.parse_args()
 
     build = None
@@ -929,6 +1032,9 @@ def main():
     if args.batch_jobs > 0:
         log(f"batched form: {args.batch_jobs}-job gather mix")
         res["batched"] = batch_bench(ctx, args.batch_jobs)
+    if args.frame_scenes > 0:
+        log(f"batched frames: {args.frame_scenes}-scene frame mix")
+        res["batched_frames"] = frames_batch_bench(ctx, args.frame_scenes)
     if args.serve_jobs > 0:
         log(f"served form: {args.serve_jobs} jobs per arrival rate")
         res["served"] = {"workload": "Poisson arrivals of render jobs: 6 resident scenes of the batched distribution "

@@ -2,6 +2,9 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <mutex>
+#include <unordered_set>
+
 #include "pm_internal.hpp"
 
 namespace pm {
@@ -57,6 +60,43 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
     for (; i < count; i += stride) p[i] = v;
 }
 
+namespace {
+// live contexts (a grid may outlive the context it was built on) and their
+// block caches, under one lock
+std::mutex g_ctx_mu;
+std::unordered_set<pm_ctx*> g_ctx_live;
+constexpr size_t kBlockCacheCap = (size_t)4 << 30;
+}  // namespace
+
+hipError_t block_take(pm_ctx* ctx, size_t bytes, void** p, size_t* got) {
+    {
+        std::lock_guard<std::mutex> lock(g_ctx_mu);
+        auto& c = ctx->block_cache;
+        size_t best = c.size();
+        for (size_t i = 0; i < c.size(); ++i)
+            if (c[i].second >= bytes && c[i].second <= 2 * bytes && (best == c.size() || c[i].second < c[best].second))
+                best = i;
+        if (best != c.size()) {
+            *p = c[best].first;
+            *got = c[best].second;
+            ctx->block_cache_bytes -= c[best].second;
+            c.erase(c.begin() + (std::ptrdiff_t)best);
+            return hipSuccess;
+        }
+    }
+    *got = bytes;
+    return hipMalloc(p, bytes);
+}
+
+bool block_give(pm_ctx* owner, void* p, size_t bytes) {
+    if (!owner || !p) return false;
+    std::lock_guard<std::mutex> lock(g_ctx_mu);
+    if (!g_ctx_live.count(owner) || owner->block_cache_bytes + bytes > kBlockCacheCap) return false;
+    owner->block_cache.emplace_back(p, bytes);
+    owner->block_cache_bytes += bytes;
+    return true;
+}
+
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count) {
     if (count == 0) return PM_OK;
     uint32_t blocks = (uint32_t)std::min<size_t>((count + 255) / 256, 8192);
@@ -95,6 +135,10 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
     // which reports it)
     (void)pm::noalloc_setup(c);
     c->err.clear();
+    {
+        std::lock_guard<std::mutex> lock(pm::g_ctx_mu);
+        pm::g_ctx_live.insert(c);
+    }
     *out = c;
     return PM_OK;
 }
@@ -102,6 +146,13 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
 pm_status pm_ctx_destroy(pm_ctx* ctx) {
     if (!ctx) return PM_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    {
+        std::lock_guard<std::mutex> lock(pm::g_ctx_mu);
+        pm::g_ctx_live.erase(ctx);
+        for (auto& b : ctx->block_cache) (void)hipFree(b.first);
+        ctx->block_cache.clear();
+        ctx->block_cache_bytes = 0;
+    }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     pm_frame_destroy(ctx->frame_state);
     if (ctx->scan_state) (void)hipFree(ctx->scan_state);

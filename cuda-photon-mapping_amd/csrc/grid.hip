@@ -149,11 +149,13 @@ pm_status build_empty(pm_ctx* ctx, const float aabb[6], float cell_size, int lay
     pm_grid* g = new pm_grid();
     // the photon planes and the permutation are empty; they point at a
     // zeroed 256-byte block so no pointer is null
-    hipError_t e = hipMalloc(&g->mem, pm::carve_size({(size_t)d.n_cells * 4, (size_t)d.n_cells * 4, 256}));
+    hipError_t e = pm::block_take(ctx, pm::carve_size({(size_t)d.n_cells * 4, (size_t)d.n_cells * 4, 256}), &g->mem,
+                                  &g->mem_bytes);
     if (e != hipSuccess) {
         delete g;
         return pm::hip_fail(ctx, e, "hipMalloc(grid)");
     }
+    g->owner = ctx;
     pm::Carve gc{static_cast<char*>(g->mem)};
     uint32_t* starts = gc.take<uint32_t>(d.n_cells);
     uint32_t* sizes = gc.take<uint32_t>(d.n_cells);
@@ -161,8 +163,7 @@ pm_status build_empty(pm_ctx* ctx, const float aabb[6], float cell_size, int lay
     pm_status st;
     if ((st = pm::fill_u32(ctx, starts, 0u, d.n_cells)) != PM_OK ||
         (st = pm::fill_u32(ctx, sizes, 0u, d.n_cells)) != PM_OK || (st = pm::fill_u32(ctx, nothing, 0u, 64)) != PM_OK) {
-        (void)hipFree(g->mem);
-        delete g;
+        pm_grid_destroy(g);
         return st;
     }
     pm_grid_desc& D = g->desc;
@@ -288,16 +289,16 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     // One build at dims d into a fresh allocation of capacity n.
     auto build_once = [&](const HostDims& dd, pm_grid*& out_g) -> pm_status {
         pm_grid* g = new pm_grid();
-        hipError_t e = hipMalloc(&g->mem, grid_bytes(dd.n_cells, n));
+        hipError_t e = pm::block_take(ctx, grid_bytes(dd.n_cells, n), &g->mem, &g->mem_bytes);
         if (e != hipSuccess) {
             delete g;
             return pm::hip_fail(ctx, e, "hipMalloc(grid)");
         }
+        g->owner = ctx;
         g->n_cap = n;
         pm_status bs = enqueue_build(ctx, d_photons9, n, aabb, dd, layout, g);
         if (bs != PM_OK) {
-            (void)hipFree(g->mem);
-            delete g;
+            pm_grid_destroy(g);
             return bs;
         }
         out_g = g;
@@ -319,8 +320,7 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         PM_HIP(ctx, hipMemcpyAsync(&mx, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (mx > 5u * (uint32_t)desired) {
-            (void)hipFree(g->mem);
-            delete g;
+            pm_grid_destroy(g);
             g = nullptr;
             cell = reference_cell(n, aabb, std::max(1, desired / 5));
             if (!(cell > 0.f) || !make_dims(aabb, cell, layout, d))
@@ -356,7 +356,10 @@ extern "C" pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint
 
 extern "C" pm_status pm_grid_destroy(pm_grid* grid) {
     if (!grid) return PM_OK;
-    if (grid->mem) (void)hipFree(grid->mem);
+    // the block goes back to the building context for its later builds (in
+    // its stream's order); past the cache's size, or with the context gone,
+    // it is freed
+    if (grid->mem && !pm::block_give(grid->owner, grid->mem, grid->mem_bytes)) (void)hipFree(grid->mem);
     delete grid;
     return PM_OK;
 }

@@ -51,6 +51,10 @@ struct pm_ctx {
     // Frame state of pm_render_frame (hit slots and query block survive the
     // gathers, which use `scratch`).
     pm_frame* frame_state = nullptr;
+    // Device blocks of destroyed grids, reused by later builds on this
+    // context (in stream order) instead of hipMalloc / hipFree per grid.
+    std::vector<std::pair<void*, size_t>> block_cache;
+    size_t block_cache_bytes = 0;
     // Gather tunings measured by pm_tune_gather on this context, and the
     // last tuning's table of measurements (cell sizes x variants, ms).
     std::vector<pm::TuneEntry> tunes;
@@ -60,7 +64,9 @@ struct pm_ctx {
 struct pm_grid {
     pm_grid_desc desc{};
     void* mem = nullptr;  // one allocation backing starts/sizes/photons/perm
+    size_t mem_bytes = 0;
     uint32_t n_cap = 0;   // photons the allocation holds (pm_grid_rebuild)
+    pm_ctx* owner = nullptr;  // the context whose block cache takes `mem` back
 };
 
 struct pm_kdtree {
@@ -158,6 +164,15 @@ pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint
                                 const uint32_t* n_dev = nullptr);
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
+
+// Device blocks for grids: a cached block of this context of at least
+// `bytes` (at most twice as large), else a fresh hipMalloc; *got = its size.
+// A returned block may still be read by work enqueued earlier on this
+// context's stream; work enqueued later is ordered after it.
+hipError_t block_take(pm_ctx* ctx, size_t bytes, void** p, size_t* got);
+// Give a block back to its context's cache (false when the context is gone
+// or the cache is full: the caller frees it).
+bool block_give(pm_ctx* owner, void* p, size_t bytes);
 
 // Planned photon traces (trace.hip): a fixed schedule of `paths` paths in
 // batches, enqueued without host synchronisation; the device fill counts

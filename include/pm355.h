@@ -131,6 +131,11 @@ pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out);
 /* PhotonGrid::copy_to_cpu (PhotonGrid.cuh:75-98). Any output may be NULL. */
 pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint32_t* h_starts, uint32_t* h_sizes,
                                float* h_photons9, uint32_t* h_perm);
+/* The grid's device memory goes back to the context that built it, whose
+ * later grid builds reuse it (ordered on that context's stream after work
+ * already enqueued there; up to 4 GB per context, the rest is freed). A grid
+ * read by work on another stream must have that work completed (or the
+ * stream synchronised) before it is destroyed. */
 pm_status pm_grid_destroy(pm_grid* grid);
 
 /* ---- k-nearest-photon gather: PhotonGrid::find_nearests +
