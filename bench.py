@@ -929,7 +929,8 @@ def main():
             "frac": achieved / profiling.L2_PEAK_GBS,
             "traffic": traffic,
             "kernel": {"stream512_chunked": "k_knn_stream512", "stream512": "k_knn_stream512",
-                       "stream1024": "k_knn_stream1024", "two_pass": "k_knn_two_pass"}.get(vname, vname)
+                       "inner": "k_knn_stream512", "stream1024": "k_knn_stream1024",
+                       "two_pass": "k_knn_two_pass"}.get(vname, vname)
                       + f" (variant {vname}, cell {frac:.3f}r; fused gather + radiance, cell-ordered queries)",
             "kernel_ms": gather_ms,
             "l2_read_bytes": l2_bytes,
@@ -941,6 +942,11 @@ def main():
             "l1_hit_rate": cs.get("l1_hit_rate") if cs else None,
             "l2_read_latency_cycles": cs.get("l2_read_latency_cycles") if cs else None,
             "counters_stale": cs["stale"] if cs else None,
+            # VALU issue: the profiled launch's wave64 VALU instructions at 4
+            # cycles each (16 lanes per SIMD) over this launch's SIMD cycles
+            # (1024 SIMDs at 2.4 GHz)
+            "valu_issue_frac": (cs["per_dispatch_median"]["SQ_INSTS_VALU"] * 4 / (t_s * 2.4e9 * 1024)
+                                if cs and "SQ_INSTS_VALU" in cs.get("per_dispatch_median", {}) else None),
             "walked_bytes": walked_bytes,
             "alg_bytes": alg_bytes,
             "alg_bytes_definition": "sum over queries of 12 + 8 C_q + 12 P_q + 32 kept + 24 (reference box at "
