@@ -1041,6 +1041,7 @@ def main():
     if args.frame_scenes > 0:
         log(f"batched frames: {args.frame_scenes}-scene frame mix")
         res["batched_frames"] = frames_batch_bench(ctx, args.frame_scenes)
+        ctx.trim()  # the mix's grid and frame blocks
     if args.serve_jobs > 0:
         log(f"served form: {args.serve_jobs} jobs per arrival rate")
         res["served"] = {"workload": "Poisson arrivals of render jobs: 6 resident scenes of the batched distribution "

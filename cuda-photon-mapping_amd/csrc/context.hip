@@ -65,7 +65,6 @@ namespace {
 // block caches, under one lock
 std::mutex g_ctx_mu;
 std::unordered_set<pm_ctx*> g_ctx_live;
-constexpr size_t kBlockCacheCap = (size_t)4 << 30;
 }  // namespace
 
 hipError_t block_take(pm_ctx* ctx, size_t bytes, void** p, size_t* got) {
@@ -85,13 +84,32 @@ hipError_t block_take(pm_ctx* ctx, size_t bytes, void** p, size_t* got) {
         }
     }
     *got = bytes;
-    return hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && ctx->block_cache_bytes > 0) {
+        // the cache holds what the device lacks: free it and try again
+        (void)hipGetLastError();
+        if (cache_release(ctx) == hipSuccess) e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
+hipError_t cache_release(pm_ctx* ctx) {
+    std::vector<std::pair<void*, size_t>> drop;
+    {
+        std::lock_guard<std::mutex> lock(g_ctx_mu);
+        drop.swap(ctx->block_cache);
+        ctx->block_cache_bytes = 0;
+    }
+    if (drop.empty()) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    for (auto& b : drop) (void)hipFree(b.first);
+    return e;
 }
 
 bool block_give(pm_ctx* owner, void* p, size_t bytes) {
     if (!owner || !p) return false;
     std::lock_guard<std::mutex> lock(g_ctx_mu);
-    if (!g_ctx_live.count(owner) || owner->block_cache_bytes + bytes > kBlockCacheCap) return false;
+    if (!g_ctx_live.count(owner) || owner->block_cache_bytes + bytes > owner->block_cache_cap) return false;
     owner->block_cache.emplace_back(p, bytes);
     owner->block_cache_bytes += bytes;
     return true;
@@ -124,12 +142,21 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out) {
     c->device = device;
     c->stream = (hipStream_t)hip_stream;
     c->cu_count = prop.multiProcessorCount;
-    const uint32_t first_epoch = 1u;  // tile counter 0, first look-back epoch 1
-    if (hipMalloc(&c->scan_counter, 256) != hipSuccess || hipMemset(c->scan_counter, 0, 256) != hipSuccess ||
-        hipMemcpy(c->scan_counter + pm::kEpochWord, &first_epoch, 4, hipMemcpyHostToDevice) != hipSuccess) {
-        if (c->scan_counter) (void)hipFree(c->scan_counter);
+    c->block_cache_cap = prop.totalGlobalMem / 4;
+    // Tile counter 0, first look-back epoch 1, written by fill kernels on the
+    // context's own stream: the null stream (hipMemset / hipMemcpy) is not
+    // ordered with a non-blocking stream, so a first scan could otherwise
+    // start before the words land.
+    if (hipMalloc(&c->scan_counter, 256) != hipSuccess) {
+        c->scan_counter = nullptr;
         delete c;
         return PM_ERR_NOMEM;
+    }
+    if (pm::fill_u32(c, c->scan_counter, 0u, 64) != PM_OK ||
+        pm::fill_u32(c, c->scan_counter + pm::kEpochWord, 1u, 1) != PM_OK) {
+        (void)hipFree(c->scan_counter);
+        delete c;
+        return PM_ERR_HIP;
     }
     // (a failure leaves only pm_grid_gather_radiance_noalloc unavailable,
     // which reports it)
@@ -166,6 +193,12 @@ pm_status pm_ctx_set_stream(pm_ctx* ctx, void* hip_stream) {
     if (!ctx) return PM_ERR_INVALID;
     PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->stream = (hipStream_t)hip_stream;
+    return PM_OK;
+}
+
+pm_status pm_ctx_trim(pm_ctx* ctx) {
+    if (!ctx) return PM_ERR_INVALID;
+    PM_HIP(ctx, pm::cache_release(ctx));
     return PM_OK;
 }
 

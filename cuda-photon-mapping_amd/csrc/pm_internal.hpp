@@ -55,6 +55,7 @@ struct pm_ctx {
     // context (in stream order) instead of hipMalloc / hipFree per grid.
     std::vector<std::pair<void*, size_t>> block_cache;
     size_t block_cache_bytes = 0;
+    size_t block_cache_cap = 0;  // a quarter of the device's memory
     // Gather tunings measured by pm_tune_gather on this context, and the
     // last tuning's table of measurements (cell sizes x variants, ms).
     std::vector<pm::TuneEntry> tunes;
@@ -165,7 +166,7 @@ pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
 
-// Device blocks for grids: a cached block of this context of at least
+// Device blocks for grids and frames: a cached block of this context of at least
 // `bytes` (at most twice as large), else a fresh hipMalloc; *got = its size.
 // A returned block may still be read by work enqueued earlier on this
 // context's stream; work enqueued later is ordered after it.
@@ -173,6 +174,8 @@ hipError_t block_take(pm_ctx* ctx, size_t bytes, void** p, size_t* got);
 // Give a block back to its context's cache (false when the context is gone
 // or the cache is full: the caller frees it).
 bool block_give(pm_ctx* owner, void* p, size_t bytes);
+// Free every cached block of the context (after its stream drains).
+hipError_t cache_release(pm_ctx* ctx);
 
 // Planned photon traces (trace.hip): a fixed schedule of `paths` paths in
 // batches, enqueued without host synchronisation; the device fill counts

@@ -285,6 +285,7 @@ __global__ __launch_bounds__(256) void k_render_combine(const DevModel* __restri
 // Per-frame device state: the ray trees' hit slots, the compacted query
 // block and the two radiance planes, in one allocation that grows as needed.
 struct pm_frame {
+    pm_ctx* owner = nullptr;  // the context whose block cache `mem` returns to
     void* mem = nullptr;
     size_t cap = 0;
     uint32_t W = 0, H = 0, npix = 0, nq = 0;
@@ -305,19 +306,27 @@ struct pm_frame {
 namespace {
 
 // Grow the frame's allocation to `need` bytes, keeping its first `keep`.
+// Blocks come from and go back to the context's block cache: a frame's old
+// block is reused by later work on the same stream, so only a frame that
+// moves to another context synchronises and frees it.
 pm_status frame_grow(pm_ctx* ctx, pm_frame* F, size_t need, size_t keep) {
     if (need <= F->cap) return PM_OK;
     void* fresh = nullptr;
+    size_t got = 0;
     const size_t want = need + need / 8;
-    hipError_t e = hipMalloc(&fresh, want);
+    hipError_t e = pm::block_take(ctx, want, &fresh, &got);
     if (e != hipSuccess) return pm::hip_fail(ctx, e, "hipMalloc(frame)");
     if (F->mem) {
         if (keep) PM_HIP(ctx, hipMemcpyAsync(fresh, F->mem, keep, hipMemcpyDeviceToDevice, ctx->stream));
-        PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        PM_HIP(ctx, hipFree(F->mem));
+        if (F->owner != ctx || !pm::block_give(ctx, F->mem, F->cap)) {
+            // (the frame's last context may be gone: wait for the device)
+            PM_HIP(ctx, F->owner == ctx ? hipStreamSynchronize(ctx->stream) : hipDeviceSynchronize());
+            PM_HIP(ctx, hipFree(F->mem));
+        }
     }
     F->mem = fresh;
-    F->cap = want;
+    F->cap = got;
+    F->owner = ctx;
     return PM_OK;
 }
 
@@ -352,7 +361,7 @@ extern "C" pm_status pm_frame_create(pm_ctx* ctx, pm_frame** out) {
 
 extern "C" pm_status pm_frame_destroy(pm_frame* F) {
     if (!F) return PM_OK;
-    if (F->mem) (void)hipFree(F->mem);
+    if (F->mem && !pm::block_give(F->owner, F->mem, F->cap)) (void)hipFree(F->mem);
     delete F;
     return PM_OK;
 }

@@ -277,8 +277,13 @@ pm_status lookback_begin(pm_ctx* ctx, uint32_t tiles, uint64_t** state, uint32_t
         ctx->scan_state = nullptr;
         size_t want = (size_t)tiles + tiles / 2 + 64;
         PM_HIP(ctx, hipMalloc(&ctx->scan_state, want * sizeof(uint64_t)));
-        PM_HIP(ctx, hipMemset(ctx->scan_state, 0, want * sizeof(uint64_t)));
         ctx->scan_state_tiles = want;
+        // zeroed on the context's stream, ahead of the scan that uses it
+        // (a hipMemset runs on the null stream, which a non-blocking stream
+        // does not wait for: a tile word cleared after its tile published
+        // would leave the next tile waiting for good)
+        pm_status st = fill_u32(ctx, reinterpret_cast<uint32_t*>(ctx->scan_state), 0u, 2 * want);
+        if (st != PM_OK) return st;
     }
     *state = ctx->scan_state;
     *counter = ctx->scan_counter;

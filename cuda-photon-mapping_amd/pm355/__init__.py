@@ -74,6 +74,7 @@ _SIGS = {
     "pm_ctx_destroy": (_I32, [_P]),
     "pm_ctx_set_stream": (_I32, [_P, _P]),
     "pm_ctx_sync": (_I32, [_P]),
+    "pm_ctx_trim": (_I32, [_P]),
     "pm_last_error": (C.c_char_p, [_P]),
     "pm_scan_u32": (_I32, [_P, _P, _P, _U32, _I32, _P]),
     "pm_grid_build": (_I32, [_P, _P, _U32, C.POINTER(_F32), _F32, _I32, _I32, C.POINTER(_P)]),
@@ -272,6 +273,11 @@ class Context:
 
     def sync(self):
         self.check(self.lib.pm_ctx_sync(self.h))
+
+    def trim(self):
+        """Free the device blocks that destroyed grids and frames left in
+        this context's cache (pm_ctx_trim)."""
+        self.check(self.lib.pm_ctx_trim(self.h))
 
     def close(self):
         if getattr(self, "h", None):

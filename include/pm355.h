@@ -86,6 +86,11 @@ pm_status pm_ctx_create(int32_t device, void* hip_stream, pm_ctx** out);
 pm_status pm_ctx_destroy(pm_ctx* ctx);
 pm_status pm_ctx_set_stream(pm_ctx* ctx, void* hip_stream);
 pm_status pm_ctx_sync(pm_ctx* ctx);
+/* Destroyed grids and frames leave their device blocks in their context's
+ * cache (up to a quarter of the device's memory) for later builds and frames
+ * of similar size; pm_ctx_trim synchronises the stream and frees them. (No
+ * counterpart in the reference, which frees per call.) */
+pm_status pm_ctx_trim(pm_ctx* ctx);
 const char* pm_last_error(const pm_ctx* ctx);
 
 /* ---- prefix scan: prescan<uint, inclusive> (PrefixScanSum.cuh:12-148) and
@@ -132,8 +137,9 @@ pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out);
 pm_status pm_grid_copy_to_host(pm_ctx* ctx, const pm_grid* grid, uint32_t* h_starts, uint32_t* h_sizes,
                                float* h_photons9, uint32_t* h_perm);
 /* The grid's device memory goes back to the context that built it, whose
- * later grid builds reuse it (ordered on that context's stream after work
- * already enqueued there; up to 4 GB per context, the rest is freed). A grid
+ * later grid builds and frames reuse it (ordered on that context's stream
+ * after work already enqueued there; up to a quarter of the device's memory
+ * per context, the rest is freed; pm_ctx_trim frees the cache). A grid
  * read by work on another stream must have that work completed (or the
  * stream synchronised) before it is destroyed. */
 pm_status pm_grid_destroy(pm_grid* grid);
@@ -401,7 +407,9 @@ pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const pm_grid* glo
  *                        with PM_RENDER_REFERENCE_CAUSTIC);
  *   (run the jobs, e.g. with pm_gather_batch, on the same stream)
  *   pm_frame_finish      per-pixel fold, HDR and uchar3, as pm_render_frame.
- * A pm_frame owns its device buffers and may be reused for the next frame. */
+ * A pm_frame owns its device buffers and may be reused for the next frame;
+ * pm_frame_destroy returns them to the cache of the context that last grew
+ * them (as pm_grid_destroy). */
 pm_status pm_frame_create(pm_ctx* ctx, pm_frame** out);
 pm_status pm_frame_destroy(pm_frame* frame);
 pm_status pm_frame_rays(pm_ctx* ctx, pm_frame* frame, const pm_scene* scene, const pm_camera* camera,

@@ -89,6 +89,42 @@ def test_batched_frames_equal_single_frames(ctx):
         assert np.array_equal(rgb8.cpu().numpy(), rgb81.cpu().numpy())
 
 
+def test_cached_blocks_reused_and_trimmed(ctx):
+    """Grids and frames destroyed on a context leave their device blocks in
+    its cache; builds and frames that reuse them (stale contents, smaller or
+    equal sizes) give the same frames, and so do they after pm_ctx_trim."""
+    from pm355.scenes import random_scene
+
+    scenes = []
+    for i, sc in enumerate([cornell_box(), random_scene(5)]):
+        ds = ctx.upload_scene(sc, accel=True)
+        g, c, _ = ctx.trace_photons(ds, 40 + i, 80_000, 8_000, 8, max_paths=3_000_000)
+        scenes.append((ds, g, c, sc.aabb(), (0.04, 0.03)[i], cornell_camera(128 + 64 * i, 96)))
+
+    def render_all():
+        frames, grids = [], []
+        for ds, g, c, aabb, r, cam in scenes:
+            gg = ctx.grid_build(g, aabb, cell_size=r)
+            gc = ctx.grid_build(c, aabb, cell_size=r)
+            grids += [gg, gc]
+            frames.append((ds, gg, gc, cam, 4, 48, r, 32, r))
+        out = [(rgb.cpu().numpy().view(np.uint32), rgb8.cpu().numpy(), hits)
+               for rgb, rgb8, hits in ctx.render_frames_batched(frames)]
+        for gr in grids:
+            gr.close()
+        return out
+
+    first = render_all()
+    for rep in range(2):
+        if rep == 1:
+            ctx.trim()
+        again = render_all()
+        for (a, a8, ah), (b, b8, bh) in zip(first, again):
+            assert ah == bh
+            assert np.array_equal(a, b), rep
+            assert np.array_equal(a8, b8), rep
+
+
 @pytest.mark.parametrize("seed", [101, 202, 303])
 def test_c2_full_frame_against_oracle(ctx, seed):
     """The per-frame configuration at full size: Cornell box with BVHs,
