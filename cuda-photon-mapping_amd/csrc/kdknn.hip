@@ -5,6 +5,18 @@
 // tests; this is a standard kd search with the same answer as the grid
 // gather: the k smallest (d2, photon index) among photons within r.
 //
+// Each query first searches the ball of 0.4 r (0.16 of the squared
+// threshold): when that ball holds at least k photons, they are the k
+// nearest overall (every photon outside it is farther than every photon
+// inside), and the query is done having visited a fraction of the nodes.
+// Otherwise the count c it found sizes a second ball, of squared radius
+// 1.5 (k / c) times the first (the growth of a surface density), searched
+// from an empty buffer when it is below 0.7 of the full threshold, and the
+// full ball follows if that one also holds fewer than k. The last search is
+// always the full one, so the kept set is that of the full search. Within a
+// search, a subtree is also skipped once the key buffer's acceptance bound
+// (the narrowed bin edge of the selection) lies below its plane distance.
+//
 // One wave64 per query, walking the tree with one LIFO stack of node ids in
 // LDS shared by the wave: each step the 64 lanes pop the top 64 entries,
 // visit one node each and push the children whose half-spaces meet the
@@ -37,7 +49,8 @@ struct KdParams {
     const float* qz;
     uint32_t nq;
     float T;
-    float prune;  // plane-distance^2 bound: T * (1 + 1e-5)
+    float prune;     // plane-distance^2 bound: T * (1 + 1e-5)
+    float T_in;  // the first search's threshold, 0.16 T
     uint32_t bT;
     int k;
     uint32_t* out_idx;
@@ -68,8 +81,8 @@ __device__ __forceinline__ float kd_dist2(float px, float py, float pz, float qx
 }
 
 // Visit node: its photon's key and which children may hold photons within r.
-__device__ __forceinline__ void visit(const KdParams& P, uint32_t node, float q0, float q1, float q2, bool& valid,
-                                      uint32_t& pid, float& d2, bool& go_l, bool& go_r) {
+__device__ __forceinline__ void visit(const KdParams& P, float prune, uint32_t node, float q0, float q1, float q2,
+                                      bool& valid, uint32_t& pid, float& d2, bool& go_l, bool& go_r) {
     valid = false;
     go_l = go_r = false;
     if (node >= P.cap) return;
@@ -84,8 +97,8 @@ __device__ __forceinline__ void visit(const KdParams& P, uint32_t node, float q0
     const float diff = (p == 0 ? q0 : p == 1 ? q1 : q2) - (p == 0 ? x : p == 1 ? y : z);
     const float dd = diff * diff;
     // left subtree: coordinate <= median; right: >= median
-    go_l = !(diff > 0.f && dd > P.prune);
-    go_r = !(diff < 0.f && dd > P.prune);
+    go_l = !(diff > 0.f && dd > prune);
+    go_r = !(diff < 0.f && dd > prune);
     go_l = go_l && 2 * node + 1 < P.cap;
     go_r = go_r && 2 * node + 2 < P.cap;
 }
@@ -98,10 +111,10 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
     if (q >= P.nq) return;  // wave-uniform
     KdLds<kCap>& L = lds[wave];
     const float q0 = P.qx[q], q1 = P.qy[q], q2 = P.qz[q];
-    const float T = P.T;
     const uint32_t k = (uint32_t)P.k;
     uint32_t cnt = 0, found = 0;
     uint64_t khi = ~0ull;
+    float T = P.T_in;
 
     auto feed = [&](bool valid, uint32_t pid, float d2) {
         bool in = valid && d2 <= T;
@@ -121,29 +134,54 @@ __global__ __launch_bounds__(256) void k_kd_knn(KdParams P) {
         cnt += na;
     };
 
-    uint32_t top = 1;  // (wave-uniform)
-    if (lane == 0) L.stk[0] = 0;
-    wave_sync();
-    while (top > 0) {
-        const uint32_t n = kStackCap - top >= 128 ? (top < 64 ? top : 64u) : 1u;
-        uint32_t node = P.cap;
-        if (lane < n) node = L.stk[top - n + lane];
-        wave_sync();  // every pop is read before the pushes reuse the slots
-        top -= n;
-        bool valid, gl, gr;
-        uint32_t pid = 0;
-        float d2 = 0.f;
-        visit(P, node, q0, q1, q2, valid, pid, d2, gl, gr);
-        feed(valid, pid, d2);
-        // right children below the left ones: left subtrees are popped first
-        const uint64_t br = wballot(gr), bl = wballot(gl);
-        const uint32_t nr = (uint32_t)__popcll(br);
-        if (gr) L.stk[top + mbcnt(br)] = 2 * node + 2;
-        if (gl) L.stk[top + nr + mbcnt(bl)] = 2 * node + 1;
-        top = __builtin_amdgcn_readfirstlane(top + nr + (uint32_t)__popcll(bl));
+    for (int pass = 0;; ++pass) {
+        if (pass > 0) {
+            // (wave-uniform) the last ball holds the k nearest, or it was the full one
+            if (found >= k || !(T < P.T)) break;
+            float next = P.T;
+            if (pass == 1) {
+                const float est = T * 1.5f * (float)k / (float)(found > 0 ? found : 1u);
+                if (est < 0.7f * P.T) next = est;
+            }
+            T = next;
+            cnt = 0;
+            found = 0;
+            khi = ~0ull;
+        }
+        const float prune0 = T < P.T ? T * (1.0f + 1e-5f) + 1e-30f : P.prune;
+        float prune = prune0;
+        uint64_t khi_seen = khi;
+        uint32_t top = 1;  // (wave-uniform)
+        if (lane == 0) L.stk[0] = 0;
+        wave_sync();
+        while (top > 0) {
+            const uint32_t n = kStackCap - top >= 128 ? (top < 64 ? top : 64u) : 1u;
+            uint32_t node = P.cap;
+            if (lane < n) node = L.stk[top - n + lane];
+            wave_sync();  // every pop is read before the pushes reuse the slots
+            top -= n;
+            bool valid, gl, gr;
+            uint32_t pid = 0;
+            float d2 = 0.f;
+            visit(P, prune, node, q0, q1, q2, valid, pid, d2, gl, gr);
+            feed(valid, pid, d2);
+            if (khi != khi_seen) {
+                // keys at or above khi are never kept: a photon is only
+                // worth visiting when its d2 is at most khi's d2 bits
+                khi_seen = khi;
+                const float b = __uint_as_float((uint32_t)(khi >> 32)) * (1.0f + 1e-5f);
+                prune = b < prune0 ? b : prune0;
+            }
+            // right children below the left ones: left subtrees are popped first
+            const uint64_t br = wballot(gr), bl = wballot(gl);
+            const uint32_t nr = (uint32_t)__popcll(br);
+            if (gr) L.stk[top + mbcnt(br)] = 2 * node + 2;
+            if (gl) L.stk[top + nr + mbcnt(bl)] = 2 * node + 1;
+            top = __builtin_amdgcn_readfirstlane(top + nr + (uint32_t)__popcll(bl));
+            wave_sync();
+        }
         wave_sync();
     }
-    wave_sync();
 
     const uint32_t m = found < k ? found : k;
     cnt = buf_select<kCap>(L.sel, lane, cnt, k, 256u, P.bT, khi);
@@ -186,6 +224,7 @@ extern "C" pm_status pm_kdtree_knn(pm_ctx* ctx, const pm_kdtree* tree, const flo
     P.nq = nq;
     P.T = radius_threshold(radius);
     P.prune = P.T * (1.0f + 1e-5f) + 1e-30f;
+    P.T_in = P.T * 0.16f;
     uint32_t tb;
     std::memcpy(&tb, &P.T, 4);
     P.bT = tb >> 22;

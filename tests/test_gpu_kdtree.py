@@ -83,7 +83,8 @@ def test_kdtree_knn_reference_fixture(ctx):
 
 
 @pytest.mark.parametrize("n,nq,r,k", [(1, 5, 2.0, 3), (9, 40, 0.5, 4), (5000, 700, 0.05, 16),
-                                      (60000, 2000, 0.03, 64), (60000, 500, 0.08, 256), (200000, 3000, 0.01, 100)])
+                                      (60000, 2000, 0.03, 64), (60000, 500, 0.08, 256), (200000, 3000, 0.01, 100),
+                                      (400000, 500, 0.08, 256)])
 def test_kdtree_knn_matches_oracle_and_brute(ctx, n, nq, r, k):
     rng = np.random.default_rng(n + k)
     pos = rng.random((3, n), dtype=np.float32)
@@ -105,11 +106,11 @@ def test_kdtree_knn_ties(ctx):
 
 
 def test_kdtree_knn_deep_stack(ctx):
-    """A 21-level tree (2^20 photons) searched with r = 0.1: the subtree
-    roots of the breadth-first phase sit at depth <= 7, and each lane's
-    depth-first stack holds one pending sibling per level below its root, so
-    it grows past the 12 entries kept in LDS into the per-lane spill array.
-    Results equal the oracle's kd search and brute force."""
+    """A 21-level tree (2^20 photons) searched with r = 0.1 (about 4,200
+    photons within r per query, 520 within the inner half-radius ball): the
+    wave's node stack holds hundreds of pending subtrees, and the key buffer
+    narrows many times, its bound then pruning subtrees. Results equal the
+    oracle's kd search and brute force."""
     rng = np.random.default_rng(77)
     pos = rng.random((3, 1 << 20), dtype=np.float32)
     q = rng.random((3, 300), dtype=np.float32)
