@@ -239,6 +239,20 @@ def test_query_order_is_stable_cell_sort(ctx):
     assert np.array_equal(order, np.argsort(cid, kind="stable"))
 
 
+@pytest.mark.parametrize("nq", [4_194_303, 4_194_304, 5_000_011])
+def test_query_order_large_is_stable_cell_sort(ctx, nq):
+    """The cell sort of millions of queries (around 2^22 keys, where the
+    scans switch to the two-level form) is numpy's stable argsort."""
+    ph, q, qn, aabb = gather_microbench(seed=7, n_photons=200_000, n_queries=nq, n_clusters=4)
+    g = ctx.grid_build(ph, aabb, cell_size=0.004)
+    order = _np(ctx.query_order(g, q), np.uint32)
+    og = O.OracleGrid(ph, aabb, cell_size=0.004)
+    from test_oracle import _numpy_cell_ids
+
+    cid = _numpy_cell_ids(q, og.aabb, og.cell, og.dims)
+    assert np.array_equal(order, np.argsort(cid, kind="stable").astype(np.uint32))
+
+
 @pytest.mark.parametrize("layout,cell", [(0, 0.01), (0, 0.005), (1, 0.005)])
 def test_box_stats_bracket_the_oracle(ctx, layout, cell):
     """The gather visits the reference's candidate box (PhotonGrid.cu:262-272)
@@ -285,6 +299,13 @@ def test_full_size_gather_properties(ctx):
     assert np.all(_np(gi, np.uint32)[~kept] == 0xFFFFFFFF)
     # oracle parity on 3000 sampled queries (oracle grid built independently)
     og = O.OracleGrid(ph, aabb, cell_size=0.01)
+    # the stable cell order of all 16M photons equals numpy's stable argsort
+    # of the cell ids
+    from test_oracle import _numpy_cell_ids
+
+    cid = _numpy_cell_ids(ph[0:3], og.aabb, og.cell, og.dims)
+    assert np.array_equal(perm, np.argsort(cid, kind="stable").astype(np.uint32))
+    del cid
     sel = np.random.default_rng(0).choice(q.shape[1], 3000, replace=False)
     oi, od, oc = og.knn(q[:, sel], 0.01, 64)
     gi_h = _np(gi, np.uint32)[sel]
