@@ -241,8 +241,8 @@ def test_query_order_is_stable_cell_sort(ctx):
 
 @pytest.mark.parametrize("nq", [4_194_303, 4_194_304, 5_000_011])
 def test_query_order_large_is_stable_cell_sort(ctx, nq):
-    """The cell sort of millions of queries (around 2^22 keys, where the
-    scans switch to the two-level form) is numpy's stable argsort."""
+    """The cell sort of millions of queries (over a thousand 4096-key radix
+    tiles per pass, full and partial last tiles) is numpy's stable argsort."""
     ph, q, qn, aabb = gather_microbench(seed=7, n_photons=200_000, n_queries=nq, n_clusters=4)
     g = ctx.grid_build(ph, aabb, cell_size=0.004)
     order = _np(ctx.query_order(g, q), np.uint32)
