@@ -19,13 +19,6 @@
 
 #include "pm_internal.hpp"
 
-// Switched off: pm_pipeline_create has been seen to block (its first frames
-// never completed on the GPU) and no cause has been found in the code yet;
-// until one is, the entry point reports PM_ERR_UNSUPPORTED. The eager calls
-// it composes (pm_trace_photons, pm_grid_build / pm_grid_rebuild,
-// pm_render_frame) are unaffected.
-constexpr bool kPipelineEnabled = false;
-
 struct pm_pipeline {
     pm_ctx* ctx = nullptr;
     pm_pipeline_desc d{};
@@ -54,10 +47,6 @@ extern "C" pm_status pm_pipeline_destroy(pm_pipeline* pl) {
 extern "C" pm_status pm_pipeline_create(pm_ctx* ctx, const pm_pipeline_desc* desc, pm_pipeline** out) {
     if (!ctx || !desc || !out) return PM_ERR_INVALID;
     *out = nullptr;
-    if (!kPipelineEnabled)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED,
-                        "pm_pipeline_create: disabled: a pipeline's creation has been seen not to finish on the GPU "
-                        "and the cause is not yet known");
     const pm_pipeline_desc& d = *desc;
     if (!d.scene || d.n_global == 0) return pm::fail(ctx, PM_ERR_INVALID, "pm_pipeline_create: incomplete description");
     if (!(d.cell_global > 0.f) || (d.n_caustic > 0 && !(d.cell_caustic > 0.f)))

@@ -132,20 +132,11 @@ void drop_workers(pm_server* S) {
 
 }  // namespace
 
-// Switched off: a served run with four worker contexts stopped making
-// progress on the GPU and no cause has been found in the code yet; until one
-// is, asking for workers reports PM_ERR_UNSUPPORTED and every stage runs on
-// the server's context.
-constexpr bool kWorkersEnabled = false;
 
 extern "C" pm_status pm_server_set_workers(pm_server* S, int32_t n_workers) {
     if (!S) return PM_ERR_INVALID;
     pm_ctx* ctx = S->ctx;
     if (n_workers < 0 || n_workers > 64) return pm::fail(ctx, PM_ERR_INVALID, "pm_server_set_workers: 0..64 workers");
-    if (n_workers > 0 && !kWorkersEnabled)
-        return pm::fail(ctx, PM_ERR_UNSUPPORTED,
-                        "pm_server_set_workers: disabled: a served run with worker contexts was seen not to "
-                        "finish on the GPU and the cause is not yet known");
     PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     drop_workers(S);
     for (int32_t w = 0; w < n_workers; ++w) {

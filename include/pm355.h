@@ -440,10 +440,7 @@ pm_status pm_frame_copy_queries(pm_ctx* ctx, const pm_frame* frame, float* d_que
  * same seed whenever the scheduled paths fill both maps, which
  * pm_pipeline_status reports (it synchronises the stream). One pipeline per
  * context at a time: frames share the context's scratch. The
- * reference-caustic mode is not supported.
- * Currently switched off: pm_pipeline_create returns PM_ERR_UNSUPPORTED
- * (a creation has been seen not to finish on the GPU; the cause is not yet
- * known). */
+ * reference-caustic mode is not supported. */
 typedef struct {
     const pm_scene* scene;
     float aabb[6];                     /* grid bounds */
@@ -510,10 +507,7 @@ pm_status pm_server_step(pm_server* server, uint32_t* frames_done);
  * threads, each driving a context on a stream of its own (created by the
  * library at the default priority), and the shared gather launch on the
  * server's context waits on all of them. 0 (the default) runs them on the
- * server's context. Frames are identical either way.
- * Currently switched off: n_workers > 0 returns PM_ERR_UNSUPPORTED (a
- * served run with workers was seen not to finish on the GPU; the cause is
- * not yet known). */
+ * server's context. Frames are identical either way. */
 pm_status pm_server_set_workers(pm_server* server, int32_t n_workers);
 pm_status pm_server_stats(const pm_server* server, pm_server_info* out);
 /* per-frame latency in seconds (eligible -> completed), oldest first */

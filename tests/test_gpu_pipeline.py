@@ -12,11 +12,7 @@ import pytest
 
 from pm355.scenes import cornell_box, cornell_camera
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="pm_pipeline_create is switched off: a pipeline's creation was seen not to finish "
-                            "on the GPU and the cause has not been found"),
-]
+pytestmark = pytest.mark.gpu
 
 NG, NC, CELL, R, K = 200_000, 50_000, 0.02, 0.02, 64
 
