@@ -530,7 +530,10 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
     both grid builds, pm_render_frame (its own gather launch).
     batched: every scene's trace and grids, then every frame's eye rays, the
     gathers of all frames in one pm_gather_batch, and each frame's fold.
-    Both on one context and stream; the frames are identical."""
+    served: the scenes as one-frame jobs of the continuous-batching server
+    (one step renders them all), on its context alone or with three worker
+    contexts whose streams run the per-job trace / grid / eye-ray stages
+    concurrently. Every form renders the same frames (checked)."""
     import numpy as np
     import torch
 
@@ -586,9 +589,40 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
                 gc.close()
         return [r8 for _, r8, _ in res]
 
+    # The server's frame f of a job is seeded seed + (f + 1) * step: a
+    # one-frame job submitted with seed - step renders the plain frame.
+    seed_step = 0x9E3779B97F4A7C15
+
+    servers = []
+
+    def served(workers):
+        # one server (and its worker contexts, with their block caches) for
+        # every repetition of the form, as a serving process keeps them
+        def fn():
+            if not servers or servers[-1][0] != workers:
+                srv = ctx.server()
+                if workers:
+                    srv.set_workers(workers)
+                servers.append((workers, srv))
+            srv = servers[-1][1]
+            done0 = srv.info()["finished"]
+            outs = []
+            for j in jobs:
+                _, o = srv.submit(j["ds"], j["aabb"], j["r"], j["ng"], j["nc"], j["cam"], 1,
+                                  (j["seed"] - seed_step) & ((1 << 64) - 1), max_depth=4, k_global=64,
+                                  r_global=j["r"], k_caustic=64, r_caustic=j["r"],
+                                  max_paths=3 * (j["ng"] + j["nc"]))
+                outs.append(o)
+            while srv.info()["finished"] < done0 + len(jobs):
+                srv.step()
+            torch.cuda.synchronize()
+            return [o[0] for o in outs]
+        return fn
+
     out = {}
     first = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
+    for name, fn in (("plain", plain), ("batched", batched), ("served", served(0)),
+                     ("served_workers3", served(3))):
         log(f"frames mix: {name}")
         first[name] = fn()
         torch.cuda.synchronize()
@@ -601,7 +635,10 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
         out[name + "_photons"] = n_ph[0] // reps
         if name == "plain":
             out["queries"] = n_q[0] // reps
-    same = all(torch.equal(a, b) for a, b in zip(first["plain"], first["batched"]))
+    same = {name: all(torch.equal(a, b) for a, b in zip(first["plain"], first[name]))
+            for name in ("batched", "served", "served_workers3")}
+    for _, srv in servers:
+        srv.close()
     jobs.clear()
     return {
         "workload": f"{n_scenes} scenes of the batched distribution (unit box + 1-8 spheres of 200-20k triangles, "
@@ -610,17 +647,22 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
                     "frame per scene (trace, grids, eye rays, gathers, fold)",
         "plain_ms": out["plain"] * 1e3,
         "batched_ms": out["batched"] * 1e3,
+        "served_ms": out["served"] * 1e3,
+        "served_workers3_ms": out["served_workers3"] * 1e3,
         "speedup": out["plain"] / out["batched"],
+        "speedup_served_workers3": out["plain"] / out["served_workers3"],
         "photons_traced": out["plain_photons"],
         "gather_queries": out["queries"],
-        "frames_identical": bool(same),
-        "note": "batched = all traces and grids, one pm_gather_batch for every frame's gathers; the per-frame "
-                "graph-captured pipeline that would also batch the traces is switched off",
+        "frames_identical": same,
+        "note": "batched = every scene's trace and grids, then one pm_gather_batch for every frame's gathers; "
+                "served = all 256 scenes as one-frame jobs of pm_server (one step: per-job trace, grids and eye "
+                "rays, one shared gather batch, per-frame fold), on the server's context or with three worker "
+                "contexts running the per-job stages on their own streams",
     }
 
 
 def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
-                configs=(("default", 0, 0), ("tuned", 0, 1))):
+                configs=(("default", 0, 0), ("tuned", 0, 1), ("tuned_workers3", 3, 1))):
     """The served form: Poisson arrivals of render jobs (scenes of the batched
     distribution, resident with their BVHs; 50k-8M photons, 5% caustic;
     frames of 128x128 up to 1920x1080; 1-30 frames each; k = 64,

@@ -14,13 +14,15 @@ using pmd::GridGeom;
 // counting atomics: counts come from the sorted keys). The same pass copies
 // each photon into a 48-byte record (three float4), so the scatter below
 // gathers one record per photon instead of nine scattered 4-byte values.
-__global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9, uint32_t n, GridGeom G,
-                                                  uint32_t* __restrict__ keys, float4* __restrict__ rec) {
+// in9: nine planes of `stride` floats (stride >= n: a photon map that holds
+// fewer photons than its capacity)
+__global__ __launch_bounds__(256) void k_grid_keys(const float* __restrict__ in9, uint32_t n, size_t stride,
+                                                  GridGeom G, uint32_t* __restrict__ keys, float4* __restrict__ rec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float v[9];
 #pragma unroll
-    for (int p = 0; p < 9; ++p) v[p] = in9[(size_t)p * n + i];
+    for (int p = 0; p < 9; ++p) v[p] = in9[(size_t)p * stride + i];
     int x = pmd::cell_coord(v[0], G.mn[0], G.cell, G.g[0]);
     int y = pmd::cell_coord(v[1], G.mn[1], G.cell, G.g[1]);
     int z = pmd::cell_coord(v[2], G.mn[2], G.cell, G.g[2]);
@@ -195,7 +197,8 @@ size_t build_scratch_bytes(uint32_t n) {
 // Enqueue the build of n photons at dims dd into g's memory (capacity
 // g->n_cap >= n): keys + records, stable key sort, cell sizes from the sorted
 // runs, starts by exclusive scan, record gather. No host synchronisation.
-pm_status enqueue_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6], const HostDims& dd,
+pm_status enqueue_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, size_t stride, const float aabb[6],
+                        const HostDims& dd,
                         int layout, pm_grid* g) {
     pm::Carve gc{static_cast<char*>(g->mem)};
     uint32_t* starts = gc.take<uint32_t>(dd.n_cells);
@@ -229,7 +232,7 @@ pm_status enqueue_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const 
     float4* rec = cv.take<float4>((size_t)n * 3);
     const uint32_t blocks = pm::div_up(n, 256);
     const GridGeom G = geom_of(aabb, dd, layout);
-    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, G, keys, rec);
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks), dim3(256), 0, ctx->stream, d_photons9, n, stride, G, keys, rec);
     PM_CHECK_LAUNCH(ctx, "k_grid_keys");
     if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, bits_for(dd.n_cells), rwork)) != PM_OK)
         return st;
@@ -244,7 +247,9 @@ pm_status enqueue_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const 
 
 }  // namespace
 
-extern "C" pm_status pm_grid_rebuild(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n) {
+namespace pm {
+
+pm_status grid_rebuild_strided(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n, size_t stride) {
     if (!ctx || !grid) return PM_ERR_INVALID;
     if (n > 0 && !d_photons9) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_rebuild: null photons");
     if (n > grid->n_cap)
@@ -256,11 +261,11 @@ extern "C" pm_status pm_grid_rebuild(pm_ctx* ctx, pm_grid* grid, const float* d_
     dd.n_cells = grid->desc.n_cells;
     float aabb[6];
     for (int a = 0; a < 6; ++a) aabb[a] = grid->desc.aabb[a];
-    return enqueue_build(ctx, d_photons9, n, aabb, dd, grid->desc.layout, grid);
+    return enqueue_build(ctx, d_photons9, n, stride, aabb, dd, grid->desc.layout, grid);
 }
 
-extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
-                                   float cell_size, int32_t desired, int32_t layout, pm_grid** out) {
+pm_status grid_build_strided(pm_ctx* ctx, const float* d_photons9, uint32_t n, size_t stride, const float aabb[6],
+                             float cell_size, int32_t desired, int32_t layout, pm_grid** out) {
     if (!ctx || !out) return PM_ERR_INVALID;
     *out = nullptr;
     if (!aabb || (n > 0 && !d_photons9)) return pm::fail(ctx, PM_ERR_INVALID, "pm_grid_build: null argument");
@@ -296,7 +301,7 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
         }
         g->owner = ctx;
         g->n_cap = n;
-        pm_status bs = enqueue_build(ctx, d_photons9, n, aabb, dd, layout, g);
+        pm_status bs = enqueue_build(ctx, d_photons9, n, stride, aabb, dd, layout, g);
         if (bs != PM_OK) {
             pm_grid_destroy(g);
             return bs;
@@ -330,6 +335,17 @@ extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_
     }
     *out = g;
     return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" pm_status pm_grid_rebuild(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n) {
+    return pm::grid_rebuild_strided(ctx, grid, d_photons9, n, n);
+}
+
+extern "C" pm_status pm_grid_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, const float aabb[6],
+                                   float cell_size, int32_t desired, int32_t layout, pm_grid** out) {
+    return pm::grid_build_strided(ctx, d_photons9, n, n, aabb, cell_size, desired, layout, out);
 }
 
 extern "C" pm_status pm_grid_describe(const pm_grid* grid, pm_grid_desc* out) {

@@ -177,6 +177,13 @@ bool block_give(pm_ctx* owner, void* p, size_t bytes);
 // Free every cached block of the context (after its stream drains).
 hipError_t cache_release(pm_ctx* ctx);
 
+// pm_grid_build / pm_grid_rebuild over the first n photons of a photons9
+// block whose planes are `stride` floats apart (a trace's output map holds
+// its count of photons at the plane stride of its capacity).
+pm_status grid_build_strided(pm_ctx* ctx, const float* d_photons9, uint32_t n, size_t stride, const float aabb[6],
+                             float cell_size, int32_t desired, int32_t layout, pm_grid** out);
+pm_status grid_rebuild_strided(pm_ctx* ctx, pm_grid* grid, const float* d_photons9, uint32_t n, size_t stride);
+
 // Planned photon traces (trace.hip): a fixed schedule of `paths` paths in
 // batches, enqueued without host synchronisation; the device fill counts
 // (u64 global, caustic) tell afterwards whether the maps filled.

@@ -37,8 +37,9 @@ struct Job {
     size_t footprint = 0;
     uint32_t done = 0;
     double ready_at = 0;  // when the next frame became eligible
-    float* g9 = nullptr;  // photon maps (device)
+    float* g9 = nullptr;  // photon maps (device; blocks of the server context's cache)
     float* c9 = nullptr;
+    size_t g9_bytes = 0, c9_bytes = 0;
     pm_frame* frame = nullptr;
     pm_grid* gg = nullptr;
     pm_grid* gc = nullptr;
@@ -77,16 +78,25 @@ void release(pm_server* S, Job& j) {
     j.gg = j.gc = nullptr;
     if (j.frame) pm_frame_destroy(j.frame);
     j.frame = nullptr;
-    if (j.g9) (void)hipFree(j.g9);
-    if (j.c9) (void)hipFree(j.c9);
+    // (the server's later jobs reuse the maps' blocks: their stages are
+    // ordered after this job's on the server's stream, which every worker
+    // stream waits for at the start of a step)
+    if (j.g9 && !pm::block_give(S->ctx, j.g9, j.g9_bytes)) (void)hipFree(j.g9);
+    if (j.c9 && !pm::block_give(S->ctx, j.c9, j.c9_bytes)) (void)hipFree(j.c9);
     j.g9 = j.c9 = nullptr;
     S->in_use -= std::min(S->in_use, j.footprint);
 }
 
 pm_status admit(pm_server* S, Job& j) {
     pm_ctx* ctx = S->ctx;
-    hipError_t e = hipMalloc(&j.g9, (size_t)std::max(1u, j.d.n_global) * 36);
-    if (e == hipSuccess) e = hipMalloc(&j.c9, (size_t)std::max(1u, j.d.n_caustic) * 36);
+    void* p = nullptr;
+    hipError_t e = pm::block_take(ctx, (size_t)std::max(1u, j.d.n_global) * 36, &p, &j.g9_bytes);
+    j.g9 = e == hipSuccess ? static_cast<float*>(p) : nullptr;
+    if (e == hipSuccess) {
+        p = nullptr;
+        e = pm::block_take(ctx, (size_t)std::max(1u, j.d.n_caustic) * 36, &p, &j.c9_bytes);
+        j.c9 = e == hipSuccess ? static_cast<float*>(p) : nullptr;
+    }
     if (e != hipSuccess) {
         release(S, j);
         return pm::hip_fail(ctx, e, "pm_server: photon maps");
@@ -195,16 +205,18 @@ namespace {
 
 // Grid of this frame's n photons: rebuilt in place when the job's grid holds
 // them, else built afresh (cell: the job's, or its tuned size).
-pm_status frame_grid(pm_ctx* ctx, pm_grid*& g, const float* ph9, uint32_t n, const pm_job_desc& d, float r, int32_t k) {
+// The map holds n photons at the plane stride of its capacity.
+pm_status frame_grid(pm_ctx* ctx, pm_grid*& g, const float* ph9, uint32_t n, size_t stride, const pm_job_desc& d,
+                     float r, int32_t k) {
     const float cell =
         (d.params.gather_flags & PM_KNN_TUNED) ? pm::tuned_cell(ctx, n, d.aabb, r, k, d.cell_size) : d.cell_size;
-    if (g && g->desc.cell_size == cell && n <= g->n_cap) return pm_grid_rebuild(ctx, g, ph9, n);
+    if (g && g->desc.cell_size == cell && n <= g->n_cap) return pm::grid_rebuild_strided(ctx, g, ph9, n, stride);
     if (g) {
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         pm_grid_destroy(g);
         g = nullptr;
     }
-    return pm_grid_build(ctx, ph9, n, d.aabb, cell, 1000, PM_LAYOUT_LINEAR, &g);
+    return pm::grid_build_strided(ctx, ph9, n, stride, d.aabb, cell, 1000, PM_LAYOUT_LINEAR, &g);
 }
 
 // One job's stages before the gathers, on context `ctx`: its next frame's
@@ -217,9 +229,12 @@ pm_status prepare(pm_ctx* ctx, Job& j, pm_gather_job* out, uint32_t& n, uint32_t
     if ((st = pm_trace_photons(ctx, d.scene, seed, d.n_global, d.n_caustic, d.max_bounces, d.max_paths, j.g9,
                                d.n_caustic ? j.c9 : nullptr, &ts)) != PM_OK)
         return st;
-    if ((st = frame_grid(ctx, j.gg, j.g9, ts.n_global, d, d.params.r_global, d.params.k_global)) != PM_OK) return st;
+    if ((st = frame_grid(ctx, j.gg, j.g9, ts.n_global, d.n_global, d, d.params.r_global, d.params.k_global)) !=
+        PM_OK)
+        return st;
     if (ts.n_caustic > 0) {
-        if ((st = frame_grid(ctx, j.gc, j.c9, ts.n_caustic, d, d.params.r_caustic, d.params.k_caustic)) != PM_OK)
+        if ((st = frame_grid(ctx, j.gc, j.c9, ts.n_caustic, d.n_caustic, d, d.params.r_caustic,
+                             d.params.k_caustic)) != PM_OK)
             return st;
     } else if (j.gc) {
         PM_HIP(ctx, hipStreamSynchronize(ctx->stream));

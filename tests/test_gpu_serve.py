@@ -31,8 +31,9 @@ def _one_at_a_time(ctx, s):
     for f in range(s["frames"]):
         seed = (s["seed"] + SEED_STEP * (f + 1)) & ((1 << 64) - 1)
         g, c, st = ctx.trace_photons(s["dscene"], seed, s["n_global"], s["n_caustic"], 10, max_paths=s["max_paths"])
-        gg = ctx.grid_build(g, s["aabb"], cell_size=s["cell_size"])
-        gc = ctx.grid_build(c, s["aabb"], cell_size=s["cell_size"]) if st[2] > 0 else None
+        # the maps' first st[1] / st[2] photons (all of them once the paths fill them)
+        gg = ctx.grid_build(g[:, :st[1]], s["aabb"], cell_size=s["cell_size"])
+        gc = ctx.grid_build(c[:, :st[2]], s["aabb"], cell_size=s["cell_size"]) if st[2] > 0 else None
         _, rgb8, _ = ctx.render_frame(s["dscene"], gg, gc, s["camera"], s["max_depth"], s["k_global"],
                                       s["r_global"], s["k_caustic"], s["r_caustic"])
         ctx.sync()
@@ -143,6 +144,25 @@ def _serve_staggered(ctx, specs, arrivals, workers=0):
     res = [o.cpu().numpy() for o in outs]
     srv.close()
     return res
+
+
+def test_maps_short_of_their_budget(ctx):
+    """Jobs whose path limit stops the photon pass before the maps fill
+    (their blocks keep the plane stride of the requested counts): served
+    frames equal the one-at-a-time frames built from the photons stored,
+    with and without worker contexts."""
+    specs = _jobs(ctx)
+    for s in specs:
+        s["max_paths"] = s["n_global"] // 4
+    for s in specs:
+        seed = (s["seed"] + SEED_STEP) & ((1 << 64) - 1)
+        _, _, st = ctx.trace_photons(s["dscene"], seed, s["n_global"], s["n_caustic"], 10, max_paths=s["max_paths"])
+        assert 0 < st[1] < s["n_global"]
+    alone = [_one_at_a_time(ctx, s) for s in specs]
+    for workers in (0, 2):
+        got = _serve_staggered(ctx, specs, arrivals=[0, 0, 1], workers=workers)
+        for a, m in zip(alone, got):
+            assert np.array_equal(m, a), workers
 
 
 def test_staggered_arrivals_equal_alone(ctx):
