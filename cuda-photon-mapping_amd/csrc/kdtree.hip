@@ -56,12 +56,16 @@ __global__ __launch_bounds__(256) void k_photon_records(const float* __restrict_
     rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
 }
 
+// blockIdx.y = axis: order + axis * n is that axis's sorted order
 __global__ __launch_bounds__(256) void k_gather_recs(const float4* __restrict__ prec,
                                                     const uint32_t* __restrict__ order, uint32_t n,
-                                                    Rec* __restrict__ out) {
+                                                    Rec* __restrict__ out0, Rec* __restrict__ out1,
+                                                    Rec* __restrict__ out2) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    uint32_t id = order[j];
+    const uint32_t axis = blockIdx.y;
+    Rec* out = axis == 0 ? out0 : (axis == 1 ? out1 : out2);
+    uint32_t id = order[(size_t)axis * n + j];
     const float4 a = prec[3 * (size_t)id];
     Rec r;
     r.c[0] = a.x;
@@ -87,30 +91,33 @@ __device__ __forceinline__ int8_t plane_of(const Rec* A0, const Rec* A1, const R
     return p;
 }
 
-__global__ void k_root(const Rec* __restrict__ A0, const Rec* __restrict__ A1, const Rec* __restrict__ A2, uint32_t n,
-                       uint2* __restrict__ seg, int8_t* __restrict__ plane) {
-    seg[0] = make_uint2(0, n);
-    plane[0] = plane_of(A0, A1, A2, 0, n);
-}
-
-// Per node of the level: record the median, its key, and the children's
-// segments.
+// Per node of a level: its split plane (the argmax extent of its segment,
+// PhotonMap.cuh:252-273: an empty node keeps +-FLT_MAX corners -> plane 0)
+// and, when `medians`, its median, the median's key and the children's
+// segments. root_n > 0: the level is the root's, whose segment is [0, root_n).
+// lookback_counter (nullable): the level partition before this launch has
+// completed, so its look-back epoch is advanced here.
 __global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
                                                     const Rec* __restrict__ A2, uint32_t first, uint32_t count,
-                                                    uint32_t cap, uint2* __restrict__ seg,
-                                                    const int8_t* __restrict__ plane, uint32_t* __restrict__ node_idx,
-                                                    int8_t* __restrict__ node_plane, uint2* __restrict__ med) {
+                                                    uint32_t cap, uint32_t root_n, bool medians, uint2* __restrict__ seg,
+                                                    int8_t* __restrict__ plane, uint32_t* __restrict__ node_idx,
+                                                    int8_t* __restrict__ node_plane, uint2* __restrict__ med,
+                                                    uint32_t* __restrict__ lookback_counter) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && lookback_counter) pm::lookback_advance(lookback_counter, pm::lookback_epoch(lookback_counter));
     if (t >= count) return;
     uint32_t node = first + t;
-    uint2 s = seg[node];
+    if (root_n > 0) seg[node] = make_uint2(0, root_n);
+    const uint2 s = root_n > 0 ? make_uint2(0, root_n) : seg[node];
+    const int8_t p = s.y > s.x ? plane_of(A0, A1, A2, s.x, s.y) : (int8_t)0;
+    plane[node] = p;
+    if (!medians) return;
     uint32_t l = 2 * node + 1, r = 2 * node + 2;
     if (s.y <= s.x) {  // empty
         if (l < cap) seg[l] = make_uint2(0, 0);
         if (r < cap) seg[r] = make_uint2(0, 0);
         return;
     }
-    int8_t p = plane[node];
     const Rec* A = p == 0 ? A0 : (p == 1 ? A1 : A2);
     uint32_t mid = s.x + (s.y - s.x) / 2;
     Rec m = A[mid];
@@ -174,7 +181,7 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
     __shared__ uint32_t s_prefix;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t == 0) {
-        // the epoch is advanced by k_child_planes, the next launch: a tile
+        // the epoch is advanced by k_level_nodes, the next launch: a tile
         // here may finish its look-back before earlier tiles have read it
         s_epoch = pm::lookback_epoch(counter);
         uint32_t id = atomicAdd(counter, 1u);
@@ -323,20 +330,6 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
         if (dst < n) (a == 0 ? B0 : a == 1 ? B1 : B2)[dst] = rec[k];  // always true for a consistent tree
         if (a == 0) node_out[j] = nn;
     }
-}
-
-__global__ __launch_bounds__(256) void k_child_planes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                     const Rec* __restrict__ A2, uint32_t first, uint32_t count,
-                                                     const uint2* __restrict__ seg, int8_t* __restrict__ plane,
-                                                     uint32_t* __restrict__ lookback_counter) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    // the level partition before this launch has completed: advance its epoch
-    if (t == 0) pm::lookback_advance(lookback_counter, pm::lookback_epoch(lookback_counter));
-    if (t >= count) return;
-    uint32_t node = first + t;
-    uint2 s = seg[node];
-    // PhotonMap.cuh:252-273: an empty child keeps +-FLT_MAX corners -> plane 0
-    plane[node] = s.y > s.x ? plane_of(A0, A1, A2, s.x, s.y) : (int8_t)0;
 }
 
 // ---- bottom levels in LDS ---------------------------------------------------
@@ -519,7 +512,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
             const uint32_t mid = sg.x + (sg.y - sg.x) / 2;
             L.node[j] = (sg.y - sg.x <= 1 || j == mid) ? (uint16_t)0xFFFFu : (uint16_t)(j < mid ? 2 * i : 2 * i + 1);
         }
-        // child planes (k_child_planes)
+        // child planes (as k_level_nodes)
         for (uint32_t c = t; c < 2 * nd; c += kLocalThreads) {
             const ushort2 s = L.seg[cur ^ 1][c];
             L.plane[cur ^ 1][c] = s.y > s.x ? local_plane(L, oi, s.x, s.y) : (int8_t)0;
@@ -624,10 +617,11 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         return s;
     };
 
-    size_t rs = pm::radix_sort_scratch_bytes(n);
-    const size_t rb = (size_t)n * sizeof(Rec);
+    // the three axis sorts run as one segmented sort of 3n keys
+    size_t rs = pm::radix_sort_scratch_bytes(n, 3);
+    const size_t rb = (size_t)n * sizeof(Rec), n3 = 3 * (size_t)n;
     size_t need = pm::carve_size({rb, rb, rb, rb, rb, rb, (size_t)n * 4, (size_t)n * 4, (size_t)cap * 8, (size_t)cap,
-                                  (size_t)cap * 8, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, rs, (size_t)n * 48});
+                                  (size_t)cap * 8, n3 * 4, n3 * 4, n3 * 4, rs, (size_t)n * 48});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, need, &work);
     if (st != PM_OK) return bail(st);
@@ -641,27 +635,27 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     uint2* seg = cv.take<uint2>(cap);
     int8_t* plane = cv.take<int8_t>(cap);
     uint2* med = cv.take<uint2>(cap);
-    uint32_t* keys = cv.take<uint32_t>(n);
-    uint32_t* skeys = cv.take<uint32_t>(n);
-    uint32_t* order = cv.take<uint32_t>(n);
+    uint32_t* keys = cv.take<uint32_t>(n3);
+    uint32_t* skeys = cv.take<uint32_t>(n3);
+    uint32_t* order = cv.take<uint32_t>(n3);
     void* rw = cv.take<char>(rs);
     float4* prec = cv.take<float4>((size_t)n * 3);
 
     const uint32_t nb = pm::div_up(n, 256);
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(k_photon_records, dim3(nb), dim3(256), 0, s, d_photons9, n, prec);
-    for (int a = 0; a < 3; ++a) {
-        hipLaunchKernelGGL(k_axis_keys, dim3(nb), dim3(256), 0, s, d_photons9 + (size_t)a * n, n, keys);
-        if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, 32, rw)) != PM_OK) return bail(st);
-        hipLaunchKernelGGL(k_gather_recs, dim3(nb), dim3(256), 0, s, prec, order, n, A[a]);
-    }
+    // planes 0-2 of the block are the three coordinates, back to back
+    hipLaunchKernelGGL(k_axis_keys, dim3(pm::div_up((uint32_t)n3, 256)), dim3(256), 0, s, d_photons9, (uint32_t)n3,
+                       keys);
+    if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, 32, rw, nullptr, 3)) != PM_OK)
+        return bail(st);
+    hipLaunchKernelGGL(k_gather_recs, dim3(nb, 3), dim3(256), 0, s, prec, order, n, A[0], A[1], A[2]);
     if ((st = pm::fill_u32(ctx, T->node_idx, kFinal, cap)) != PM_OK) return bail(st);
     // node_plane = -1; the carve pads every block to 256 bytes, so whole
     // words past cap stay inside the allocation.
     if ((st = pm::fill_u32(ctx, reinterpret_cast<uint32_t*>(T->node_plane), 0xFFFFFFFFu, (cap + 3) / 4)) != PM_OK)
         return bail(st);
     if ((st = pm::fill_u32(ctx, node_of, 0u, n)) != PM_OK) return bail(st);
-    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, A[0], A[1], A[2], n, seg, plane);
 
     // Levels [0, Lg) run on the whole arrays; from level Lg every segment
     // holds at most kLocal records (a node of size s has children of at most
@@ -670,6 +664,9 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     for (uint32_t smax = n; smax > (uint32_t)kLocal; smax /= 2) ++Lg;
     // a segment of at most 1024 records spans at most 11 levels (node arrays of kLocal)
     if (L - Lg > 11) return bail(pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_build: internal level split"));
+    // the root's plane (and, above the LDS levels, its median)
+    hipLaunchKernelGGL(k_level_nodes, dim3(1), dim3(256), 0, s, A[0], A[1], A[2], 0u, 1u, cap, n, Lg > 0, seg, plane,
+                       T->node_idx, T->node_plane, med, nullptr);
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
         uint32_t first = (1u << lvl) - 1, count = 1u << lvl;
         if (lvl == Lg) {
@@ -678,8 +675,6 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
                                L - lvl, seg, plane, T->node_idx, T->node_plane);
             break;
         }
-        hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(count, 256)), dim3(256), 0, s, A[0], A[1], A[2], first,
-                           count, cap, seg, plane, T->node_idx, T->node_plane, med);
         if (lvl + 1 == L) break;  // every segment of the last level has size <= 1
         const uint32_t tiles = pm::div_up(3ull * n, kPTile);
         uint64_t* lstate;
@@ -689,9 +684,10 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
                            n, node_of, node_of2, seg, plane, med, lstate, lcounter, tiles);
         for (int a = 0; a < 3; ++a) std::swap(A[a], B[a]);
         std::swap(node_of, node_of2);
+        // the children's planes, and their medians above the LDS levels
         uint32_t cf = (1u << (lvl + 1)) - 1, cc = 1u << (lvl + 1);
-        hipLaunchKernelGGL(k_child_planes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, seg,
-                           plane, lcounter);
+        hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, cap,
+                           0u, lvl + 1 < Lg, seg, plane, T->node_idx, T->node_plane, med, lcounter);
     }
     hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx,
                        T->node_plane, cap, T->photons, T->nodes);

@@ -153,16 +153,18 @@ float tuned_cell(const pm_ctx* ctx, uint32_t n, const float aabb[6], float radiu
 pm_status scan_u32(pm_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, uint64_t* total);
 
 // Stable LSD radix sort of (key, value) pairs over key bits [0, key_bits).
-// Sorted output ends in keys_out / vals_out. vals_in may be null (identity).
-// n_dev (nullable): the element count on the device, at most n; only the
-// first *n_dev elements are sorted (and written).
+// Sorted output ends in keys_out / vals_out. vals_in may be null (identity:
+// the index within the segment). segs arrays of n pairs laid end to end are
+// sorted each on its own, in the same launches. n_dev (nullable, one
+// segment): the element count on the device, at most n; only the first
+// *n_dev elements are sorted (and written).
 pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                            uint32_t* vals_out, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev = nullptr);
-size_t radix_sort_scratch_bytes(uint32_t n);
+                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev = nullptr, uint32_t segs = 1);
+size_t radix_sort_scratch_bytes(uint32_t n, uint32_t segs = 1);
 pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                                 uint32_t* vals_out, uint32_t n, int key_bits, void* work,
-                                const uint32_t* n_dev = nullptr);
+                                const uint32_t* n_dev = nullptr, uint32_t segs = 1);
 
 pm_status fill_u32(pm_ctx* ctx, uint32_t* p, uint32_t value, size_t count);
 

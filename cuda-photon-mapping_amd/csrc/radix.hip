@@ -6,6 +6,11 @@
 // in LDS in digit order and writes each digit's run contiguously. Stability
 // makes the grid build's in-cell order (and the kd-tree's tie order) follow
 // the input index, which the reference's atomicAdd slots do not guarantee.
+//
+// Several equal-length arrays laid end to end (segments) sort in the same
+// launches: workgroup b takes tile b % tiles of segment b / tiles, and the
+// histogram table is segment-major, so the scan's offsets place every
+// segment's keys in its own range of the output.
 #include "pm_internal.hpp"
 
 namespace {
@@ -17,13 +22,15 @@ constexpr int kWaveKeys = 64 * kItems;     // 1024 keys per wave
 constexpr int kTile = kThreads * kItems;   // 4096 keys per tile
 constexpr int kBins = 256;
 
-// n_dev (nullable): the element count on the device, at most n (the launch
-// covers n; tiles past the device count contribute nothing).
+// n: the segment length; n_dev (nullable, one segment only): the element
+// count on the device, at most n (the launch covers n; tiles past the device
+// count contribute nothing).
 __global__ __launch_bounds__(kThreads) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                      uint32_t mask, uint32_t* __restrict__ hist, uint32_t tiles,
                                                      const uint32_t* __restrict__ n_dev) {
     __shared__ uint32_t s_hist[kBins];
-    const uint32_t t = threadIdx.x, tile = blockIdx.x;
+    const uint32_t t = threadIdx.x, seg = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const uint32_t* sk = keys + (size_t)seg * n;
     if (n_dev) n = min(n, *n_dev);
     s_hist[t] = 0;
     __syncthreads();
@@ -31,10 +38,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_hist(const uint32_t* __restrict
 #pragma unroll 4
     for (int i = 0; i < kItems; ++i) {
         size_t idx = base + (size_t)i * kThreads + t;
-        if (idx < n) atomicAdd(&s_hist[(keys[idx] >> shift) & mask], 1u);
+        if (idx < n) atomicAdd(&s_hist[(sk[idx] >> shift) & mask], 1u);
     }
     __syncthreads();
-    hist[(size_t)t * tiles + tile] = s_hist[t];
+    hist[((size_t)seg * kBins + t) * tiles + tile] = s_hist[t];
 }
 
 __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restrict__ keys_in,
@@ -43,8 +50,10 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restr
                                                         uint32_t n, int shift, uint32_t mask,
                                                         const uint32_t* __restrict__ offsets, uint32_t tiles,
                                                         const uint32_t* __restrict__ n_dev) {
+    const uint32_t seg = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const size_t sbase = (size_t)seg * n;  // the segment's first element
     if (n_dev) n = min(n, *n_dev);
-    if ((size_t)blockIdx.x * kTile >= n) return;  // (block-uniform, before any barrier)
+    if ((size_t)tile * kTile >= n) return;  // (block-uniform, before any barrier)
     __shared__ uint32_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
     __shared__ uint32_t s_whist[kWaves][kBins];
@@ -52,9 +61,9 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restr
     __shared__ uint32_t s_goff[kBins];
     __shared__ uint32_t s_scan[kWaves];
 
-    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6, tile = blockIdx.x;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int w = 0; w < kWaves; ++w) s_whist[w][t] = 0;
-    s_goff[t] = offsets[(size_t)t * tiles + tile];
+    s_goff[t] = offsets[((size_t)seg * kBins + t) * tiles + tile];
     __syncthreads();
 
     const size_t wbase = (size_t)tile * kTile + (size_t)wave * kWaveKeys;
@@ -64,8 +73,8 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restr
     for (int i = 0; i < kItems; ++i) {
         size_t idx = wbase + (size_t)i * 64 + lane;
         bool valid = idx < n;
-        key[i] = valid ? keys_in[idx] : 0u;
-        val[i] = valid ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+        key[i] = valid ? keys_in[sbase + idx] : 0u;
+        val[i] = valid ? (vals_in ? vals_in[sbase + idx] : (uint32_t)idx) : 0u;
         uint32_t d = (key[i] >> shift) & mask;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -136,18 +145,23 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint32_t* __restr
 
 namespace pm {
 
-size_t radix_sort_scratch_bytes(uint32_t n) {
-    uint32_t tiles = div_up(n ? n : 1, kTile);
-    return carve_size({(size_t)n * 4, (size_t)n * 4, (size_t)kBins * tiles * 4, (size_t)kBins * tiles * 4});
+size_t radix_sort_scratch_bytes(uint32_t n, uint32_t segs) {
+    const uint32_t tiles = div_up(n ? n : 1, kTile);
+    const size_t all = (size_t)n * segs;
+    return carve_size({all * 4, all * 4, (size_t)segs * kBins * tiles * 4, (size_t)segs * kBins * tiles * 4});
 }
 
 pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                            uint32_t* vals_out, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int key_bits,
-                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev) {
-    if (n == 0) return PM_OK;
+                           uint32_t* hist, uint32_t* offs, const uint32_t* n_dev, uint32_t segs) {
+    if (n == 0 || segs == 0) return PM_OK;
+    if (segs > 1 && n_dev) return fail(ctx, PM_ERR_INVALID, "radix sort: a device count sorts one segment");
     if (key_bits < 1) key_bits = 1;
     if (key_bits > 32) key_bits = 32;
     const uint32_t tiles = div_up(n, kTile);
+    const size_t table = (size_t)segs * kBins * tiles;
+    if (table > 0xFFFFFFFFull || (uint64_t)n * segs > 0xFFFFFFFFull)
+        return fail(ctx, PM_ERR_UNSUPPORTED, "radix sort: %u x %u keys", segs, n);
     const int passes = (key_bits + 7) / 8;
     const uint32_t* src_k = keys_in;
     const uint32_t* src_v = vals_in;
@@ -159,13 +173,13 @@ pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t*
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* dk = to_out ? keys_out : keys_tmp;
         uint32_t* dv = to_out ? vals_out : vals_tmp;
-        hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kThreads), 0, ctx->stream, src_k, n, shift, mask, hist, tiles,
-                           n_dev);
+        hipLaunchKernelGGL(k_rs_hist, dim3(segs * tiles), dim3(kThreads), 0, ctx->stream, src_k, n, shift, mask, hist,
+                           tiles, n_dev);
         PM_CHECK_LAUNCH(ctx, "k_rs_hist");
-        pm_status s = scan_u32(ctx, hist, offs, kBins * tiles, false, nullptr);
+        pm_status s = scan_u32(ctx, hist, offs, (uint32_t)table, false, nullptr);
         if (s != PM_OK) return s;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(kThreads), 0, ctx->stream, src_k, src_v, dk, dv, n, shift,
-                           mask, offs, tiles, n_dev);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(segs * tiles), dim3(kThreads), 0, ctx->stream, src_k, src_v, dk, dv, n,
+                           shift, mask, offs, tiles, n_dev);
         PM_CHECK_LAUNCH(ctx, "k_rs_scatter");
         src_k = dk;
         src_v = dv;
@@ -173,16 +187,18 @@ pm_status radix_sort_pairs(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t*
     return PM_OK;
 }
 
-// Sort using a caller-provided work block of radix_sort_scratch_bytes(n).
+// Sort using a caller-provided work block of radix_sort_scratch_bytes(n, segs).
 pm_status radix_sort_pairs_auto(pm_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                                uint32_t* vals_out, uint32_t n, int key_bits, void* work, const uint32_t* n_dev) {
-    uint32_t tiles = div_up(n ? n : 1, kTile);
+                                uint32_t* vals_out, uint32_t n, int key_bits, void* work, const uint32_t* n_dev,
+                                uint32_t segs) {
+    const uint32_t tiles = div_up(n ? n : 1, kTile);
+    const size_t all = (size_t)n * segs;
     Carve cv{static_cast<char*>(work)};
-    uint32_t* kt = cv.take<uint32_t>(n);
-    uint32_t* vt = cv.take<uint32_t>(n);
-    uint32_t* hist = cv.take<uint32_t>((size_t)kBins * tiles);
-    uint32_t* offs = cv.take<uint32_t>((size_t)kBins * tiles);
-    return radix_sort_pairs(ctx, keys_in, vals_in, keys_out, vals_out, kt, vt, n, key_bits, hist, offs, n_dev);
+    uint32_t* kt = cv.take<uint32_t>(all);
+    uint32_t* vt = cv.take<uint32_t>(all);
+    uint32_t* hist = cv.take<uint32_t>((size_t)segs * kBins * tiles);
+    uint32_t* offs = cv.take<uint32_t>((size_t)segs * kBins * tiles);
+    return radix_sort_pairs(ctx, keys_in, vals_in, keys_out, vals_out, kt, vt, n, key_bits, hist, offs, n_dev, segs);
 }
 
 }  // namespace pm
