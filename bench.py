@@ -254,6 +254,57 @@ def frame_bench(ctx, n_frames):
         tstages.append(sm)
     tstages = np.median(np.array(tstages), 0)
     ctx_kd.close()
+    # Pipelined: the same frame without the kd-trees (the render reads the
+    # grids only) as one pm_pipeline frame -- trace, both grids rebuilt in
+    # place, eye rays, both gathers, fold -- enqueued with every count on the
+    # device, and the frame captured into a HIP graph and replayed. Its
+    # context (one pipeline per context) runs on a stream of its own, as
+    # graph capture needs; the frame is checked against the eager calls.
+    pstream = torch.cuda.Stream(ctx.device)
+    npix = 1280 * 720
+    with torch.cuda.stream(pstream):
+        pctx = pm355.Context(ctx.device.index)
+        pl = pctx.pipeline(ds, aabb, 0.02, 0.02, 2_000_000, 500_000, cam, max_bounces=10, max_paths=200_000_000,
+                           plan_seed=1, max_depth=4, k_global=100, r_global=0.02, k_caustic=100, r_caustic=0.02)
+        p8 = torch.zeros(npix * 3, dtype=torch.uint8, device=ctx.device)
+        p3 = torch.zeros((3, npix), dtype=torch.float32, device=ctx.device)
+        plat = []
+        for i in range(n_frames + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pl.frame(100 + i, p8, p3)
+            pstream.synchronize()
+            if i > 0:
+                plat.append((time.perf_counter() - t0) * 1e3)
+        pinfo = pl.status()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=pstream):
+            rc = pctx.lib.pm_pipeline_frame(pl.h, 7, pm355._ptr(p3), pm355._ptr(p8))
+        pctx.check(rc)
+        glat = []
+        for i in range(n_frames + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            graph.replay()
+            pstream.synchronize()
+            if i > 0:
+                glat.append((time.perf_counter() - t0) * 1e3)
+        g8 = p8.clone()
+    g, c, _ = ctx.trace_photons(ds, 7, 2_000_000, 500_000, 10, max_paths=200_000_000)
+    gg, gc = ctx.grid_build(g, aabb, cell_size=0.02), ctx.grid_build(c, aabb, cell_size=0.02)
+    _, e8, _ = ctx.render_frame(ds, gg, gc, cam, 4, 100, 0.02, 100, 0.02)
+    torch.cuda.synchronize()
+    pipelined = {"p50_ms": float(np.percentile(plat, 50)), "p99_ms": float(np.percentile(plat, 99)),
+                 "graph_replay_p50_ms": float(np.percentile(glat, 50)),
+                 "graph_replay_p99_ms": float(np.percentile(glat, 99)),
+                 "maps_complete": bool(pinfo["complete"]), "frame_equals_eager": bool(torch.equal(g8, e8)),
+                 "note": "no kd-trees (the render reads the grids only); compare one_stream_stage_ms trace + "
+                         "grid_builds + render"}
+    for h in (gg, gc):
+        h.close()
+    del graph, g, c
+    pl.close()
+    pctx.close()
     lat, stages, stored = out[True]
     slat = out[False][0]
     valu = None
@@ -286,6 +337,7 @@ def frame_bench(ctx, n_frames):
         "one_stream_stage_ms": {"trace": out[False][1][0], "grid_builds": out[False][1][1],
                                 "kdtree_builds": out[False][1][2], "render": out[False][1][3]},
         "trace_valu": valu,
+        "pipelined": pipelined,
         "tuned": {"p50_ms": float(np.percentile(tlat, 50)), "p99_ms": float(np.percentile(tlat, 99)),
                   "stage_ms": {"trace": tstages[0], "grid_builds": tstages[1], "kdtree_builds": tstages[2],
                                "render": tstages[3]},

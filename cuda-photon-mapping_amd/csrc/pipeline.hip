@@ -9,11 +9,12 @@
 // pm_pipeline_create renders one planning frame eagerly (pm_trace_photons,
 // pm_grid_build, the frame's phases) to size everything: the photon maps and
 // grids, the trace's path schedule (the planning frame's paths plus a
-// quarter), the query block (every pixel's full ray tree), and the
-// context's scratch, which one pipelined frame then grows to its final
-// size. pm_pipeline_frame only enqueues. The maps, grids and image equal
-// what the eager calls produce for the same seed whenever the scheduled
-// paths fill the maps, which pm_pipeline_status reports.
+// quarter), the query block (the planning frame's hit count: the ray trees
+// depend only on the scene and the camera), and the context's scratch,
+// which one pipelined frame then grows to its final size.
+// pm_pipeline_frame only enqueues. The maps, grids and image equal what the
+// eager calls produce for the same seed whenever the scheduled paths fill
+// the maps, which pm_pipeline_status reports.
 #include <algorithm>
 #include <new>
 
@@ -81,12 +82,18 @@ extern "C" pm_status pm_pipeline_create(pm_ctx* ctx, const pm_pipeline_desc* des
     if (d.n_caustic > 0 &&
         (st = pm_grid_build(ctx, pl->c9, d.n_caustic, d.aabb, d.cell_caustic, 1000, PM_LAYOUT_LINEAR, &pl->gc)) != PM_OK)
         return bail(st);
+    // the planning frame's render: its hit count sizes the query block (the
+    // ray trees depend only on the scene and the camera)
+    pm_frame_stats fs{};
+    if ((st = pm_render_frame(ctx, d.scene, pl->gg, pl->gc, &d.camera, &d.params, nullptr, nullptr, &fs)) != PM_OK)
+        return bail(st);
     // the path schedule: the planning frame's paths and a quarter more
     pl->paths = std::min<uint64_t>(d.max_paths, ts.paths + ts.paths / 4 + 4096);
     if ((st = pm::trace_plan_create(ctx, d.scene, d.n_global, d.n_caustic, d.max_bounces, pl->paths, &pl->trace)) !=
         PM_OK)
         return bail(st);
-    if ((st = pm::frame_plan_create(ctx, d.scene, &d.camera, &d.params, &pl->frame)) != PM_OK) return bail(st);
+    if ((st = pm::frame_plan_create(ctx, d.scene, &d.camera, &d.params, fs.hits, &pl->frame)) != PM_OK)
+        return bail(st);
     // one pipelined frame grows the context's scratch and scan state to what
     // every later frame uses
     if ((st = pm_pipeline_frame(pl, d.plan_seed, nullptr, nullptr)) != PM_OK) return bail(st);
@@ -116,7 +123,8 @@ extern "C" pm_status pm_pipeline_status(pm_pipeline* pl, pm_pipeline_info* out) 
     out->scheduled_paths = pl->paths;
     out->n_global = (uint32_t)std::min<uint64_t>(h[0], pl->d.n_global);
     out->n_caustic = (uint32_t)std::min<uint64_t>(h[1], pl->d.n_caustic);
-    out->complete = h[0] >= pl->d.n_global && h[1] >= pl->d.n_caustic ? 1 : 0;
+    out->complete =
+        h[0] >= pl->d.n_global && h[1] >= pl->d.n_caustic && h[2] <= pm::frame_plan_capacity(pl->frame) ? 1 : 0;
     out->hits = h[2];
     return PM_OK;
 }

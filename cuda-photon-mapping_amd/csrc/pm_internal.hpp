@@ -197,13 +197,14 @@ const uint64_t* trace_plan_counts(const TracePlan* T);
 void trace_plan_destroy(TracePlan* T);
 
 // Planned frames (render.hip): pm_render_frame's phases with the hit count
-// kept on the device (query block sized for every pixel's full ray tree).
+// kept on the device (query block sized for `hits`, the planning frame's).
 struct FramePlan;
 pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera* camera,
-                            const pm_render_params* params, FramePlan** out);
+                            const pm_render_params* params, uint64_t hits, FramePlan** out);
 pm_status frame_plan_enqueue(pm_ctx* ctx, FramePlan* F, const pm_grid* global_grid, const pm_grid* caustic_grid,
                              float* d_rgb3, uint8_t* d_rgb8);
 const uint64_t* frame_plan_hits(const FramePlan* F);
+uint32_t frame_plan_capacity(const FramePlan* F);
 void frame_plan_destroy(FramePlan* F);
 
 // Fused gather + radiance over a query block of capacity `cap` (the planes'

@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_render_compact(const float* __restrict_
     for (uint32_t h = 0; h < c; ++h) {
         size_t s = (size_t)h * npix + pix;
         uint32_t q = o + h;
+        if (q >= nq) break;  // (a query block sized below the hit count keeps its first nq)
         q3[q] = hp[s];
         q3[nq + q] = hp[plane + s];
         q3[2 * (size_t)nq + q] = hp[2 * plane + s];
@@ -250,6 +251,7 @@ __global__ __launch_bounds__(256) void k_render_combine(const DevModel* __restri
     for (uint32_t h = 0; h < c; ++h) {
         size_t s = (size_t)h * npix + pix;
         uint32_t q = o + h;
+        if (q >= nq) break;
         int code = hm[s];
         const DevModel& mat = models[code & 0xFFFF];
         bool outside = (code >> 16) & 1;
@@ -559,8 +561,9 @@ extern "C" pm_status pm_render_frame(pm_ctx* ctx, const pm_scene* scene, const p
 
 // ---------------------------------------------------------------------------
 // Planned frames: pm_render_frame's phases enqueued without reading the hit
-// count back. The query block is sized for every pixel's full ray tree
-// (npix * (2^max_depth - 1) hits); the hit count stays on the device and the
+// count back. The ray trees depend only on the scene and the camera, so a
+// frame's hit count is the same every frame: the query block is sized by the
+// planning frame's count (`hits`), the count stays on the device, and the
 // gathers skip positions past it. Same kernels, same frame.
 namespace pm {
 
@@ -578,7 +581,7 @@ struct FramePlan {
 };
 
 pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera* camera,
-                            const pm_render_params* params, FramePlan** out) {
+                            const pm_render_params* params, uint64_t hits, FramePlan** out) {
     *out = nullptr;
     if (!scene->d_tris9 || !scene->d_normals9 || !scene->h_models || scene->n_models <= 0 || scene->n_models > 0xFFFF)
         return fail(ctx, PM_ERR_INVALID, "frame plan: empty scene");
@@ -597,13 +600,13 @@ pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera*
     F->H = (uint32_t)camera->height;
     F->npix = (uint32_t)npix64;
     F->hmax = (1 << R.max_depth) - 1;
-    F->cap = (uint32_t)cap64;
+    F->cap = (uint32_t)std::min<uint64_t>(cap64, std::max<uint64_t>(hits, 1));
     F->nm = scene->n_models;
     F->R = R;
-    const size_t cap = F->cap, npix = F->npix;
-    const size_t bytes = carve_size({(size_t)F->nm * sizeof(DevModel), (size_t)F->nm * 32, cap * 12, cap * 12,
-                                     cap * 12, cap * 4, npix * 4, npix * 4, 64, cap * 12, cap * 12, cap * 12, cap * 12,
-                                     cap * 4});
+    const size_t cap = F->cap, npix = F->npix, slots = (size_t)cap64;
+    const size_t bytes = carve_size({(size_t)F->nm * sizeof(DevModel), (size_t)F->nm * 32, slots * 12, slots * 12,
+                                     slots * 12, slots * 4, npix * 4, npix * 4, 64, cap * 12, cap * 12, cap * 12,
+                                     cap * 12, cap * 4});
     hipError_t e = hipMalloc(&F->mem, bytes);
     if (e != hipSuccess) {
         delete F;
@@ -612,10 +615,10 @@ pm_status frame_plan_create(pm_ctx* ctx, const pm_scene* scene, const pm_camera*
     Carve cv{static_cast<char*>(F->mem)};
     F->d_models = cv.take<DevModel>(F->nm);
     void* tmp = cv.take<char>((size_t)F->nm * 32);
-    float* hp = cv.take<float>(cap * 3);
-    float* hn = cv.take<float>(cap * 3);
-    F->hc = cv.take<float>(cap * 3);
-    F->hm = cv.take<int>(cap);
+    float* hp = cv.take<float>(slots * 3);
+    float* hn = cv.take<float>(slots * 3);
+    F->hc = cv.take<float>(slots * 3);
+    F->hm = cv.take<int>(slots);
     F->count = cv.take<uint32_t>(npix);
     F->off = cv.take<uint32_t>(npix);
     F->total = cv.take<uint64_t>(1);
@@ -658,6 +661,7 @@ void frame_plan_destroy(FramePlan* F) {
 }
 
 const uint64_t* frame_plan_hits(const FramePlan* F) { return F->total; }
+uint32_t frame_plan_capacity(const FramePlan* F) { return F->cap; }
 
 pm_status frame_plan_enqueue(pm_ctx* ctx, FramePlan* F, const pm_grid* global_grid, const pm_grid* caustic_grid,
                              float* d_rgb3, uint8_t* d_rgb8) {

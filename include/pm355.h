@@ -432,7 +432,8 @@ pm_status pm_frame_copy_queries(pm_ctx* ctx, const pm_frame* frame, float* d_que
  * renders one planning frame eagerly (plan_seed) to size the photon maps,
  * the grids (built at the given cell sizes), the trace's path schedule (the
  * planning frame's paths plus a quarter, at most max_paths) and the query
- * block (every pixel's full ray tree), and grows the context's scratch; it
+ * block (the planning frame's hit count: ray trees depend only on the scene
+ * and the camera), and grows the context's scratch; it
  * fails with PM_ERR_CAPACITY if max_paths cannot fill the maps.
  * pm_pipeline_frame enqueues one frame for `seed` (uchar3 and/or HDR floats
  * into caller buffers, either nullable): no allocation, no host
