@@ -107,3 +107,65 @@ def test_pipeline_frame_replays_from_graph():
         pl.close()
         ctx.close()
         ref_ctx.close()
+
+
+def test_batched_pipeline_frames_replay_from_one_graph():
+    """Three scenes' frames (three pipelines, one per context, on one stream)
+    captured into a single HIP graph: every replay renders each scene's frame
+    exactly as the eager calls do, with eager work in between replays."""
+    import torch
+
+    import pm355
+    from pm355.scenes import camera, random_scene
+
+    scenes = [cornell_box(), random_scene(4), random_scene(9)]
+    cams = [cornell_camera(96, 72), camera((0.5, 0.6, 1.6), (0.1, -0.2, -1.0), (0, 1, 0), 50.0, 80, 64),
+            cornell_camera(64, 48)]
+    seeds = (31, 32, 33)
+    ncs = (NC // 4, NC // 4, 0)  # the third frame without a caustic map
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ref_ctx = pm355.Context(0)
+        ctxs, pls, dss, outs = [], [], [], []
+        for sc, cam, nc in zip(scenes, cams, ncs):
+            c = pm355.Context(0)
+            ds = c.upload_scene(sc, accel=True)
+            pl = c.pipeline(ds, sc.aabb(), CELL, CELL, NG // 4, nc, cam, max_bounces=10,
+                            max_paths=200_000_000, plan_seed=2, max_depth=4, k_global=K, r_global=R, k_caustic=K,
+                            r_caustic=R)
+            npix = int(cam["width"].item()) * int(cam["height"].item())
+            outs.append((torch.zeros((3, npix), dtype=torch.float32, device="cuda"),
+                         torch.zeros(npix * 3, dtype=torch.uint8, device="cuda")))
+            ctxs.append(c)
+            pls.append(pl)
+            dss.append(ds)
+        refs = []
+        for ds, sc, cam, seed, nc in zip(dss, scenes, cams, seeds, ncs):
+            g, c_, st = ref_ctx.trace_photons(ds, seed, NG // 4, nc, 10, max_paths=200_000_000)
+            gg = ref_ctx.grid_build(g[:, :st[1]], sc.aabb(), cell_size=CELL)
+            gc = ref_ctx.grid_build(c_[:, :st[2]], sc.aabb(), cell_size=CELL) if st[2] > 0 else None
+            e3, e8, _ = ref_ctx.render_frame(ds, gg, gc, cam, 4, K, R, K, R)
+            ref_ctx.sync()
+            refs.append((e3.clone(), e8.clone()))
+        s.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=s):
+            for c, pl, seed, (o3, o8) in zip(ctxs, pls, seeds, outs):
+                assert c.lib.pm_pipeline_frame(pl.h, seed, pm355._ptr(o3), pm355._ptr(o8)) == pm355.PM_OK
+        for rep in range(3):
+            print(f"batched graph replay rep={rep}", flush=True)
+            for o3, o8 in outs:
+                o3.fill_(-1.0)
+                o8.zero_()
+            _eager(ref_ctx, dss[0], scenes[0], cams[0], 90 + rep)  # eager work in between
+            graph.replay()
+            s.synchronize()
+            for i, (pl, (o3, o8), (e3, e8)) in enumerate(zip(pls, outs, refs)):
+                assert pl.status()["complete"], i
+                assert torch.equal(o3.view(torch.int32), e3.view(torch.int32)), (rep, i)
+                assert torch.equal(o8, e8), (rep, i)
+        del graph
+        for pl, c in zip(pls, ctxs):
+            pl.close()
+            c.close()
+        ref_ctx.close()
