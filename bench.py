@@ -591,6 +591,10 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
 
     from pm355.scenes import camera, random_scene
 
+    # the earlier sections' cached device blocks go back first: the server's
+    # default budget is a share of the memory free when it is created
+    ctx.trim()
+    torch.cuda.empty_cache()
     rng = np.random.default_rng(seed)
     res_set = [(128, 128), (256, 256), (640, 480), (1280, 720)]
     jobs = []
@@ -729,6 +733,8 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
 
     from pm355.scenes import camera, random_scene
 
+    ctx.trim()
+    torch.cuda.empty_cache()
     scenes = [random_scene(100 + i) for i in range(6)]
     boxes = [sc.aabb() for sc in scenes]
     dss = [ctx.upload_scene(sc, accel=True) for sc in scenes]
