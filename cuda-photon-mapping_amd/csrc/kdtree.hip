@@ -5,7 +5,10 @@
 // of quicksorts:
 //   * three presorted orders, one per axis, each a stable radix sort of the
 //     coordinate (so ties are ordered by photon index, the total order the
-//     tree is defined on), stored as 16-byte records {x, y, z, id};
+//     tree is defined on); a photon's rank on an axis is its position in
+//     that sort, and its record in an axis's order holds its ranks on the two
+//     other axes (8 bytes): comparing ranks compares (coordinate, index), and
+//     in the split axis's own order the side is the position;
 //   * at every level each node's segment [from, to) has its median at
 //     mid = from + (to - from)/2 in the order of the node's split axis;
 //     the other two orders are stable-partitioned around that median in one
@@ -23,10 +26,38 @@
 
 namespace {
 
+// a photon in LDS (the subtrees finished by one workgroup)
 struct Rec {
     float c[3];
     uint32_t id;
 };
+
+// A photon's record in the order of axis a: its ranks on the two other axes,
+// other_axis(a, 0) < other_axis(a, 1).
+struct Rk {
+    uint32_t r[2];
+};
+
+__host__ __device__ __forceinline__ int other_axis(int a, int i) {
+    return i == 0 ? (a == 0 ? 1 : 0) : (a == 2 ? 1 : 2);
+}
+
+// The level phase's arrays: the three orders (A[a][j]: the record at
+// position j of axis a's order), each axis's sorted photon indices
+// (order[a n + r]: the photon of rank r) and ranks (rank4[4 id + a]), and
+// the input planes (coordinates by photon index).
+struct Orders {
+    const Rk* A[3];
+    const uint32_t* order;
+    const uint32_t* rank4;
+    const float* ph9;
+    uint32_t n;
+};
+
+// the photon at position j of axis a's order
+__device__ __forceinline__ uint32_t id_at(const Orders& O, int a, uint32_t j) {
+    return O.order[(size_t)other_axis(a, 0) * O.n + O.A[a][j].r[0]];
+}
 
 constexpr uint32_t kFinal = 0xFFFFFFFFu;
 
@@ -56,49 +87,56 @@ __global__ __launch_bounds__(256) void k_photon_records(const float* __restrict_
     rec[3 * (size_t)i + 2] = make_float4(v[8], 0.f, 0.f, 0.f);
 }
 
-// blockIdx.y = axis: order + axis * n is that axis's sorted order
-__global__ __launch_bounds__(256) void k_gather_recs(const float4* __restrict__ prec,
-                                                    const uint32_t* __restrict__ order, uint32_t n,
-                                                    Rec* __restrict__ out0, Rec* __restrict__ out1,
-                                                    Rec* __restrict__ out2) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+// rank4[order[a n + r]].a = r (blockIdx.y = axis a): each photon's three
+// ranks in one 16-byte entry, so a record is one gather
+__global__ __launch_bounds__(256) void k_ranks(const uint32_t* __restrict__ order, uint32_t n,
+                                              uint32_t* __restrict__ rank4) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
+    if (r >= n) return;
+    rank4[4 * (size_t)order[(size_t)a * n + r] + a] = r;
+}
+
+// the records of axis a's order (blockIdx.y = a): ranks on the other axes
+__global__ __launch_bounds__(256) void k_rank_recs(const uint32_t* __restrict__ order,
+                                                  const uint4* __restrict__ rank4, uint32_t n,
+                                                  Rk* __restrict__ out0, Rk* __restrict__ out1,
+                                                  Rk* __restrict__ out2) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
     if (j >= n) return;
-    const uint32_t axis = blockIdx.y;
-    Rec* out = axis == 0 ? out0 : (axis == 1 ? out1 : out2);
-    uint32_t id = order[(size_t)axis * n + j];
-    const float4 a = prec[3 * (size_t)id];
-    Rec r;
-    r.c[0] = a.x;
-    r.c[1] = a.y;
-    r.c[2] = a.z;
-    r.id = id;
-    out[j] = r;
+    const uint4 rk = rank4[order[(size_t)a * n + j]];
+    Rk r;
+    r.r[0] = a == 0 ? rk.y : rk.x;
+    r.r[1] = a == 2 ? rk.y : rk.z;
+    (a == 0 ? out0 : a == 1 ? out1 : out2)[j] = r;
 }
 
 // PhotonMap::find_largest_plane (PhotonMap.cuh:161-178) from the extreme
-// records of a non-empty segment in each axis order.
-__device__ __forceinline__ int8_t plane_of(const Rec* A0, const Rec* A1, const Rec* A2, uint32_t f, uint32_t t) {
-    float d0 = fabsf(A0[t - 1].c[0] - A0[f].c[0]);
-    float d1 = fabsf(A1[t - 1].c[1] - A1[f].c[1]);
-    float d2 = fabsf(A2[t - 1].c[2] - A2[f].c[2]);
+// photons of a non-empty segment in each axis order.
+__device__ __forceinline__ int8_t plane_of(const Orders& O, uint32_t f, uint32_t t) {
+    float d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float* c = O.ph9 + (size_t)a * O.n;
+        d[a] = fabsf(c[id_at(O, a, t - 1)] - c[id_at(O, a, f)]);
+    }
     int8_t p = 0;
-    float mx = d0;
-    if (d1 > mx) {
-        mx = d1;
+    float mx = d[0];
+    if (d[1] > mx) {
+        mx = d[1];
         p = 1;
     }
-    if (d2 > mx) p = 2;
+    if (d[2] > mx) p = 2;
     return p;
 }
 
 // Per node of a level: its split plane (the argmax extent of its segment,
 // PhotonMap.cuh:252-273: an empty node keeps +-FLT_MAX corners -> plane 0)
 // and, when `medians`, its median, the median's key and the children's
-// segments. root_n > 0: the level is the root's, whose segment is [0, root_n).
-// lookback_counter (nullable): the level partition before this launch has
-// completed, so its look-back epoch is advanced here.
-__global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                    const Rec* __restrict__ A2, uint32_t first, uint32_t count,
+// segments (med = {the median's rank on the plane, its index}). root_n > 0:
+// the level is the root's, whose segment is [0, root_n). lookback_counter
+// (nullable): the level partition before this launch has completed, so its
+// look-back epoch is advanced here.
+__global__ __launch_bounds__(256) void k_level_nodes(Orders O, uint32_t first, uint32_t count,
                                                     uint32_t cap, uint32_t root_n, bool medians, uint2* __restrict__ seg,
                                                     int8_t* __restrict__ plane, uint32_t* __restrict__ node_idx,
                                                     int8_t* __restrict__ node_plane, uint2* __restrict__ med,
@@ -109,7 +147,7 @@ __global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0,
     uint32_t node = first + t;
     if (root_n > 0) seg[node] = make_uint2(0, root_n);
     const uint2 s = root_n > 0 ? make_uint2(0, root_n) : seg[node];
-    const int8_t p = s.y > s.x ? plane_of(A0, A1, A2, s.x, s.y) : (int8_t)0;
+    const int8_t p = s.y > s.x ? plane_of(O, s.x, s.y) : (int8_t)0;
     plane[node] = p;
     if (!medians) return;
     uint32_t l = 2 * node + 1, r = 2 * node + 2;
@@ -118,12 +156,11 @@ __global__ __launch_bounds__(256) void k_level_nodes(const Rec* __restrict__ A0,
         if (r < cap) seg[r] = make_uint2(0, 0);
         return;
     }
-    const Rec* A = p == 0 ? A0 : (p == 1 ? A1 : A2);
     uint32_t mid = s.x + (s.y - s.x) / 2;
-    Rec m = A[mid];
-    node_idx[node] = m.id;
+    const uint32_t id = id_at(O, p, mid);
+    node_idx[node] = id;
     node_plane[node] = p;
-    med[node] = make_uint2(__float_as_uint(m.c[p]), m.id);
+    med[node] = make_uint2(O.rank4[4 * (size_t)id + p], id);
     if (s.y - s.x == 1) {
         if (l < cap) seg[l] = make_uint2(0, 0);
         if (r < cap) seg[r] = make_uint2(0, 0);
@@ -171,8 +208,8 @@ __device__ __forceinline__ uint64_t seg_state(uint32_t epoch, uint64_t status, u
 }
 
 __global__ __launch_bounds__(kPThreads) void k_level_partition(
-    const Rec* __restrict__ A0, const Rec* __restrict__ A1, const Rec* __restrict__ A2, Rec* __restrict__ B0,
-    Rec* __restrict__ B1, Rec* __restrict__ B2, uint32_t n, const uint32_t* __restrict__ node_in,
+    const Rk* __restrict__ A0, const Rk* __restrict__ A1, const Rk* __restrict__ A2, Rk* __restrict__ B0,
+    Rk* __restrict__ B1, Rk* __restrict__ B2, uint32_t n, const uint32_t* __restrict__ node_in,
     uint32_t* __restrict__ node_out, const uint2* __restrict__ seg, const int8_t* __restrict__ plane,
     const uint2* __restrict__ med, uint64_t* __restrict__ state, uint32_t* __restrict__ counter, uint32_t n_tiles) {
     __shared__ uint32_t s_tile, s_epoch;
@@ -197,7 +234,7 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
     // is scanned: positions, then their nodes' segments, planes and medians.
     uint32_t val[kPItems], incl[kPItems], node[kPItems];
     uint2 sgk[kPItems];
-    Rec rec[kPItems];
+    Rk rec[kPItems];
 #pragma unroll
     for (int k = 0; k < kPItems; ++k) {
         const uint32_t g = tile * kPTile + (uint32_t)k * kPThreads + t;
@@ -224,7 +261,16 @@ __global__ __launch_bounds__(kPThreads) void k_level_partition(
         uint32_t v = kSegH;
         if (node[k] != kFinal) {
             const uint32_t a = g >= 2 * n ? 2u : (g >= n ? 1u : 0u), j = g - a * n;
-            const int sd = side_of(rec[k], pl[k], md[k]);
+            // the split axis's own order: the position decides; another
+            // order: the record's rank on the split axis against the median's
+            int sd;
+            if (pl[k] == (int8_t)a) {
+                const uint32_t mid = sgk[k].x + (sgk[k].y - sgk[k].x) / 2;
+                sd = j < mid ? 0 : (j == mid ? 1 : 2);
+            } else {
+                const uint32_t rr = rec[k].r[pl[k] == other_axis((int)a, 0) ? 0 : 1];
+                sd = rr < md[k].x ? 0 : (rr == md[k].x ? 1 : 2);
+            }
             v = (j == sgk[k].x ? kSegH : 0u) | (sd == 0 ? 1u : 0u) | (sd == 1 ? kSegM : 0u);
         }
         val[k] = v;
@@ -360,9 +406,15 @@ struct LocalLds {
 
 // pos0[id] = position of photon id in the axis-0 order (for the subtrees'
 // index maps).
-__global__ __launch_bounds__(256) void k_pos0(const Rec* __restrict__ A0, uint32_t n, uint32_t* __restrict__ pos0) {
+// For the LDS levels: ids0[j] = the photon at position j of the axis-0
+// order, and posr0[r] = the position in that order of the photon of axis-0
+// rank r (the order-1 and order-2 records carry that rank).
+__global__ __launch_bounds__(256) void k_pos0(Orders O, uint32_t* __restrict__ ids0, uint32_t* __restrict__ posr0) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) pos0[A0[j].id] = j;
+    if (j >= O.n) return;
+    const uint32_t id = id_at(O, 0, j);
+    ids0[j] = id;
+    posr0[O.rank4[4 * (size_t)id]] = j;
 }
 
 // Exclusive scan of L.flag[0, m) (0/1) into L.pref.
@@ -414,9 +466,9 @@ __device__ __forceinline__ int8_t local_plane(const LocalLds& L, const int (&oi)
     return p;
 }
 
-__global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __restrict__ A0, const Rec* __restrict__ A1,
-                                                                 const Rec* __restrict__ A2,
-                                                                 const uint32_t* __restrict__ pos0, uint32_t first,
+__global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(Orders O, const float4* __restrict__ prec,
+                                                                 const uint32_t* __restrict__ ids0,
+                                                                 const uint32_t* __restrict__ posr0, uint32_t first,
                                                                  uint32_t depth_left, const uint2* __restrict__ gseg,
                                                                  const int8_t* __restrict__ gplane,
                                                                  uint32_t* __restrict__ node_idx,
@@ -426,13 +478,16 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_subtrees(const Rec* __r
     const uint2 rs = gseg[root];
     if (rs.y <= rs.x || rs.y - rs.x > (uint32_t)kLocal) return;  // empty subtree; larger ones never reach here
     const uint32_t m = rs.y - rs.x;
-    // the segment holds the same photons in each order: axes 1 and 2 map their
-    // records to axis-0 positions through pos0
+    // the segment holds the same photons in each order: the photons of the
+    // axis-0 order with their coordinates, and axes 1 and 2 mapped to
+    // axis-0 positions by their records' axis-0 ranks
     for (uint32_t j = t; j < m; j += kLocalThreads) {
-        L.rec[j] = A0[rs.x + j];
+        const uint32_t id = ids0[rs.x + j];
+        const float4 c = prec[3 * (size_t)id];
+        L.rec[j] = Rec{{c.x, c.y, c.z}, id};
         L.ordb[0][j] = (uint16_t)j;
-        L.ordb[1][j] = (uint16_t)(pos0[A1[rs.x + j].id] - rs.x);
-        L.ordb[2][j] = (uint16_t)(pos0[A2[rs.x + j].id] - rs.x);
+        L.ordb[1][j] = (uint16_t)(posr0[O.A[1][rs.x + j].r[0]] - rs.x);
+        L.ordb[2][j] = (uint16_t)(posr0[O.A[2][rs.x + j].r[0]] - rs.x);
         L.node[j] = 0;
     }
     if (t == 0) {
@@ -601,43 +656,45 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     T->cap = cap;
     T->levels = L;
     size_t bytes = pm::carve_size({(size_t)cap * 4, (size_t)cap, (size_t)cap * 36, (size_t)cap * 16});
-    hipError_t e = hipMalloc(&T->mem, bytes);
+    // (a block of the context's cache when one fits, as for grids)
+    hipError_t e = pm::block_take(ctx, bytes, &T->mem, &T->mem_bytes);
     if (e != hipSuccess) {
         delete T;
         return pm::hip_fail(ctx, e, "hipMalloc(kdtree)");
     }
+    T->owner = ctx;
     pm::Carve tc{static_cast<char*>(T->mem)};
     T->node_idx = tc.take<uint32_t>(cap);
     T->node_plane = tc.take<int8_t>(cap);
     T->photons = tc.take<float>((size_t)cap * 9);
     T->nodes = tc.take<float4>(cap);
     auto bail = [&](pm_status s) {
-        (void)hipFree(T->mem);
-        delete T;
+        pm_kdtree_destroy(T);
         return s;
     };
 
     // the three axis sorts run as one segmented sort of 3n keys
     size_t rs = pm::radix_sort_scratch_bytes(n, 3);
-    const size_t rb = (size_t)n * sizeof(Rec), n3 = 3 * (size_t)n;
+    const size_t rb = (size_t)n * sizeof(Rk), n3 = 3 * (size_t)n;
     size_t need = pm::carve_size({rb, rb, rb, rb, rb, rb, (size_t)n * 4, (size_t)n * 4, (size_t)cap * 8, (size_t)cap,
-                                  (size_t)cap * 8, n3 * 4, n3 * 4, n3 * 4, rs, (size_t)n * 48});
+                                  (size_t)cap * 8, n3 * 4, n3 * 4, n3 * 4, (size_t)n * 16, rs, (size_t)n * 48});
     void* work = nullptr;
     pm_status st = pm::scratch_reserve(ctx, need, &work);
     if (st != PM_OK) return bail(st);
     pm::Carve cv{static_cast<char*>(work)};
-    Rec* A[3];
-    Rec* B[3];
-    for (int a = 0; a < 3; ++a) A[a] = cv.take<Rec>(n);
-    for (int a = 0; a < 3; ++a) B[a] = cv.take<Rec>(n);
+    Rk* A[3];
+    Rk* B[3];
+    for (int a = 0; a < 3; ++a) A[a] = cv.take<Rk>(n);
+    for (int a = 0; a < 3; ++a) B[a] = cv.take<Rk>(n);
     uint32_t* node_of = cv.take<uint32_t>(n);
     uint32_t* node_of2 = cv.take<uint32_t>(n);
     uint2* seg = cv.take<uint2>(cap);
     int8_t* plane = cv.take<int8_t>(cap);
     uint2* med = cv.take<uint2>(cap);
-    uint32_t* keys = cv.take<uint32_t>(n3);
-    uint32_t* skeys = cv.take<uint32_t>(n3);
-    uint32_t* order = cv.take<uint32_t>(n3);
+    uint32_t* keys = cv.take<uint32_t>(n3);   // sort keys
+    uint32_t* skeys = cv.take<uint32_t>(n3);  // sorted keys, then ids0 and posr0 (the LDS levels)
+    uint32_t* order = cv.take<uint32_t>(n3);  // each axis's photons in rank order
+    uint32_t* rank4 = cv.take<uint32_t>(4 * (size_t)n);  // each photon's three ranks
     void* rw = cv.take<char>(rs);
     float4* prec = cv.take<float4>((size_t)n * 3);
 
@@ -649,7 +706,18 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
                        keys);
     if ((st = pm::radix_sort_pairs_auto(ctx, keys, nullptr, skeys, order, n, 32, rw, nullptr, 3)) != PM_OK)
         return bail(st);
-    hipLaunchKernelGGL(k_gather_recs, dim3(nb, 3), dim3(256), 0, s, prec, order, n, A[0], A[1], A[2]);
+    hipLaunchKernelGGL(k_ranks, dim3(nb, 3), dim3(256), 0, s, order, n, rank4);
+    hipLaunchKernelGGL(k_rank_recs, dim3(nb, 3), dim3(256), 0, s, order, reinterpret_cast<const uint4*>(rank4), n,
+                       A[0], A[1], A[2]);
+    auto orders = [&] {
+        Orders O;
+        for (int a = 0; a < 3; ++a) O.A[a] = A[a];
+        O.order = order;
+        O.rank4 = rank4;
+        O.ph9 = d_photons9;
+        O.n = n;
+        return O;
+    };
     if ((st = pm::fill_u32(ctx, T->node_idx, kFinal, cap)) != PM_OK) return bail(st);
     // node_plane = -1; the carve pads every block to 256 bytes, so whole
     // words past cap stay inside the allocation.
@@ -665,14 +733,15 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
     // a segment of at most 1024 records spans at most 11 levels (node arrays of kLocal)
     if (L - Lg > 11) return bail(pm::fail(ctx, PM_ERR_INVALID, "pm_kdtree_build: internal level split"));
     // the root's plane (and, above the LDS levels, its median)
-    hipLaunchKernelGGL(k_level_nodes, dim3(1), dim3(256), 0, s, A[0], A[1], A[2], 0u, 1u, cap, n, Lg > 0, seg, plane,
+    hipLaunchKernelGGL(k_level_nodes, dim3(1), dim3(256), 0, s, orders(), 0u, 1u, cap, n, Lg > 0, seg, plane,
                        T->node_idx, T->node_plane, med, nullptr);
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
         uint32_t first = (1u << lvl) - 1, count = 1u << lvl;
         if (lvl == Lg) {
-            hipLaunchKernelGGL(k_pos0, dim3(nb), dim3(256), 0, s, A[0], n, keys);  // keys: free from here on
-            hipLaunchKernelGGL(k_local_subtrees, dim3(count), dim3(kLocalThreads), 0, s, A[0], A[1], A[2], keys, first,
-                               L - lvl, seg, plane, T->node_idx, T->node_plane);
+            uint32_t *ids0 = skeys, *posr0 = skeys + n;
+            hipLaunchKernelGGL(k_pos0, dim3(nb), dim3(256), 0, s, orders(), ids0, posr0);
+            hipLaunchKernelGGL(k_local_subtrees, dim3(count), dim3(kLocalThreads), 0, s, orders(), prec, ids0, posr0,
+                               first, L - lvl, seg, plane, T->node_idx, T->node_plane);
             break;
         }
         if (lvl + 1 == L) break;  // every segment of the last level has size <= 1
@@ -686,8 +755,8 @@ extern "C" pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint3
         std::swap(node_of, node_of2);
         // the children's planes, and their medians above the LDS levels
         uint32_t cf = (1u << (lvl + 1)) - 1, cc = 1u << (lvl + 1);
-        hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, A[0], A[1], A[2], cf, cc, cap,
-                           0u, lvl + 1 < Lg, seg, plane, T->node_idx, T->node_plane, med, lcounter);
+        hipLaunchKernelGGL(k_level_nodes, dim3(pm::div_up(cc, 256)), dim3(256), 0, s, orders(), cf, cc, cap, 0u,
+                           lvl + 1 < Lg, seg, plane, T->node_idx, T->node_plane, med, lcounter);
     }
     hipLaunchKernelGGL(k_tree_photons, dim3(pm::div_up(cap, 256)), dim3(256), 0, s, prec, T->node_idx,
                        T->node_plane, cap, T->photons, T->nodes);
@@ -718,7 +787,7 @@ extern "C" pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* t, uin
 
 extern "C" pm_status pm_kdtree_destroy(pm_kdtree* t) {
     if (!t) return PM_OK;
-    if (t->mem) (void)hipFree(t->mem);
+    if (t->mem && !pm::block_give(t->owner, t->mem, t->mem_bytes)) (void)hipFree(t->mem);
     delete t;
     return PM_OK;
 }

@@ -73,6 +73,8 @@ struct pm_grid {
 struct pm_kdtree {
     uint32_t n = 0, cap = 0, levels = 0;
     void* mem = nullptr;
+    size_t mem_bytes = 0;
+    pm_ctx* owner = nullptr;  // the context whose block cache `mem` returns to
     uint32_t* node_idx = nullptr;  // [cap]
     int8_t* node_plane = nullptr;  // [cap]
     float* photons = nullptr;      // photons9 in node order [9][cap] (empty slots hold +inf positions)

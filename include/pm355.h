@@ -283,6 +283,9 @@ typedef struct {
 pm_status pm_kdtree_build(pm_ctx* ctx, const float* d_photons9, uint32_t n, pm_kdtree** out);
 pm_status pm_kdtree_describe(const pm_kdtree* tree, pm_kdtree_desc* out);
 pm_status pm_kdtree_copy_to_host(pm_ctx* ctx, const pm_kdtree* tree, uint32_t* h_node_idx, int8_t* h_node_plane);
+/* The tree's device memory goes back to the cache of the context that built
+ * it, as pm_grid_destroy's does (work reading the tree on another stream
+ * must have completed). */
 pm_status pm_kdtree_destroy(pm_kdtree* tree);
 
 /* k nearest photons on the kd-tree, the query side of cpm::PhotonMap
