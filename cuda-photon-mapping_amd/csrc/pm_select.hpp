@@ -37,12 +37,21 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += t;
-    }
+// Inclusive wave64 prefix sum in registers (DPP, no LDS): shifts right by
+// 1, 2, 4, 8 lanes inside each row of 16 (a lane with no source adds 0), then
+// row 0's last lane into row 1 and row 2's into row 3 (row_bcast:15), then
+// lane 31 into rows 2 and 3 (row_bcast:31).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_add(uint32_t v) {
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t) {
+    v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+    v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+    v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -90,8 +99,9 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
     uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
     int ls = __builtin_ctzll(wballot(cum >= k));
     uint32_t hstar = 63u - (uint32_t)ls;
-    uint32_t A = __shfl(cum, ls, 64);
-    uint32_t prev = __shfl(cum, ls > 0 ? ls - 1 : 0, 64);
+    // (ls is wave-uniform: read the lanes directly)
+    uint32_t A = __builtin_amdgcn_readlane(cum, ls);
+    uint32_t prev = __builtin_amdgcn_readlane(cum, ls > 0 ? ls - 1 : 0);
     uint64_t khi = (uint64_t)(bT - hstar + 1u) << 54, klo = 0;
     uint32_t below = 0;
     if (hstar < 63u) {
@@ -112,8 +122,8 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
         uint32_t cu = wave_incl_scan(L.hist[lane], lane);
         uint32_t need = k - below;
         int js = __builtin_ctzll(wballot(cu >= need));
-        uint32_t pv = __shfl(cu, js > 0 ? js - 1 : 0, 64);
-        uint32_t at = __shfl(cu, js, 64);
+        uint32_t pv = __builtin_amdgcn_readlane(cu, js > 0 ? js - 1 : 0);
+        uint32_t at = __builtin_amdgcn_readlane(cu, js);
         uint64_t nlo = klo + ((uint64_t)js << s);
         uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
         if (nhi > khi) nhi = khi;
