@@ -313,12 +313,9 @@ __device__ __forceinline__ void radiance_epilogue(const KnnParams& P, const uint
             }
         }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        res0 += __shfl_xor(res0, d, 64);
-        res1 += __shfl_xor(res1, d, 64);
-        res2 += __shfl_xor(res2, d, 64);
-    }
+    res0 = wave_sum_f32(res0);
+    res1 = wave_sum_f32(res1);
+    res2 = wave_sum_f32(res2);
     if (lane == 0) {
         float coef = 1.f;
         if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
@@ -378,8 +375,7 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
     const float inv_denom = 1.16529458f, log2e = 1.44269504f;
     float dmax = 0.f;
     for (uint32_t e = lane; e < m; e += 64) dmax = fmaxf(dmax, __uint_as_float((uint32_t)(L.buf[e] >> 32)));
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, d, 64));
+    dmax = wave_max_nonneg_f32(dmax);
     const float rmax = sqrtf(dmax);
     float res0 = 0.f, res1 = 0.f, res2 = 0.f;
     if (m > 0) {
@@ -401,12 +397,9 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
             }
         }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        res0 += __shfl_xor(res0, d, 64);
-        res1 += __shfl_xor(res1, d, 64);
-        res2 += __shfl_xor(res2, d, 64);
-    }
+    res0 = wave_sum_f32(res0);
+    res1 = wave_sum_f32(res1);
+    res2 = wave_sum_f32(res2);
     if (lane == 0) {
         float coef = 1.f;
         if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
@@ -672,12 +665,9 @@ __device__ __forceinline__ void large_one(const KnnParams& P, WaveLds<CAP>& L, u
                 }
             }
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            res0 += __shfl_xor(res0, d, 64);
-            res1 += __shfl_xor(res1, d, 64);
-            res2 += __shfl_xor(res2, d, 64);
-        }
+        res0 = wave_sum_f32(res0);
+        res1 = wave_sum_f32(res1);
+        res2 = wave_sum_f32(res2);
         if (lane == 0) {
             float coef = 1.f;
             if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
@@ -836,8 +826,7 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
         kstar = ((uint64_t)hi << 32) | lo;
         rmax2 = __uint_as_float(hi);
     } else {
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, d, 64));
+        dmax = wave_max_nonneg_f32(dmax);
         rmax2 = dmax;
     }
     if (P.out_count && lane == 0) P.out_count[q] = m;
@@ -917,12 +906,9 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
         take(qn);
         consume();
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        res0 += __shfl_xor(res0, d, 64);
-        res1 += __shfl_xor(res1, d, 64);
-        res2 += __shfl_xor(res2, d, 64);
-    }
+    res0 = wave_sum_f32(res0);
+    res1 = wave_sum_f32(res1);
+    res2 = wave_sum_f32(res2);
     if (lane == 0) {
         float coef = 1.f;
         if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
@@ -1075,12 +1061,9 @@ __global__ __launch_bounds__(256) void k_radiance(KnnParams P) {
             }
         }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        res0 += __shfl_xor(res0, d, 64);
-        res1 += __shfl_xor(res1, d, 64);
-        res2 += __shfl_xor(res2, d, 64);
-    }
+    res0 = wave_sum_f32(res0);
+    res1 = wave_sum_f32(res1);
+    res2 = wave_sum_f32(res2);
     if (lane == 0) {
         float coef = m > 0 ? __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f)) : 0.f;
         P.rgb[q] = res0 * coef;

@@ -55,6 +55,32 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t) {
     return v;
 }
 
+// Wave64 reductions of floats by the same DPP steps, read back from lane 63
+// (wave-uniform): the sum is associated as the scan adds it; the maximum is
+// of nonnegative values (a lane with no source contributes 0).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_src(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += dpp_src<0x111, 0xf>(v);
+    v += dpp_src<0x112, 0xf>(v);
+    v += dpp_src<0x114, 0xf>(v);
+    v += dpp_src<0x118, 0xf>(v);
+    v += dpp_src<0x142, 0xa>(v);
+    v += dpp_src<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_max_nonneg_f32(float v) {
+    v = fmaxf(v, dpp_src<0x111, 0xf>(v));
+    v = fmaxf(v, dpp_src<0x112, 0xf>(v));
+    v = fmaxf(v, dpp_src<0x114, 0xf>(v));
+    v = fmaxf(v, dpp_src<0x118, 0xf>(v));
+    v = fmaxf(v, dpp_src<0x142, 0xa>(v));
+    v = fmaxf(v, dpp_src<0x143, 0xc>(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
     uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
