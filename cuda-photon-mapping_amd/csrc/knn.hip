@@ -132,7 +132,8 @@ __device__ __forceinline__ bool row_span(const GridGeom& G, const uint32_t* __re
         const float gy = axis_gap(qy, y, G.g[1], G.mn[1], G.cell), gz = axis_gap(qz, z, G.g[2], G.mn[2], G.cell);
         const float dyz2 = gy * gy + gz * gz;
         if (!(dyz2 <= thr)) return false;
-        const float dx = sqrtf(thr - dyz2 + 1e-3f * T);
+        // (the hardware square root: its ulp is far inside the margins)
+        const float dx = __builtin_amdgcn_sqrtf(thr - dyz2 + 1e-3f * T);
         const float e = (fabsf(qx) + dx) * 1e-6f;
         const int xl = max(B.lo[0], pmd::cell_coord(qx - dx - e, G.mn[0], G.cell, G.g[0]));
         const int xh = min(B.hi[0], pmd::cell_coord(qx + dx + e, G.mn[0], G.cell, G.g[0]));
@@ -362,10 +363,12 @@ __device__ __forceinline__ void sort_and_emit(const KnnParams& P, WaveLds<CAP>& 
 //
 // The filter weight w = alpha (1 - (1 - e^{-beta d^2 / (2 r^2)}) / (1 - e^{-beta}))
 // is evaluated as A + B e^{x} with x = d^2 (-beta / (2 r^2)) from the squared
-// distance itself (no square root), per-query reciprocals and the hardware
-// exponential; against the reference's FP32 sequence of square root,
-// divisions and expf this moves each weight by a few 1e-7 relative, inside
-// the estimate's 1e-5 bound. The cos > 0 test keeps the reference's rounding.
+// distance itself (no square root), the hardware exponential, and the
+// filter radius and the per-query factors (-beta / (2 r^2), 1 / (pi r^2))
+// from the hardware square root and reciprocal; against the reference's FP32
+// sequence of square roots, divisions and expf this moves the estimate by a
+// few 1e-7 relative, inside its 1e-5 bound. The cos > 0 test keeps the
+// reference's rounding.
 template <int CAP>
 __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLds<CAP>& L, uint32_t lane, uint32_t q,
                                                 uint32_t m) {
@@ -376,12 +379,14 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
     float dmax = 0.f;
     for (uint32_t e = lane; e < m; e += 64) dmax = fmaxf(dmax, __uint_as_float((uint32_t)(L.buf[e] >> 32)));
     dmax = wave_max_nonneg_f32(dmax);
-    const float rmax = sqrtf(dmax);
+    // hardware square root and reciprocals (a few 1e-7 relative on the
+    // estimate, inside its 1e-5 bound)
+    const float rmax = __builtin_amdgcn_sqrtf(dmax);
     float res0 = 0.f, res1 = 0.f, res2 = 0.f;
     if (m > 0) {
         const float filter_r = __fmul_rn(rmax, rmax);
         // x log2(e) = d^2 * s
-        const float s = __fdiv_rn(-beta, 2.f * filter_r) * log2e;
+        const float s = -beta * __builtin_amdgcn_rcpf(2.f * filter_r) * log2e;
         const float wa = alpha * (1.f - inv_denom), wb = alpha * inv_denom;
         const float nx = P.nrm[q], ny = P.nrm[P.nq + q], nz = P.nrm[2 * (size_t)P.nq + q];
         for (uint32_t e = lane; e < m; e += 64) {
@@ -402,7 +407,7 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
     res2 = wave_sum_f32(res2);
     if (lane == 0) {
         float coef = 1.f;
-        if (m > 0) coef = __fdiv_rn(1.f, __fmul_rn(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)), 1.f));
+        if (m > 0) coef = __builtin_amdgcn_rcpf(__fmul_rn(pi_f, __fmul_rn(rmax, rmax)));
         P.rgb[q] = m > 0 ? res0 * coef : 0.f;
         P.rgb[P.nq + q] = m > 0 ? res1 * coef : 0.f;
         P.rgb[2 * (size_t)P.nq + q] = m > 0 ? res2 * coef : 0.f;
