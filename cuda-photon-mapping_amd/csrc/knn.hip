@@ -236,11 +236,21 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
         int cs = __builtin_ctzll(rows);
         rows &= rows - 1;
         uint32_t rb = __builtin_amdgcn_readlane(b, cs), rl = __builtin_amdgcn_readlane(len, cs), cc = 0;
-        bool va0 = lane < rl, vb0 = lane + 64 < rl;
-        uint32_t o0 = (rb + lane) * 4u;
-        float xa0 = bload(rx, o0), ya0 = bload(ry, o0), za0 = bload(rz, o0);
-        float xb0 = bload(rx, o0 + 256u), yb0 = bload(ry, o0 + 256u), zb0 = bload(rz, o0 + 256u);
-        for (;;) {
+        // Two register sets of a step's candidates: each half of the loop
+        // body processes one and loads the next step into the other, so
+        // the prefetched positions are never copied.
+        struct Step {
+            bool va, vb;
+            uint32_t o;
+            float xa, ya, za, xb, yb, zb;
+        };
+        Step s0, s1;
+        s0.va = lane < rl;
+        s0.vb = lane + 64 < rl;
+        s0.o = (rb + lane) * 4u;
+        s0.xa = bload(rx, s0.o), s0.ya = bload(ry, s0.o), s0.za = bload(rz, s0.o);
+        s0.xb = bload(rx, s0.o + 256u), s0.yb = bload(ry, s0.o + 256u), s0.zb = bload(rz, s0.o + 256u);
+        auto half = [&](const Step& cur, Step& nxt) -> bool {
             bool has_next = true;
             uint32_t ncc = cc + 128, nrb = rb, nrl = rl;
             if (ncc >= rl) {
@@ -255,24 +265,20 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
                 }
             }
             // (after the last step this reads past the row once, harmlessly)
-            const bool va1 = ncc + lane < nrl, vb1 = ncc + 64 + lane < nrl;
-            const uint32_t o1 = (nrb + ncc + lane) * 4u;
-            const float xa1 = bload(rx, o1), ya1 = bload(ry, o1), za1 = bload(rz, o1);
-            const float xb1 = bload(rx, o1 + 256u), yb1 = bload(ry, o1 + 256u), zb1 = bload(rz, o1 + 256u);
-            f(va0, o0 >> 2, dist2(xa0, ya0, za0, qx, qy, qz), vb0, (o0 >> 2) + 64u, dist2(xb0, yb0, zb0, qx, qy, qz));
-            if (!has_next) break;
+            nxt.va = ncc + lane < nrl;
+            nxt.vb = ncc + 64 + lane < nrl;
+            nxt.o = (nrb + ncc + lane) * 4u;
+            nxt.xa = bload(rx, nxt.o), nxt.ya = bload(ry, nxt.o), nxt.za = bload(rz, nxt.o);
+            nxt.xb = bload(rx, nxt.o + 256u), nxt.yb = bload(ry, nxt.o + 256u), nxt.zb = bload(rz, nxt.o + 256u);
+            f(cur.va, cur.o >> 2, dist2(cur.xa, cur.ya, cur.za, qx, qy, qz), cur.vb, (cur.o >> 2) + 64u,
+              dist2(cur.xb, cur.yb, cur.zb, qx, qy, qz));
+            if (!has_next) return false;
             cc = __builtin_amdgcn_readfirstlane(ncc);
             rb = __builtin_amdgcn_readfirstlane(nrb);
             rl = __builtin_amdgcn_readfirstlane(nrl);
-            va0 = va1;
-            vb0 = vb1;
-            o0 = o1;
-            xa0 = xa1;
-            ya0 = ya1;
-            za0 = za1;
-            xb0 = xb1;
-            yb0 = yb1;
-            zb0 = zb1;
+            return true;
+        };
+        while (half(s0, s1) && half(s1, s0)) {
         }
     }
 }
