@@ -216,9 +216,15 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
     }
 }
 
-// walk with 128-candidate steps: lane l holds candidates l and l + 64 of
-// the step, so a step's scalar cursor and bookkeeping serve two candidates
-// per lane. f(va, sa, d2a, vb, sb, d2b).
+// walk with 128-candidate steps over the runs laid end to end: a step is
+// the next 128 positions of the concatenated runs (lane l holds positions
+// p + l and p + 64 + l), so runs shorter than a step share one instead of
+// leaving lanes idle, and the candidates arrive in the same order as run by
+// run. Position t of run j (its stream start pre_j, first slot b_j) is slot
+// b_j + t - pre_j: each lane carries the byte offset 4 (b_j - pre_j) of the
+// run holding its position, updated only for the runs that start inside the
+// step. The cursor (step start, next run, its start) is wave-uniform, in
+// SGPRs. f(va, sa, d2a, vb, sb, d2b).
 template <class F>
 __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
                                       F&& f) {
@@ -226,57 +232,60 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
     const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
     const int nseg = linear ? ny * nz : nx * ny * nz;
     const __amdgpu_buffer_rsrc_t rx = plane_rsrc(P.px, P.n), ry = plane_rsrc(P.py, P.n), rz = plane_rsrc(P.pz, P.n);
+    const int ilane = (int)lane;
+    const uint32_t lane4 = lane * 4u;
     for (int sbase = 0; sbase < nseg; sbase += 64) {
-        const int s = sbase + (int)lane;
+        const int s = sbase + ilane;
         uint32_t b = 0, len = 0;
         if (s < nseg) row_span(P.G, P.starts, P.sizes, B, s, qx, qy, qz, P.T, b, len);
-        const int ns = min(64, nseg - sbase);
-        uint64_t rows = wballot(len > 0) & (ns == 64 ? ~0ull : ((1ull << ns) - 1));
-        if (!rows) continue;
-        int cs = __builtin_ctzll(rows);
-        rows &= rows - 1;
-        uint32_t rb = __builtin_amdgcn_readlane(b, cs), rl = __builtin_amdgcn_readlane(len, cs), cc = 0;
-        // Two register sets of a step's candidates: each half of the loop
-        // body processes one and loads the next step into the other, so
-        // the prefetched positions are never copied.
+        const uint32_t incl = wave_incl_scan(len, lane);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+        if (total == 0) continue;
+        const uint32_t pre = incl - len;
+        const uint32_t d4 = (b - pre) * 4u;  // mod 2^32: (d4 + 4 t) is 4 x slot
+        uint64_t rows = wballot(len > 0);
+        // the first non-empty run starts the stream at 0
+        uint32_t nstart = 0, cur = 0;
         struct Step {
             bool va, vb;
-            uint32_t o;
+            uint32_t oa, ob;
             float xa, ya, za, xb, yb, zb;
         };
-        Step s0, s1;
-        s0.va = lane < rl;
-        s0.vb = lane + 64 < rl;
-        s0.o = (rb + lane) * 4u;
-        s0.xa = bload(rx, s0.o), s0.ya = bload(ry, s0.o), s0.za = bload(rz, s0.o);
-        s0.xb = bload(rx, s0.o + 256u), s0.yb = bload(ry, s0.o + 256u), s0.zb = bload(rz, s0.o + 256u);
-        auto half = [&](const Step& cur, Step& nxt) -> bool {
-            bool has_next = true;
-            uint32_t ncc = cc + 128, nrb = rb, nrl = rl;
-            if (ncc >= rl) {
-                if (!rows) {
-                    has_next = false;
-                } else {
-                    int ncs = __builtin_ctzll(rows);
-                    rows &= rows - 1;
-                    nrb = __builtin_amdgcn_readlane(b, ncs);
-                    nrl = __builtin_amdgcn_readlane(len, ncs);
-                    ncc = 0;
-                }
+        // offsets and positions of the step at p; the runs starting before
+        // p + 128 are entered (lanes at or past a run's start take its offset)
+        auto load = [&](Step& w, uint32_t p) {
+            uint32_t da = cur, db = cur;
+            while (rows && nstart < p + 128u) {
+                const int j = __builtin_ctzll(rows);
+                rows &= rows - 1;
+                const uint32_t dj = __builtin_amdgcn_readlane(d4, j);
+                const int t0 = (int)(nstart - p);
+                da = ilane >= t0 ? dj : da;
+                db = ilane >= t0 - 64 ? dj : db;
+                cur = dj;
+                nstart = rows ? __builtin_amdgcn_readlane(pre, __builtin_ctzll(rows)) : 0xffffffffu;
             }
-            // (after the last step this reads past the row once, harmlessly)
-            nxt.va = ncc + lane < nrl;
-            nxt.vb = ncc + 64 + lane < nrl;
-            nxt.o = (nrb + ncc + lane) * 4u;
-            nxt.xa = bload(rx, nxt.o), nxt.ya = bload(ry, nxt.o), nxt.za = bload(rz, nxt.o);
-            nxt.xb = bload(rx, nxt.o + 256u), nxt.yb = bload(ry, nxt.o + 256u), nxt.zb = bload(rz, nxt.o + 256u);
-            f(cur.va, cur.o >> 2, dist2(cur.xa, cur.ya, cur.za, qx, qy, qz), cur.vb, (cur.o >> 2) + 64u,
-              dist2(cur.xb, cur.yb, cur.zb, qx, qy, qz));
-            if (!has_next) return false;
-            cc = __builtin_amdgcn_readfirstlane(ncc);
-            rb = __builtin_amdgcn_readfirstlane(nrb);
-            rl = __builtin_amdgcn_readfirstlane(nrl);
-            return true;
+            const int left = (int)(total - p);
+            w.va = ilane < left;
+            w.vb = ilane < left - 64;
+            w.oa = da + lane4 + p * 4u;
+            w.ob = db + lane4 + p * 4u + 256u;
+            w.xa = bload(rx, w.oa), w.ya = bload(ry, w.oa), w.za = bload(rz, w.oa);
+            w.xb = bload(rx, w.ob), w.yb = bload(ry, w.ob), w.zb = bload(rz, w.ob);
+        };
+        // Two register sets: each half of the loop body loads the next step
+        // into one set while the other is processed, so nothing is copied.
+        Step s0, s1;
+        uint32_t p = 0;
+        load(s0, 0);
+        auto half = [&](const Step& c, Step& nxt) -> bool {
+            const uint32_t pn = p + 128u;
+            const bool more = pn < total;
+            if (more) load(nxt, pn);
+            f(c.va, c.oa >> 2, dist2(c.xa, c.ya, c.za, qx, qy, qz), c.vb, c.ob >> 2,
+              dist2(c.xb, c.yb, c.zb, qx, qy, qz));
+            p = pn;
+            return more;
         };
         while (half(s0, s1) && half(s1, s0)) {
         }
@@ -398,13 +407,16 @@ __device__ __forceinline__ void radiance_buffer(const KnnParams& P, const WaveLd
         for (uint32_t e = lane; e < m; e += 64) {
             const uint64_t v = L.buf[e];
             const uint32_t idx = (uint32_t)v;
+            // power is read with the direction (one memory round trip),
+            // and used only where the photon faces the surface
             const float dx = P.dir[idx], dy = P.dir[P.n + idx], dz = P.dir[2 * (size_t)P.n + idx];
+            const float p0 = P.power[idx], p1 = P.power[P.n + idx], p2 = P.power[2 * (size_t)P.n + idx];
             const float cosNL = __fadd_rn(__fadd_rn(__fmul_rn(nx, -dx), __fmul_rn(ny, -dy)), __fmul_rn(nz, -dz));
             if (cosNL > 0.f) {
                 const float w = wa + wb * __builtin_amdgcn_exp2f(__uint_as_float((uint32_t)(v >> 32)) * s);
-                res0 += P.power[idx] * w;
-                res1 += P.power[P.n + idx] * w;
-                res2 += P.power[2 * (size_t)P.n + idx] * w;
+                res0 += p0 * w;
+                res1 += p1 * w;
+                res2 += p2 * w;
             }
         }
     }
@@ -427,8 +439,19 @@ __device__ __forceinline__ void finish(const KnnParams& P, WaveLds<CAP>& L, uint
     const uint32_t m = found < k ? found : k;
     if (!P.out_idx) {
         // no neighbour lists: narrow to exactly the k smallest and estimate
-        // from the buffer without sorting
-        cnt = buf_select<CAP>(L, lane, cnt, k, k, P.bT, khi);
+        // from the buffer without sorting (up to 256 keys: selected from
+        // registers, the keys read once)
+        if (cnt <= 256) {
+            uint64_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = (uint32_t)j * 64 + lane;
+                v[j] = e < cnt ? L.buf[e] : ~0ull;
+            }
+            cnt = reg_select<4, CAP>(L, lane, v, cnt, k, P.bT, khi);
+        } else {
+            cnt = buf_select<CAP>(L, lane, cnt, k, k, P.bT, khi);
+        }
         wave_sync();
         if (P.out_count && lane == 0) P.out_count[q] = m;
         if (P.rgb) radiance_buffer<CAP>(P, L, lane, q, m);
@@ -518,6 +541,32 @@ __device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k
     cnt += na + nb;
 }
 
+// Per-query set-up of the streaming gather: the query's position (x, y,
+// z: set by the caller), its cell, and with the inner box the d2 bits below
+// which a key is closer than every photon outside it (less a few ulps of
+// rounding: every such photon has d2 >= gap^2). Uniform inputs give uniform
+// results; a wave that serves several queries computes theirs one per lane.
+struct QSetup {
+    uint32_t q;
+    float x, y, z;
+    int c[3];
+    uint32_t gbits;
+};
+__device__ __forceinline__ void query_setup(const KnnParams& P, QSetup& S) {
+    const float qv[3] = {S.x, S.y, S.z};
+    float gap = __builtin_inff();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int c = pmd::cell_coord(qv[a], P.G.mn[a], P.G.cell, P.G.g[a]);
+        S.c[a] = c;
+        if (P.h > 0) {
+            const int lo = max(0, c - P.h), hi = min(P.G.g[a] - 1, c + P.h);
+            gap = fminf(gap, inner_gap(qv[a], lo, hi, P.G.g[a], P.G.mn[a], P.G.cell));
+        }
+    }
+    S.gbits = P.h > 0 ? min(__float_as_uint(gap * gap * 0.999f), __float_as_uint(P.T) + 1u) : 0u;
+}
+
 // Single pass: in-radius keys stream into an LDS buffer of CAP keys; when it
 // would overflow, buf_select keeps only the smallest bin range holding >= k
 // keys and tightens the acceptance bound. Exact: a dropped key has >= k kept
@@ -529,29 +578,28 @@ __device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k
 // be, those k keys are the answer (m = k); otherwise the query walks the
 // whole box again from an empty buffer. Both outcomes are exact.
 template <int CAP>
-__device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
-    Box B;
-    float qx, qy, qz;
-    int qc[3];
-    box_of(P, q, B, qx, qy, qz, qc);
+__device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, const QSetup& S) {
+    const uint32_t q = S.q;
+    const float qx = S.x, qy = S.y, qz = S.z;
     const float T = P.T;
     const uint32_t k = (uint32_t)P.k;
-    // the inner box first when asked for; gbits: the d2 bits below which a
-    // key is closer than every photon outside it (less a few ulps of
-    // rounding: every such photon has d2 >= gap^2)
+    // the candidate box: the query's cell +- rc, clamped (PhotonGrid.cu:262-272)
+    Box B;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        B.lo[a] = max(0, S.c[a] - P.rc);
+        B.hi[a] = min(P.G.g[a] - 1, S.c[a] + P.rc);
+    }
+    // the inner box (the query's cell +- h) first when asked for
     Box W = B;
-    uint32_t gbits = 0;
+    const uint32_t gbits = S.gbits;
     bool inner = P.h > 0;
     if (inner) {
-        const float qv[3] = {qx, qy, qz};
-        float gap = __builtin_inff();
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            W.lo[a] = max(0, qc[a] - P.h);
-            W.hi[a] = min(P.G.g[a] - 1, qc[a] + P.h);
-            gap = fminf(gap, inner_gap(qv[a], W.lo[a], W.hi[a], P.G.g[a], P.G.mn[a], P.G.cell));
+            W.lo[a] = max(0, S.c[a] - P.h);
+            W.hi[a] = min(P.G.g[a] - 1, S.c[a] + P.h);
         }
-        gbits = __builtin_amdgcn_readfirstlane(min(__float_as_uint(gap * gap * 0.999f), __float_as_uint(T) + 1u));
     }
     uint32_t cnt, found;
     uint64_t khi;
@@ -575,6 +623,20 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
         wave_sync();
     }
     finish<CAP>(P, L, lane, q, cnt, found, khi);
+}
+
+// One query on the wave (its set-up wave-uniform).
+template <int CAP>
+__device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
+    // the query is the wave's: its coordinates (and so the box, the row
+    // count and the cursor of walk) are wave-uniform, held in SGPRs
+    QSetup S;
+    S.q = q;
+    S.x = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qx[q])));
+    S.y = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qy[q])));
+    S.z = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qz[q])));
+    query_setup(P, S);
+    stream_core<CAP>(P, L, lane, S);
 }
 
 // 512-key buffer: 17 KB of LDS per workgroup, so 7 waves per SIMD fit when

@@ -174,6 +174,73 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
     return out;
 }
 
+// buf_select to exactly the k smallest keys, for at most 64 R buffered keys
+// read into registers once (v[j] = entry 64 j + lane, ~0 past cnt): the same
+// bins and rounds, counted from the registers instead of re-reading the
+// buffer, so the bound and the kept set are those of buf_select. The kept
+// keys are then written back to the buffer's front in buffer order (one
+// pass, every read already done). Returns the count kept (min(cnt, k)).
+template <int R, int CAP>
+__device__ uint32_t reg_select(WaveLds<CAP>& L, uint32_t lane, const uint64_t (&v)[R], uint32_t cnt, uint32_t k,
+                               uint32_t bT, uint64_t& khi_io) {
+    if (cnt <= k) return cnt;
+    L.hist[lane] = 0;
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        if ((uint32_t)j * 64 + lane < cnt) {
+            const uint32_t h = bT - (uint32_t)(v[j] >> 54);
+            atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
+        }
+    }
+    wave_sync();
+    const uint32_t cum = wave_incl_scan(L.hist[63 - lane], lane);
+    const int ls = __builtin_ctzll(wballot(cum >= k));
+    const uint32_t hstar = 63u - (uint32_t)ls;
+    uint32_t A = __builtin_amdgcn_readlane(cum, ls);
+    const uint32_t prev = __builtin_amdgcn_readlane(cum, ls > 0 ? ls - 1 : 0);
+    uint64_t khi = (uint64_t)(bT - hstar + 1u) << 54, klo = 0;
+    uint32_t below = 0;
+    if (hstar < 63u) {
+        klo = (uint64_t)(bT - hstar) << 54;
+        below = ls > 0 ? prev : 0u;
+    }
+    if (khi > khi_io) khi = khi_io;
+    while (A > k) {
+        const uint64_t width = khi - klo;
+        const int s = width <= 64 ? 0 : 64 - __builtin_clzll(width - 1) - 6;
+        L.hist[lane] = 0;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (v[j] >= klo && v[j] < khi) atomicAdd(&L.hist[(uint32_t)((v[j] - klo) >> s)], 1u);
+        wave_sync();
+        const uint32_t cu = wave_incl_scan(L.hist[lane], lane);
+        const uint32_t need = k - below;
+        const int js = __builtin_ctzll(wballot(cu >= need));
+        const uint32_t pv = __builtin_amdgcn_readlane(cu, js > 0 ? js - 1 : 0);
+        const uint32_t at = __builtin_amdgcn_readlane(cu, js);
+        const uint64_t nlo = klo + ((uint64_t)js << s);
+        uint64_t nhi = klo + ((uint64_t)(js + 1) << s);
+        if (nhi > khi) nhi = khi;
+        A = below + at;
+        below += js > 0 ? pv : 0u;
+        klo = nlo;
+        khi = nhi;
+    }
+    // the kept keys (invalid entries are ~0, never below the bound), in order
+    uint32_t out = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const bool keep = v[j] < khi;
+        const uint64_t bal = wballot(keep);
+        if (keep) L.buf[out + mbcnt(bal)] = v[j];
+        out += (uint32_t)__popcll(bal);
+    }
+    khi_io = khi;
+    return out;
+}
+
 // Ascending bitonic sort of buf[0, n) in LDS by one wave; n is a power of two.
 __device__ __forceinline__ void lds_bitonic_sort(uint64_t* buf, uint32_t n, uint32_t lane) {
     for (uint32_t size = 2; size <= n; size <<= 1) {
