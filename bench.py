@@ -59,6 +59,27 @@ def native_oracle():
         return f"g++ -O3 -ffp-contract=off -fopenmp, portable build (the -march=native build failed: {e})"
 
 
+def insts_per_query(cs):
+    """Wave instructions per query of the profiled gather launch (SQ_INSTS_*
+    over SQ_WAVES; the launch runs one wave per query)."""
+    m = (cs or {}).get("per_dispatch_median", {})
+    w = m.get("SQ_WAVES")
+    if not w:
+        return None
+    names = {"valu": "SQ_INSTS_VALU", "salu": "SQ_INSTS_SALU", "lds": "SQ_INSTS_LDS",
+             "vmem_rd": "SQ_INSTS_VMEM_RD", "smem": "SQ_INSTS_SMEM"}
+    return {k: m[c] / w for k, c in names.items() if c in m}
+
+
+def valu_lane_util(cs):
+    """Active lanes per VALU instruction over 64 (SQ_THREAD_CYCLES_VALU over
+    64 x SQ_ACTIVE_INST_VALU)."""
+    m = (cs or {}).get("per_dispatch_median", {})
+    if m.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in m:
+        return m["SQ_THREAD_CYCLES_VALU"] / (64.0 * m["SQ_ACTIVE_INST_VALU"])
+    return None
+
+
 def cpu_baseline(ph, q, qn, aabb, r, k, build, run_s=3.0, runs=5):
     """Oracle (C++ restatement of the reference CPU path) on the host cores:
     grid build once, then kNN + radiance over a seeded sample of the queries
@@ -1042,11 +1063,11 @@ def main():
             "l1_hit_rate": cs.get("l1_hit_rate") if cs else None,
             "l2_read_latency_cycles": cs.get("l2_read_latency_cycles") if cs else None,
             "counters_stale": cs["stale"] if cs else None,
-            # VALU issue: the profiled launch's wave64 VALU instructions at 4
-            # cycles each (16 lanes per SIMD) over this launch's SIMD cycles
-            # (1024 SIMDs at 2.4 GHz)
-            "valu_issue_frac": (cs["per_dispatch_median"]["SQ_INSTS_VALU"] * 4 / (t_s * 2.4e9 * 1024)
-                                if cs and "SQ_INSTS_VALU" in cs.get("per_dispatch_median", {}) else None),
+            # the profiled launch's instructions per query (one wave per
+            # query) and the VALU lane utilisation (active lanes per VALU
+            # instruction / 64)
+            "insts_per_query": insts_per_query(cs),
+            "valu_lane_util": valu_lane_util(cs),
             "walked_bytes": walked_bytes,
             "alg_bytes": alg_bytes,
             "alg_bytes_definition": "sum over queries of 12 + 8 C_q + 12 P_q + 32 kept + 24 (reference box at "
