@@ -542,29 +542,46 @@ __device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k
 }
 
 // Per-query set-up of the streaming gather: the query's position (x, y,
-// z: set by the caller), its cell, and with the inner box the d2 bits below
-// which a key is closer than every photon outside it (less a few ulps of
-// rounding: every such photon has d2 >= gap^2). Uniform inputs give uniform
-// results; a wave that serves several queries computes theirs one per lane.
+// z: set by the caller, wave-uniform), its cell, and with the inner box the
+// d2 bits below which a key is closer than every photon outside it (less a
+// few ulps of rounding: every such photon has d2 >= gap^2). The three axes
+// are worked side by side in lanes 0..2 (the cell hash's division and the
+// inner box's two face gaps, each axis by the same formulas as one at a
+// time) and read back; the gap is the exact minimum over the axes.
 struct QSetup {
     uint32_t q;
     float x, y, z;
     int c[3];
     uint32_t gbits;
 };
-__device__ __forceinline__ void query_setup(const KnnParams& P, QSetup& S) {
-    const float qv[3] = {S.x, S.y, S.z};
-    float gap = __builtin_inff();
+__device__ __forceinline__ void query_setup(const KnnParams& P, QSetup& S, uint32_t lane) {
+    // (the query's coordinates and the per-axis grid values are taken as
+    // wave-uniform values first, so the lane selects below stay selects and
+    // never become an indexed load from memory)
+    const float mn0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.G.mn[0])));
+    const float mn1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.G.mn[1])));
+    const float mn2 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.G.mn[2])));
+    const int g0 = __builtin_amdgcn_readfirstlane(P.G.g[0]), g1 = __builtin_amdgcn_readfirstlane(P.G.g[1]),
+              g2 = __builtin_amdgcn_readfirstlane(P.G.g[2]);
+    const float x = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(S.x)));
+    const float y = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(S.y)));
+    const float z = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(S.z)));
+    const int a = lane < 3u ? (int)lane : 0;
+    const float v = a == 0 ? x : a == 1 ? y : z;
+    const float mn = a == 0 ? mn0 : a == 1 ? mn1 : mn2;
+    const int g = a == 0 ? g0 : a == 1 ? g1 : g2;
+    const int c = pmd::cell_coord(v, mn, P.G.cell, g);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int c = pmd::cell_coord(qv[a], P.G.mn[a], P.G.cell, P.G.g[a]);
-        S.c[a] = c;
-        if (P.h > 0) {
-            const int lo = max(0, c - P.h), hi = min(P.G.g[a] - 1, c + P.h);
-            gap = fminf(gap, inner_gap(qv[a], lo, hi, P.G.g[a], P.G.mn[a], P.G.cell));
-        }
+    for (int i = 0; i < 3; ++i) S.c[i] = __builtin_amdgcn_readlane(c, i);
+    S.gbits = 0u;
+    if (P.h > 0) {
+        const int lo = max(0, c - P.h), hi = min(g - 1, c + P.h);
+        const uint32_t gb = __float_as_uint(inner_gap(v, lo, hi, g, mn, P.G.cell));
+        const float gap = fminf(fminf(__uint_as_float(__builtin_amdgcn_readlane(gb, 0)),
+                                      __uint_as_float(__builtin_amdgcn_readlane(gb, 1))),
+                                __uint_as_float(__builtin_amdgcn_readlane(gb, 2)));
+        S.gbits = min(__float_as_uint(gap * gap * 0.999f), __float_as_uint(P.T) + 1u);
     }
-    S.gbits = P.h > 0 ? min(__float_as_uint(gap * gap * 0.999f), __float_as_uint(P.T) + 1u) : 0u;
 }
 
 // Single pass: in-radius keys stream into an LDS buffer of CAP keys; when it
@@ -635,7 +652,7 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
     S.x = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qx[q])));
     S.y = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qy[q])));
     S.z = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qz[q])));
-    query_setup(P, S);
+    query_setup(P, S, lane);
     stream_core<CAP>(P, L, lane, S);
 }
 
