@@ -677,6 +677,12 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
         # every repetition of the form, as a serving process keeps them
         def fn():
             if not servers or servers[-1][0] != workers:
+                # the previous form's server (and the blocks it cached) goes
+                # first: each form's default budget sees the same free memory
+                while servers:
+                    servers.pop()[1].close()
+                ctx.trim()
+                torch.cuda.empty_cache()
                 srv = ctx.server()
                 if workers:
                     srv.set_workers(workers)
@@ -775,6 +781,10 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
             r = float(rng.uniform(0.01, 0.05))
             jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 20, frames=int(rng.integers(1, 31)), r=r,
                              seed=int(rng.integers(1, 1 << 62))))
+        # every configuration's server sees the same free memory (its
+        # default budget is a share of it)
+        ctx.trim()
+        torch.cuda.empty_cache()
         srv = ctx.server()
         if workers:
             srv.set_workers(workers)
