@@ -117,26 +117,58 @@ __device__ __forceinline__ float inner_gap(float v, int lo, int hi, int g, float
 // photon passing the FP32 test d2 <= T is never cut: the squared gaps are
 // compared with T (1 + 1e-3) (rounding of d2 and of the gaps is a few ulps),
 // and the slab half-width is sqrt(T (1 + 1e-3) - gap_yz^2 + 1e-3 T) plus a
-// relative 1e-6 before it is hashed with the grid's own monotone cell_coord.
+// relative 1e-6 before its cells are bounded (cell_bounds, conservative
+// against the grid's own monotone cell_coord).
 // The kept candidates are a subset of the reference's box holding all of
 // its candidates within r, so results are unchanged.
+// The cells [xl, xh] that the grid's hash (cell_coord: the correctly
+// rounded quotient (p - mn) / cell, truncated and clamped to [0, g - 1];
+// monotone in p) gives the positions in [lo, hi]: the quotients are taken
+// with the hardware reciprocal, within 2^-21 relative of the divided ones,
+// moved out by a relative and an absolute 2^-20, floored and clamped like
+// the hash, so every such position's cell is inside (a bound can only come
+// out one cell wider).
+__device__ __forceinline__ void cell_bounds(float lo, float hi, float mn, float rcell, int g, int& xl, int& xh) {
+    float tl = (lo - mn) * rcell, th = (hi - mn) * rcell;
+    tl = tl - (fabsf(tl) * 0x1p-20f + 0x1p-20f);
+    th = th + (fabsf(th) * 0x1p-20f + 0x1p-20f);
+    const float top = (float)(g - 1);
+    xl = (int)floorf(fminf(fmaxf(tl, 0.f), top));  // (NaN gives 0 and g - 1)
+    xh = (int)floorf(fmaxf(fminf(th, top), 0.f));
+}
+
+// q = s / d and r = s % d for s < 2^20 (and d >= 1): (s + 0.5) / d is at
+// least 0.5 / d from the integers around it, and the product with the
+// hardware reciprocal is within (s + 0.5) 2^-22 / d of it.
+__device__ __forceinline__ void small_divmod(int s, int d, float rd, int& q, int& r) {
+    q = (int)(((float)s + 0.5f) * rd);
+    r = s - q * d;
+}
+
 __device__ __forceinline__ bool row_span(const GridGeom& G, const uint32_t* __restrict__ starts,
                                          const uint32_t* __restrict__ sizes, const Box& B, int s, float qx, float qy,
                                          float qz, float T, uint32_t& b, uint32_t& len) {
-    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     const float thr = T * 1.001f;
     b = 0;
     len = 0;
     if (G.layout == PM_LAYOUT_LINEAR) {
-        const int y = B.lo[1] + s % ny, z = B.lo[2] + s / ny;
+        int zi, yi;
+        if (ny * nz < (1 << 20))
+            small_divmod(s, ny, __builtin_amdgcn_rcpf((float)ny), zi, yi);
+        else
+            zi = s / ny, yi = s % ny;
+        const int y = B.lo[1] + yi, z = B.lo[2] + zi;
         const float gy = axis_gap(qy, y, G.g[1], G.mn[1], G.cell), gz = axis_gap(qz, z, G.g[2], G.mn[2], G.cell);
         const float dyz2 = gy * gy + gz * gz;
         if (!(dyz2 <= thr)) return false;
         // (the hardware square root: its ulp is far inside the margins)
         const float dx = __builtin_amdgcn_sqrtf(thr - dyz2 + 1e-3f * T);
         const float e = (fabsf(qx) + dx) * 1e-6f;
-        const int xl = max(B.lo[0], pmd::cell_coord(qx - dx - e, G.mn[0], G.cell, G.g[0]));
-        const int xh = min(B.hi[0], pmd::cell_coord(qx + dx + e, G.mn[0], G.cell, G.g[0]));
+        int xl, xh;
+        cell_bounds(qx - dx - e, qx + dx + e, G.mn[0], __builtin_amdgcn_rcpf(G.cell), G.g[0], xl, xh);
+        xl = max(B.lo[0], xl);
+        xh = min(B.hi[0], xh);
         if (xl > xh) return false;
         const uint32_t c0 = pmd::cell_index(G, xl, y, z);
         const uint32_t c1 = c0 + (uint32_t)(xh - xl);
