@@ -175,21 +175,26 @@ __device__ uint32_t buf_select(WaveLds<CAP>& L, uint32_t lane, uint32_t cnt, uin
 }
 
 // buf_select to exactly the k smallest keys, for at most 64 R buffered keys
-// read into registers once (v[j] = entry 64 j + lane, ~0 past cnt): the same
-// bins and rounds, counted from the registers instead of re-reading the
-// buffer, so the bound and the kept set are those of buf_select. The kept
+// read into registers once (v[j] = entry 64 j + lane, ~0 past cnt): rounds
+// of bins like buf_select's (finer in the first), counted from the
+// registers instead of re-reading the buffer; the kept set is the same (the
+// k smallest keys, which are unique). The kept
 // keys are then written back to the buffer's front in buffer order (one
 // pass, every read already done). Returns the count kept (min(cnt, k)).
 template <int R, int CAP>
 __device__ uint32_t reg_select(WaveLds<CAP>& L, uint32_t lane, const uint64_t (&v)[R], uint32_t cnt, uint32_t k,
                                uint32_t bT, uint64_t& khi_io) {
     if (cnt <= k) return cnt;
+    // the first round bins four per octave (the top two mantissa bits of
+    // d2; 2 bT + 1 bounds bits(T) >> 21), so the crossing bin holds fewer
+    // keys for the rounds that split it
+    const uint32_t bT2 = (bT << 1) | 1u;
     L.hist[lane] = 0;
     wave_sync();
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         if ((uint32_t)j * 64 + lane < cnt) {
-            const uint32_t h = bT - (uint32_t)(v[j] >> 54);
+            const uint32_t h = bT2 - (uint32_t)(v[j] >> 53);
             atomicAdd(&L.hist[h < 63u ? h : 63u], 1u);
         }
     }
@@ -199,10 +204,10 @@ __device__ uint32_t reg_select(WaveLds<CAP>& L, uint32_t lane, const uint64_t (&
     const uint32_t hstar = 63u - (uint32_t)ls;
     uint32_t A = __builtin_amdgcn_readlane(cum, ls);
     const uint32_t prev = __builtin_amdgcn_readlane(cum, ls > 0 ? ls - 1 : 0);
-    uint64_t khi = (uint64_t)(bT - hstar + 1u) << 54, klo = 0;
+    uint64_t khi = (uint64_t)(bT2 - hstar + 1u) << 53, klo = 0;
     uint32_t below = 0;
     if (hstar < 63u) {
-        klo = (uint64_t)(bT - hstar) << 54;
+        klo = (uint64_t)(bT2 - hstar) << 53;
         below = ls > 0 ? prev : 0u;
     }
     if (khi > khi_io) khi = khi_io;
