@@ -626,11 +626,32 @@ __device__ __forceinline__ void query_setup(const KnnParams& P, QSetup& S, uint3
 // smallest of them is closer than every photon outside the inner box can
 // be, those k keys are the answer (m = k); otherwise the query walks the
 // whole box again from an empty buffer. Both outcomes are exact.
-// kArgs: P is the launch's own kernel argument (at offset 0 of the argument
-// segment), so the finish re-reads its parameters there (scalar loads)
-// instead of keeping them in SGPRs across the walk.
-template <int CAP, bool kArgs = false>
-__device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, const QSetup& S) {
+// The finish's parameters, re-read where P came from (scalar loads) instead
+// of kept in SGPRs across the walk: kFrom = kFromArgs when P is the launch's
+// own kernel argument (at offset 0 of the argument segment), kFromRecord
+// when P was loaded from the record src; kFromNone uses P.
+constexpr int kFromNone = 0, kFromArgs = 1, kFromRecord = 2;
+template <class Ptr>
+__device__ __forceinline__ void finish_params(KnnParams& Pf, Ptr kp) {
+    // (a pointer the compiler cannot see through, so the loads stay here;
+    // every field the finish reads is taken from it)
+    asm volatile("" : "+s"(kp));
+    Pf.k = kp->k;
+    Pf.n = kp->n;
+    Pf.nq = kp->nq;
+    Pf.bT = kp->bT;
+    Pf.power = kp->power;
+    Pf.dir = kp->dir;
+    Pf.nrm = kp->nrm;
+    Pf.rgb = kp->rgb;
+    Pf.out_idx = kp->out_idx;
+    Pf.out_dist = kp->out_dist;
+    Pf.out_count = kp->out_count;
+}
+
+template <int CAP, int kFrom = kFromNone>
+__device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, const QSetup& S,
+                                            const KnnParams* src = nullptr) {
     const uint32_t q = S.q;
     const float qx = S.x, qy = S.y, qz = S.z;
     const float T = P.T;
@@ -674,24 +695,13 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
         inner = false;
         wave_sync();
     }
-    if constexpr (kArgs) {
-        // (a pointer the compiler cannot see through, so the loads stay
-        // here; every field the finish reads is taken from it)
-        using ArgPtr = const __attribute__((address_space(4))) KnnParams*;
-        ArgPtr kp = (ArgPtr)__builtin_amdgcn_kernarg_segment_ptr();
-        asm volatile("" : "+s"(kp));
+    if constexpr (kFrom == kFromArgs) {
         KnnParams Pf = P;
-        Pf.k = kp->k;
-        Pf.n = kp->n;
-        Pf.nq = kp->nq;
-        Pf.bT = kp->bT;
-        Pf.power = kp->power;
-        Pf.dir = kp->dir;
-        Pf.nrm = kp->nrm;
-        Pf.rgb = kp->rgb;
-        Pf.out_idx = kp->out_idx;
-        Pf.out_dist = kp->out_dist;
-        Pf.out_count = kp->out_count;
+        finish_params(Pf, (const __attribute__((address_space(4))) KnnParams*)__builtin_amdgcn_kernarg_segment_ptr());
+        finish<CAP>(Pf, L, lane, q, cnt, found, khi);
+    } else if constexpr (kFrom == kFromRecord) {
+        KnnParams Pf = P;
+        finish_params(Pf, src);
         finish<CAP>(Pf, L, lane, q, cnt, found, khi);
     } else {
         finish<CAP>(P, L, lane, q, cnt, found, khi);
@@ -699,8 +709,9 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
 }
 
 // One query on the wave (its set-up wave-uniform).
-template <int CAP, bool kArgs = false>
-__device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q) {
+template <int CAP, int kFrom = kFromNone>
+__device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, uint32_t lane, uint32_t q,
+                                           const KnnParams* src = nullptr) {
     // the query is the wave's: its coordinates (and so the box, the row
     // count and the cursor of walk) are wave-uniform, held in SGPRs
     QSetup S;
@@ -709,7 +720,7 @@ __device__ __forceinline__ void stream_one(const KnnParams& P, WaveLds<CAP>& L, 
     S.y = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qy[q])));
     S.z = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.qz[q])));
     query_setup(P, S, lane);
-    stream_core<CAP, kArgs>(P, L, lane, S);
+    stream_core<CAP, kFrom>(P, L, lane, S, src);
 }
 
 // 512-key buffer: 17 KB of LDS per workgroup, so 7 waves per SIMD fit when
@@ -718,7 +729,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     __shared__ WaveLds<512> lds[kWaves];
     uint32_t q;
     if (!query_of_wave(P, q)) return;  // wave-uniform; no workgroup barriers below
-    stream_one<512, true>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
+    stream_one<512, kFromArgs>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
 }
 
 // 1024-key buffer: fewer refinements per query, LDS-limited to 4 waves per SIMD.
@@ -726,7 +737,7 @@ __global__ __launch_bounds__(256) void k_knn_stream1024(KnnParams P) {
     __shared__ WaveLds<1024> lds[kWaves];
     uint32_t q;
     if (!query_of_wave(P, q)) return;
-    stream_one<1024, true>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
+    stream_one<1024, kFromArgs>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, q);
 }
 
 // Large k (257..2048, e.g. the shipped renderer's k = 2000 / 400): one wave
@@ -1098,7 +1109,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     const uint32_t g = order ? order[p] : p;
     const uint32_t j = job_of(qbase, n_jobs, g);
     const KnnParams P = jobs[j];
-    stream_one<512>(P, lds[threadIdx.x >> 6], threadIdx.x & 63, __builtin_amdgcn_readfirstlane(g - qbase[j]));
+    stream_one<512, kFromRecord>(P, lds[threadIdx.x >> 6], threadIdx.x & 63,
+                                 __builtin_amdgcn_readfirstlane(g - qbase[j]), jobs + j);
 }
 
 // Persistent form: workgroups stay resident and their waves take tiles of
