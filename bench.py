@@ -671,6 +671,12 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
     seed_step = 0x9E3779B97F4A7C15
 
     servers = []
+    # every server gets the same admission budget, a share of the device's
+    # memory (the default, a share of the memory free at creation, depends on
+    # what earlier sections left allocated; admission counts each job's
+    # worst-case footprint, far above what its frame allocates)
+    budget = int(0.8 * torch.cuda.get_device_properties(ctx.device).total_memory)
+    steps_taken = {}
 
     def served(workers):
         # one server (and its worker contexts, with their block caches) for
@@ -683,7 +689,7 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
                     servers.pop()[1].close()
                 ctx.trim()
                 torch.cuda.empty_cache()
-                srv = ctx.server()
+                srv = ctx.server(budget)
                 if workers:
                     srv.set_workers(workers)
                 servers.append((workers, srv))
@@ -696,8 +702,11 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
                                   r_global=j["r"], k_caustic=64, r_caustic=j["r"],
                                   max_paths=3 * (j["ng"] + j["nc"]))
                 outs.append(o)
+            steps = 0
             while srv.info()["finished"] < done0 + len(jobs):
                 srv.step()
+                steps += 1
+            steps_taken[workers] = steps
             torch.cuda.synchronize()
             return [o[0] for o in outs]
         return fn
@@ -737,6 +746,8 @@ def frames_batch_bench(ctx, n_scenes=256, reps=2, seed=7):
         "photons_traced": out["plain_photons"],
         "gather_queries": out["queries"],
         "frames_identical": same,
+        "server_budget_bytes": budget,
+        "served_steps": {"served": steps_taken.get(0), "served_workers3": steps_taken.get(3)},
         "note": "batched = every scene's trace and grids, then one pm_gather_batch for every frame's gathers; "
                 "served = all 256 scenes as one-frame jobs of pm_server (one step: per-job trace, grids and eye "
                 "rays, one shared gather batch, per-frame fold), on the server's context or with three worker "
@@ -781,11 +792,11 @@ def serve_bench(ctx, rates=(10, 50, 200), n_jobs=50, seed=11,
             r = float(rng.uniform(0.01, 0.05))
             jobs.append(dict(at=t, scene=s, w=w, h=h, ng=ng, nc=ng // 20, frames=int(rng.integers(1, 31)), r=r,
                              seed=int(rng.integers(1, 1 << 62))))
-        # every configuration's server sees the same free memory (its
-        # default budget is a share of it)
+        # every configuration's server gets the same admission budget, a share
+        # of the device's memory (not of what earlier sections left free)
         ctx.trim()
         torch.cuda.empty_cache()
-        srv = ctx.server()
+        srv = ctx.server(int(0.8 * torch.cuda.get_device_properties(ctx.device).total_memory))
         if workers:
             srv.set_workers(workers)
         log(f"served: {rate} jobs/s, {cname}")
