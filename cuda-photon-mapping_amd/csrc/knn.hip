@@ -248,6 +248,11 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
     }
 }
 
+// Lanes [0, n) as a mask (n clamped to [0, 64]).
+__device__ __forceinline__ uint64_t lanes_below(int n) {
+    return n >= 64 ? ~0ull : n <= 0 ? 0ull : ((1ull << n) - 1ull);
+}
+
 // walk with 128-candidate steps over the runs laid end to end: a step is
 // the next 128 positions of the concatenated runs (lane l holds positions
 // p + l and p + 64 + l), so runs shorter than a step share one instead of
@@ -256,7 +261,9 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
 // b_j + t - pre_j: each lane carries the byte offset 4 (b_j - pre_j) of the
 // run holding its position, updated only for the runs that start inside the
 // step. The cursor (step start, next run, its start) is wave-uniform, in
-// SGPRs. f(va, sa, d2a, vb, sb, d2b).
+// SGPRs. f(am, sa, d2a, bm, sb, d2b): am / bm are the lane masks of the
+// lanes that hold a candidate (wave-uniform, SGPRs); d2 is computed on every
+// lane.
 template <class F>
 __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
                                       F&& f) {
@@ -279,7 +286,7 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
         // the first non-empty run starts the stream at 0
         uint32_t nstart = 0, cur = 0;
         struct Step {
-            bool va, vb;
+            int left;  // stream positions left from the step's first
             uint32_t oa, ob;
             float xa, ya, za, xb, yb, zb;
         };
@@ -297,9 +304,7 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
                 cur = dj;
                 nstart = rows ? __builtin_amdgcn_readlane(pre, __builtin_ctzll(rows)) : 0xffffffffu;
             }
-            const int left = (int)(total - p);
-            w.va = ilane < left;
-            w.vb = ilane < left - 64;
+            w.left = (int)(total - p);
             w.oa = da + lane4 + p * 4u;
             w.ob = db + lane4 + p * 4u + 256u;
             w.xa = bload(rx, w.oa), w.ya = bload(ry, w.oa), w.za = bload(rz, w.oa);
@@ -314,8 +319,8 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
             const uint32_t pn = p + 128u;
             const bool more = pn < total;
             if (more) load(nxt, pn);
-            f(c.va, c.oa >> 2, dist2(c.xa, c.ya, c.za, qx, qy, qz), c.vb, c.ob >> 2,
-              dist2(c.xb, c.yb, c.zb, qx, qy, qz));
+            f(lanes_below(c.left), c.oa >> 2, dist2(c.xa, c.ya, c.za, qx, qy, qz), lanes_below(c.left - 64),
+              c.ob >> 2, dist2(c.xb, c.yb, c.zb, qx, qy, qz));
             p = pn;
             return more;
         };
@@ -540,36 +545,32 @@ __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
 }
 
 // One step of the streaming selection over two candidates per lane (index
-// sa / sb, squared distance d2a / d2b, valid flags va / vb): counts the
-// candidates within the radius, appends the in-radius keys below the
-// acceptance bound khi to the LDS buffer, and narrows the buffer first when
-// the step would overflow it.
+// sa / sb, squared distance d2a / d2b, lanes holding one: masks am / bm):
+// counts the candidates within the radius, appends the in-radius keys below
+// the acceptance bound khi to the LDS buffer, and narrows the buffer first
+// when the step would overflow it. Every predicate is a ballot of a plain
+// compare combined in SGPRs; the writing lanes come back from their mask.
 template <int CAP>
-__device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k, uint32_t bT, float T, bool va,
-                                      uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b, uint32_t& cnt,
+__device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k, uint32_t bT, float T, uint64_t am,
+                                      uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b, uint32_t& cnt,
                                       uint32_t& found, uint64_t& khi) {
-    // each ballot is taken of a plain compare (no bool round trip): an
-    // invalid lane gets a threshold no d2 >= 0 (or NaN) passes
-    const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
-    const uint64_t ba = wballot(ina), bb = wballot(inb);
+    // (a NaN d2 passes no compare; lanes past the step are masked)
+    const uint64_t ba = wballot(d2a <= T) & am, bb = wballot(d2b <= T) & bm;
     found += (uint32_t)(__popcll(ba) + __popcll(bb));
     const uint64_t ka = make_key(d2a, sa), kb = make_key(d2b, sb);
-    bool la = ka < khi, lb = kb < khi;
-    uint64_t bala = ba & wballot(la), balb = bb & wballot(lb);
+    uint64_t bala = ba & wballot(ka < khi), balb = bb & wballot(kb < khi);
     uint32_t na = (uint32_t)__popcll(bala), nb = (uint32_t)__popcll(balb);
     if (cnt + na + nb > (uint32_t)CAP) {
         // leaves at most CAP / 2 keys, room for this step's 128
         wave_sync();
         cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, bT, khi);
-        la = ka < khi;
-        lb = kb < khi;
-        bala = ba & wballot(la);
-        balb = bb & wballot(lb);
+        bala = ba & wballot(ka < khi);
+        balb = bb & wballot(kb < khi);
         na = (uint32_t)__popcll(bala);
         nb = (uint32_t)__popcll(balb);
     }
-    if (ina && la) L.buf[cnt + mbcnt(bala)] = ka;
-    if (inb && lb) L.buf[cnt + na + mbcnt(balb)] = kb;
+    if (__builtin_amdgcn_inverse_ballot_w64(bala)) L.buf[cnt + mbcnt(bala)] = ka;
+    if (__builtin_amdgcn_inverse_ballot_w64(balb)) L.buf[cnt + na + mbcnt(balb)] = kb;
     cnt += na + nb;
 }
 
@@ -682,8 +683,8 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
         cnt = 0;
         found = 0;
         khi = inner ? (uint64_t)gbits << 32 : ~0ull;
-        walk2(P, lane, W, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
-            push2<CAP>(L, lane, k, P.bT, T, va, sa, d2a, vb, sb, d2b, cnt, found, khi);
+        walk2(P, lane, W, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+            push2<CAP>(L, lane, k, P.bT, T, am, sa, d2a, bm, sb, d2b, cnt, found, khi);
         });
         wave_sync();
         // with at least k such keys, the k smallest keys overall are among
@@ -904,7 +905,8 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
     wave_sync();
     uint32_t found = 0;
     float dmax = 0.f;
-    walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t, float d2a, bool vb, uint32_t, float d2b) {
+    walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t, float d2a, uint64_t bm, uint32_t, float d2b) {
+        const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
         const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
         found += (uint32_t)(__popcll(wballot(ina)) + __popcll(wballot(inb)));
         if (ina) {
@@ -955,7 +957,8 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
             return;
         }
         uint32_t cnt = 0;
-        walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
+        walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+            const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
             const bool ina = d2a <= (va ? T : -1.0f) && wide_bin(d2a, bT20) == bstar;
             const bool inb = d2b <= (vb ? T : -1.0f) && wide_bin(d2b, bT20) == bstar;
             const uint64_t ba = wballot(ina), bb = wballot(inb);
@@ -1036,7 +1039,8 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
         };
         uint32_t qn = 0;
         wave_sync();  // the buffer's previous contents are dead
-        walk2(P, lane, B, qx, qy, qz, [&](bool va, uint32_t sa, float d2a, bool vb, uint32_t sb, float d2b) {
+        walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+            const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
             const bool sela = d2a <= (va ? T : -1.0f) && make_key(d2a, sa) <= kstar;
             const bool selb = d2b <= (vb ? T : -1.0f) && make_key(d2b, sb) <= kstar;
             const uint64_t ba = wballot(sela), bbal = wballot(selb);
