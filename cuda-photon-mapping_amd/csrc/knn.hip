@@ -248,11 +248,6 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
     }
 }
 
-// Lanes [0, n) as a mask (n clamped to [0, 64]).
-__device__ __forceinline__ uint64_t lanes_below(int n) {
-    return n >= 64 ? ~0ull : n <= 0 ? 0ull : ((1ull << n) - 1ull);
-}
-
 // walk with 128-candidate steps over the runs laid end to end: a step is
 // the next 128 positions of the concatenated runs (lane l holds positions
 // p + l and p + 64 + l), so runs shorter than a step share one instead of
@@ -319,7 +314,8 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
             const uint32_t pn = p + 128u;
             const bool more = pn < total;
             if (more) load(nxt, pn);
-            f(lanes_below(c.left), c.oa >> 2, dist2(c.xa, c.ya, c.za, qx, qy, qz), lanes_below(c.left - 64),
+            // (each mask is one lane compare's ballot)
+            f(wballot(ilane < c.left), c.oa >> 2, dist2(c.xa, c.ya, c.za, qx, qy, qz), wballot(ilane + 64 < c.left),
               c.ob >> 2, dist2(c.xb, c.yb, c.zb, qx, qy, qz));
             p = pn;
             return more;
