@@ -258,10 +258,11 @@ __device__ __forceinline__ void walk(const KnnParams& P, uint32_t lane, const Bo
 // step. The cursor (step start, next run, its start) is wave-uniform, in
 // SGPRs. f(am, sa, d2a, bm, sb, d2b): am / bm are the lane masks of the
 // lanes that hold a candidate (wave-uniform, SGPRs); d2 is computed on every
-// lane.
+// lane. Rows and their cells are pruned against Trow (T, or a smaller bound
+// when only keys below it can be kept).
 template <class F>
 __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const Box& B, float qx, float qy, float qz,
-                                      F&& f) {
+                                      float Trow, F&& f) {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     const bool linear = P.G.layout == PM_LAYOUT_LINEAR;
     const int nseg = linear ? ny * nz : nx * ny * nz;
@@ -271,7 +272,7 @@ __device__ __forceinline__ void walk2(const KnnParams& P, uint32_t lane, const B
     for (int sbase = 0; sbase < nseg; sbase += 64) {
         const int s = sbase + ilane;
         uint32_t b = 0, len = 0;
-        if (s < nseg) row_span(P.G, P.starts, P.sizes, B, s, qx, qy, qz, P.T, b, len);
+        if (s < nseg) row_span(P.G, P.starts, P.sizes, B, s, qx, qy, qz, Trow, b, len);
         const uint32_t incl = wave_incl_scan(len, lane);
         const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
         if (total == 0) continue;
@@ -679,9 +680,14 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
         cnt = 0;
         found = 0;
         khi = inner ? (uint64_t)gbits << 32 : ~0ull;
-        walk2(P, lane, W, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
-            push2<CAP>(L, lane, k, P.bT, T, am, sa, d2a, bm, sb, d2b, cnt, found, khi);
-        });
+        // (the inner walk keeps only keys below gbits: its rows are pruned
+        // against that bound; found then counts fewer candidates, which
+        // matters only when the whole box is walked again)
+        const float Trow = inner ? fminf(T, __uint_as_float(gbits)) : T;
+        walk2(P, lane, W, qx, qy, qz, Trow,
+              [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+                  push2<CAP>(L, lane, k, P.bT, T, am, sa, d2a, bm, sb, d2b, cnt, found, khi);
+              });
         wave_sync();
         // with at least k such keys, the k smallest keys overall are among
         // them (the buffer holds every in-radius key below its bound, or at
@@ -901,7 +907,7 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
     wave_sync();
     uint32_t found = 0;
     float dmax = 0.f;
-    walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t, float d2a, uint64_t bm, uint32_t, float d2b) {
+    walk2(P, lane, B, qx, qy, qz, T, [&](uint64_t am, uint32_t, float d2a, uint64_t bm, uint32_t, float d2b) {
         const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
         const bool ina = d2a <= (va ? T : -1.0f), inb = d2b <= (vb ? T : -1.0f);
         found += (uint32_t)(__popcll(wballot(ina)) + __popcll(wballot(inb)));
@@ -953,7 +959,7 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
             return;
         }
         uint32_t cnt = 0;
-        walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+        walk2(P, lane, B, qx, qy, qz, T, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
             const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
             const bool ina = d2a <= (va ? T : -1.0f) && wide_bin(d2a, bT20) == bstar;
             const bool inb = d2b <= (vb ? T : -1.0f) && wide_bin(d2b, bT20) == bstar;
@@ -1035,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_gather_wide(KnnParams P, uint32_t* __re
         };
         uint32_t qn = 0;
         wave_sync();  // the buffer's previous contents are dead
-        walk2(P, lane, B, qx, qy, qz, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
+        walk2(P, lane, B, qx, qy, qz, T, [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
             const bool va = __builtin_amdgcn_inverse_ballot_w64(am), vb = __builtin_amdgcn_inverse_ballot_w64(bm);
             const bool sela = d2a <= (va ? T : -1.0f) && make_key(d2a, sa) <= kstar;
             const bool selb = d2b <= (vb ? T : -1.0f) && make_key(d2b, sb) <= kstar;
