@@ -543,26 +543,25 @@ __device__ __forceinline__ bool query_of_wave(const KnnParams& P, uint32_t& q) {
 
 // One step of the streaming selection over two candidates per lane (index
 // sa / sb, squared distance d2a / d2b, lanes holding one: masks am / bm):
-// counts the candidates within the radius, appends the in-radius keys below
-// the acceptance bound khi to the LDS buffer, and narrows the buffer first
-// when the step would overflow it. Every predicate is a ballot of a plain
-// compare combined in SGPRs; the writing lanes come back from their mask.
+// appends the keys below the acceptance bound khi to the LDS buffer, and
+// narrows the buffer first when the step would overflow it. khi never
+// exceeds (bits(T) + 1) << 32, so a key below it is within the radius (d2
+// >= 0: its bits order as its value; a NaN d2 is above every bound). Every
+// predicate is a ballot of a plain compare combined in SGPRs; the writing
+// lanes come back from their mask.
 template <int CAP>
-__device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k, uint32_t bT, float T, uint64_t am,
+__device__ __forceinline__ void push2(WaveLds<CAP>& L, uint32_t lane, uint32_t k, uint32_t bT, uint64_t am,
                                       uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b, uint32_t& cnt,
-                                      uint32_t& found, uint64_t& khi) {
-    // (a NaN d2 passes no compare; lanes past the step are masked)
-    const uint64_t ba = wballot(d2a <= T) & am, bb = wballot(d2b <= T) & bm;
-    found += (uint32_t)(__popcll(ba) + __popcll(bb));
+                                      uint64_t& khi) {
     const uint64_t ka = make_key(d2a, sa), kb = make_key(d2b, sb);
-    uint64_t bala = ba & wballot(ka < khi), balb = bb & wballot(kb < khi);
+    uint64_t bala = am & wballot(ka < khi), balb = bm & wballot(kb < khi);
     uint32_t na = (uint32_t)__popcll(bala), nb = (uint32_t)__popcll(balb);
     if (cnt + na + nb > (uint32_t)CAP) {
         // leaves at most CAP / 2 keys, room for this step's 128
         wave_sync();
         cnt = buf_select<CAP>(L, lane, cnt, k, CAP / 2, bT, khi);
-        bala = ba & wballot(ka < khi);
-        balb = bb & wballot(kb < khi);
+        bala = am & wballot(ka < khi);
+        balb = bm & wballot(kb < khi);
         na = (uint32_t)__popcll(bala);
         nb = (uint32_t)__popcll(balb);
     }
@@ -672,21 +671,20 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
             W.hi[a] = min(P.G.g[a] - 1, S.c[a] + P.h);
         }
     }
-    uint32_t cnt, found;
+    uint32_t cnt;
     uint64_t khi;
     for (;;) {
-        // in the inner box only keys closer than every photon outside it are
-        // buffered: the acceptance bound starts at gbits
+        // the acceptance bound starts at the radius (d2 <= T), and in the
+        // inner box at gbits: only keys closer than every photon outside it
+        // are buffered there
         cnt = 0;
-        found = 0;
-        khi = inner ? (uint64_t)gbits << 32 : ~0ull;
+        khi = inner ? (uint64_t)gbits << 32 : (uint64_t)(__float_as_uint(T) + 1u) << 32;
         // (the inner walk keeps only keys below gbits: its rows are pruned
-        // against that bound; found then counts fewer candidates, which
-        // matters only when the whole box is walked again)
+        // against that bound)
         const float Trow = inner ? fminf(T, __uint_as_float(gbits)) : T;
         walk2(P, lane, W, qx, qy, qz, Trow,
               [&](uint64_t am, uint32_t sa, float d2a, uint64_t bm, uint32_t sb, float d2b) {
-                  push2<CAP>(L, lane, k, P.bT, T, am, sa, d2a, bm, sb, d2b, cnt, found, khi);
+                  push2<CAP>(L, lane, k, P.bT, am, sa, d2a, bm, sb, d2b, cnt, khi);
               });
         wave_sync();
         // with at least k such keys, the k smallest keys overall are among
@@ -698,6 +696,11 @@ __device__ __forceinline__ void stream_core(const KnnParams& P, WaveLds<CAP>& L,
         inner = false;
         wave_sync();
     }
+    // The count within the radius that the finish needs is min(found, k),
+    // and that is min(cnt, k): without a narrowing the buffer holds every
+    // key within the radius (cnt = found), and a narrowing keeps at least k
+    // (it starts above CAP - 128 >= k keys), so cnt >= k after one.
+    const uint32_t found = cnt;
     if constexpr (kFrom == kFromArgs) {
         KnnParams Pf = P;
         finish_params(Pf, (const __attribute__((address_space(4))) KnnParams*)__builtin_amdgcn_kernarg_segment_ptr());
