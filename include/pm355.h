@@ -501,7 +501,12 @@ typedef struct {
     uint64_t bytes_in_use;    /* admitted jobs' footprint */
 } pm_server_info;
 
-/* budget_bytes = 0: 80% of the device memory free at creation */
+/* budget_bytes = 0: 80% of the device memory free at creation (which
+ * depends on what the process holds then; pass a fixed budget for an
+ * admission that does not). Admission counts each job's worst-case
+ * footprint: its photon maps and grid copies (76 B per photon) plus a frame
+ * of 2^min(max_depth, 6) - 1 ray-tree slots of 100 B per pixel and 16 B per
+ * pixel, plus 64 MiB; a frame allocates far less (its hits only). */
 pm_status pm_server_create(pm_ctx* ctx, size_t budget_bytes, pm_server** out);
 pm_status pm_server_destroy(pm_server* server);
 pm_status pm_server_submit(pm_server* server, const pm_job_desc* job, uint64_t* job_id);
