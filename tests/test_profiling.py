@@ -59,3 +59,27 @@ def test_valu_summary(tmp_path):
                             ("k_trace_emit", "SQ_INSTS_VALU_TRANS_F32", 2)])
     s = profiling.summarize_valu([str(tmp_path / "v")], 2, {}, str(tmp_path / "v.json"))
     assert s["fp32_flop_per_trace"] == 64 * (10 + 1 + 2 + 2 * 5) / 2
+
+
+def _bench_module():
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_bench_instruction_and_lane_figures():
+    """bench.py's roofline figures from a counter summary: instructions per
+    query (one wave per query) and active lanes per VALU instruction."""
+    b = _bench_module()
+    cs = {"per_dispatch_median": {"SQ_WAVES": 1000.0, "SQ_INSTS_VALU": 600_000.0, "SQ_INSTS_SALU": 500_000.0,
+                                  "SQ_INSTS_LDS": 23_000.0, "SQ_INSTS_VMEM_RD": 32_000.0, "SQ_INSTS_SMEM": 21_000.0,
+                                  "SQ_ACTIVE_INST_VALU": 625_000.0, "SQ_THREAD_CYCLES_VALU": 28_000_000.0}}
+    ipq = b.insts_per_query(cs)
+    assert ipq == {"valu": 600.0, "salu": 500.0, "lds": 23.0, "vmem_rd": 32.0, "smem": 21.0}
+    assert abs(b.valu_lane_util(cs) - 28_000_000.0 / (64 * 625_000.0)) < 1e-12
+    # missing counters give None, not a made-up figure
+    assert b.insts_per_query(None) is None and b.insts_per_query({"per_dispatch_median": {}}) is None
+    assert b.valu_lane_util({"per_dispatch_median": {"SQ_WAVES": 1.0}}) is None
